@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Forward + adjoint RHS evaluations per second of the NeuralODE hot path.
+
+Workload (the headline configuration): state D=64, MLP 64-1024-1024-1024-64
+with softplus, B=4096 trajectories u0 ~ N(0,1), tspan (0,1) with 32 saveat
+points, Tsit5 rtol=1e-4 (atol 1e-6), MSE loss against the saveat states of a
+random linear-plus-tanh teacher ODE, gradient by InterpolatingAdjoint.  GEMMs
+and VJPs in BF16 on MFMA (FP32 accumulation); state, stage sums and adjoint
+state in FP32; time / controller / dL/dp quadrature in FP64.
+
+One "step" = one forward solve + loss cotangent + one adjoint solve, all on
+inputs resident in HBM.  ``value`` = batched RHS evaluations (forward RHS +
+adjoint VJP, each over all B trajectories) per second over the timed steps.
+
+Also reported: the same rate for the FP32 GPU path (the 5x comparison), the
+dominant GEMM's roofline from HIP events on the library's stream, and
+(``cpu_baseline`` leg, rank 0 only) the oracle restated in numpy FP32 timed on
+a bounded sample, plus the BF16/FP32 gradient error against the FP64 oracle on
+that sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "diffeqflux.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = ("fwd+adjoint RHS evals/sec, D=64 H=1024 B=4096; BF16-MFMA vs FP32 at grad rel err")
+PEAK = {"bf16": 2500.0, "fp32": 157.3}   # MI355X dense TFLOP/s
+ROLE_NAMES = {0: "fwd_hidden", 1: "fwd_stage", 2: "adj_hidden", 3: "bwd_data", 4: "wgrad",
+              5: "bwd_last"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------- workload
+D, H, NH = 64, 1024, 3
+WIDTHS = [D] + [H] * NH + [D]
+ACTS = ["softplus"] * NH + ["identity"]
+TSPAN = (0.0, 1.0)
+NSAVE = 32
+RTOL, ATOL = 1e-4, 1e-6
+
+
+def glorot_params(seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, o in zip(WIDTHS[:-1], WIDTHS[1:]):
+        lim = math.sqrt(6.0 / (i + o))
+        out += [rng.uniform(-lim, lim, i * o), np.zeros(o)]
+    return np.concatenate(out).astype(np.float32)
+
+
+def teacher_targets(u0, ts, seed=1, substeps=8):
+    """Saveat states of u' = A u + tanh(C u), A = -I + 0.5 G/sqrt(D),
+    C = G'/sqrt(D), by classical RK4 (synthetic targets only)."""
+    rng = np.random.default_rng(seed)
+    A = -np.eye(D) + 0.5 * rng.standard_normal((D, D)) / math.sqrt(D)
+    C = rng.standard_normal((D, D)) / math.sqrt(D)
+    f = lambda u: u @ A.T + np.tanh(u @ C.T)
+    u = u0.astype(np.float64)
+    out = [u.copy()]
+    for a, b in zip(ts[:-1], ts[1:]):
+        h = (b - a) / substeps
+        for _ in range(substeps):
+            k1 = f(u); k2 = f(u + h / 2 * k1); k3 = f(u + h / 2 * k2); k4 = f(u + h * k3)
+            u = u + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        out.append(u.copy())
+    return np.stack(out).astype(np.float32)
+
+
+def make_inputs(B, seed=1):
+    u0 = np.random.default_rng(seed).standard_normal((B, D)).astype(np.float32)
+    ts = np.linspace(TSPAN[0], TSPAN[1], NSAVE)
+    return u0, ts, teacher_targets(u0, ts)
+
+
+def flops_per_launch(role, layer, B):
+    i, o = WIDTHS[layer], WIDTHS[layer + 1]
+    return 2.0 * B * i * o
+
+
+# ---------------------------------------------------------------- GPU legs
+class Runner:
+    def __init__(self, precision, B, p, u0, ts, targets):
+        import torch
+        from diffeqflux_hip import Handle
+        self.torch = torch
+        self.h = Handle(WIDTHS, ACTS, 0, precision, B)
+        self.h.set_params(p)
+        self.opts = Handle.make_opts(TSPAN[0], TSPAN[1], abstol=ATOL, reltol=RTOL)
+        self.interior = list(ts[1:-1])
+        self.u0 = torch.from_numpy(u0).cuda()
+        self.tgt = torch.from_numpy(targets).cuda()
+        self.B = B
+
+    def step(self):
+        out, _, sf = self.h.forward(self.u0, self.opts, self.interior)
+        cot = (2.0 / out.numel()) * (out - self.tgt)
+        self.torch.cuda.current_stream().synchronize()
+        du0, dp, sa = self.h.backward(cot)
+        return sf, sa, du0, dp
+
+    def timed(self, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nf = nfa = 0
+        last = None
+        for _ in range(steps):
+            last = self.step()
+            nf += last[0]["nf"]
+            nfa += last[1]["nf"]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return dt, nf, nfa, last
+
+
+def profile_dominant(run):
+    """One profiled step with every GEMM timed: the (role, layer) with the
+    largest total time is the dominant kernel."""
+    run.h.prof_enable(None)
+    run.step()
+    recs = run.h.prof_collect()
+    run.h.prof_enable([])
+    tot = {}
+    for k, ms in recs:
+        tot[k] = tot.get(k, 0.0) + ms
+    dom = max(tot, key=tot.get)
+    return dom, tot
+
+
+def roofline_for(run, dom, precision, durations):
+    """achieved = algorithmic FLOPs per launch / average duration of the
+    launches that did work (launches gated off on the device -- FSAL stage
+    without re-evaluation, attempts enqueued after the end -- return at once
+    and are excluded by duration)."""
+    d = np.array(durations, dtype=np.float64)
+    if d.size == 0:
+        return None, {}
+    thr = 0.25 * np.percentile(d, 90)
+    real = d[d >= thr]
+    avg_ms = float(real.mean())
+    fl = flops_per_launch(dom[0], dom[1], run.B)
+    achieved = fl / (avg_ms * 1e-3) / 1e12
+    peak = PEAK[precision]
+    info = {"kernel": f"{ROLE_NAMES[dom[0]]}[layer {dom[1] + 1}]",
+            "launches": int(d.size), "launches_with_work": int(real.size),
+            "avg_ms": avg_ms, "flop_per_launch": fl}
+    return {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None}, info
+
+
+# ---------------------------------------------------------------- CPU leg
+def cpu_baseline_leg(p, Bs, gpu_grads):
+    """The oracle (numpy FP32 restatement of the reference algorithm) timed on
+    a bounded sample: one forward + adjoint solve at Bs trajectories (same D,
+    H, tolerances, saveat, loss).  Also the FP64 oracle gradient on the same
+    sample, against which the GPU gradients of that sample are checked."""
+    from oracle import adjoint as OA
+    from oracle.mlp import MLP
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:
+        cores = len(os.sched_getaffinity(0))
+    m = MLP(WIDTHS, ACTS)
+    u0, ts, tgt = make_inputs(Bs)
+
+    def solve(dtype):
+        pp, uu = p.astype(dtype), u0.astype(dtype)
+        f = OA.forward(m, pp, uu, TSPAN[0], TSPAN[1], saveat=ts, abstol=ATOL, reltol=RTOL)
+        pred = np.array(f.u)
+        cot = (2.0 / pred.size) * (pred - tgt.astype(dtype))
+        gu, gp, a = OA.backward(m, pp, f, list(cot), abstol=ATOL, reltol=RTOL)
+        return f, a, gu, gp
+
+    t0 = time.perf_counter()
+    f32, a32, _, _ = solve(np.float32)
+    cpu_s = time.perf_counter() - t0
+    evals = f32.nf + a32.nf
+    _, _, gu64, gp64 = solve(np.float64)
+    errs = {}
+    for name, (du0, dp) in gpu_grads.items():
+        errs[name] = float(np.linalg.norm(np.asarray(dp, np.float64) - gp64) / np.linalg.norm(gp64))
+        errs[name + "_du0"] = float(np.linalg.norm(np.asarray(du0, np.float64) - gu64)
+                                    / np.linalg.norm(gu64))
+    return {"value": round(evals / cpu_s * Bs / 4096.0, 3),
+            "unit": "RHS evals/s (full-batch equivalent)", "cores": int(cores), "kind": "port",
+            "sample": f"one FP32 forward+adjoint solve of the oracle at B={Bs} (same D/H/tol/"
+                      f"saveat/loss), {evals} batched evals in {cpu_s:.1f} s, rate scaled by "
+                      f"{Bs}/4096"}, errs
+
+
+# ---------------------------------------------------------------- main
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-batch", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    p = glorot_params()
+    u0, ts, tgt = make_inputs(args.batch)
+    log(f"[bench] {torch.cuda.get_device_name(0)} B={args.batch} {args.precision}")
+
+    run = Runner(args.precision, args.batch, p, u0, ts, tgt)
+    run.step()  # warm the allocator / code objects
+    dom, tot = profile_dominant(run)
+    log("[bench] per-role GEMM time (ms, one step):",
+        {f"{ROLE_NAMES[k[0]]}{k[1] + 1}": round(v, 2) for k, v in sorted(tot.items())})
+    run.h.prof_enable([dom])
+    dt, nf, nfa, last = run.timed(args.steps, args.warmup)
+    durs = [ms for _, ms in run.h.prof_collect()]
+    run.h.prof_enable([])
+    value = (nf + nfa) / dt
+    roof, rinfo = roofline_for(run, dom, args.precision, durs)
+    sf, sa = last[0], last[1]
+    log(f"[bench] {args.precision}: {value:.1f} evals/s, {1e3 * dt / args.steps:.2f} ms/step, "
+        f"fwd {sf['naccept']}+{sf['nreject']} steps nf {sf['nf']}, adj {sa['naccept']}+"
+        f"{sa['nreject']} steps nf {sa['nf']}; dominant {rinfo}")
+
+    extra = {}
+    if not args.no_fp32 and args.precision == "bf16":
+        r32 = Runner("fp32", args.batch, p, u0, ts, tgt)
+        dt32, nf32, nfa32, _ = r32.timed(max(2, args.steps // 5), 1)
+        v32 = (nf32 + nfa32) / dt32
+        extra["fp32_evals_per_s"] = round(v32, 2)
+        extra["speedup_vs_fp32"] = round(value / v32, 3)
+        log(f"[bench] fp32: {v32:.1f} evals/s -> bf16 speedup {value / v32:.2f}x")
+        del r32
+
+    cpu = None
+    if not args.no_cpu:
+        Bs = args.cpu_batch
+        us, tss, tgs = make_inputs(Bs)
+        grads = {}
+        for prec in ("bf16", "fp32"):
+            rs = Runner(prec, Bs, p, us, tss, tgs)
+            _, _, du0, dp = rs.step()
+            grads[prec] = (du0.cpu().numpy(), dp.cpu().numpy())
+        cpu, errs = cpu_baseline_leg(p, Bs, grads)
+        extra["grad_rel_err_bf16"] = round(errs["bf16"], 6)
+        extra["grad_rel_err_fp32"] = round(errs["fp32"], 8)
+        extra["grad_rel_err_du0_bf16"] = round(errs["bf16_du0"], 6)
+        extra["grad_rel_err_sample"] = f"||dL/dp - FP64 oracle|| / ||oracle||, B={Bs} sample"
+        log(f"[bench] cpu baseline {cpu}; grad rel err {errs}")
+
+    rec = {
+        "metric": METRIC, "value": round(value, 2), "unit": "RHS evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic (u0 ~ N(0,1), teacher-ODE targets, Glorot weights)",
+        "config": {"workload": "NeuralODE fwd+InterpolatingAdjoint, D=64, MLP "
+                               "64-1024-1024-1024-64 softplus, B=4096, tspan (0,1), 32 saveat, "
+                               "Tsit5 rtol=1e-4 atol=1e-6, MSE loss",
+                   "batch": args.batch, "precision": args.precision},
+        "traj_per_s": round(args.batch * args.steps / dt, 2),
+        "nf_fwd": sf["nf"], "nf_adj": sa["nf"],
+        "steps_fwd": f"{sf['naccept']}+{sf['nreject']}",
+        "steps_adj": f"{sa['naccept']}+{sa['nreject']}",
+        "roofline": roof, "roofline_kernel": rinfo, "cpu_baseline": cpu,
+    }
+    rec.update(extra)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1136 @@
+// Device kernels of the NeuralODE hot path for MI355X (gfx950 / CDNA4).
+//
+// * gemm_tn: one LDS-tiled MFMA GEMM template (BF16 v_mfma_f32_32x32x16_bf16
+//   or FP32 v_mfma_f32_32x32x2f32, FP32 accumulation) with the Dense-layer
+//   epilogues fused in: bias + activation (+ act', + transposed copy), the
+//   Tsit5 stage combination and embedded-error partial sums, the adjoint
+//   stage combination with the forward dense-output interpolation, and the
+//   parameter-gradient quadrature fold.
+// * single-workgroup controllers: FP64 PI step-size control, initial-dt
+//   heuristic, tstop landing, saveat bookkeeping, retcodes.
+// * elementwise kernels: solve initialisation, step begin (saves, FSAL,
+//   cotangent jumps, mu commit), mu error norm, bias-gradient fold.
+//
+// Layouts (all device buffers are allocated zero-filled; padded regions are
+// never written with non-zeros):
+//   state vectors      FP32 [Bp][D]            (Julia D x B column-major bytes)
+//   GEMM activations   T    [Bp][Fp]           (K contiguous for the next layer)
+//   transposed copies  T    [Fp][Bp]           (batch contiguous: K of weight grads)
+//   weights            T    Wr[out_p][in_p] (forward), Wc[in_p][out_p] (backward)
+// with Bp = B rounded up to 128 and Fp = F rounded up to 64.
+#include "node_internal.h"
+
+namespace nodehip {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__constant__ float cA[7][7];
+__constant__ float cBT[7];
+
+static const double kA[7][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365,
+     -0.09249506636175525, 0, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159,
+     -0.071584973281401, -0.028269050394068383, 0, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742,
+     -3.290069515436081, 2.324710524099774, 0}};
+static const double kBT[7] = {-0.00178001105222577714, -0.0008164344596567469,
+                              0.007880878010261995,    -0.1447110071732629,
+                              0.5823571654525552,      -0.45808210592918697,
+                              0.015151515151515152};
+__device__ const double dC[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+// free 4th-order dense output: b1(t) = t(r11 + t(r12 + t(r13 + t r14))),
+// bi(t) = t^2 (ri2 + t(ri3 + t ri4))
+__device__ const double dR1[4] = {1.0, -2.763706197274826, 2.9132554618219126,
+                                  -1.0530884977290216};
+__device__ const double dRI[6][3] = {
+    {0.13169999999999998, -0.2234, 0.1017},
+    {3.9302962368947516, -5.941033872131505, 2.490627285651253},
+    {-12.411077166933676, 30.33818863028232, -16.548102889244902},
+    {37.50931341651104, -88.1789048947664, 47.37952196281928},
+    {-27.896526289197286, 65.09189467479366, -34.8706578614966},
+    {1.5, -4.0, 2.5}};
+
+void upload_tableau() {
+  float a[7][7], bt[7];
+  for (int i = 0; i < 7; ++i) {
+    for (int j = 0; j < 7; ++j) a[i][j] = (float)kA[i][j];
+    bt[i] = (float)kBT[i];
+  }
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(cA), a, sizeof(a));
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(cBT), bt, sizeof(bt));
+}
+
+// PI controller constants (OrdinaryDiffEq defaults for Tsit5)
+#define BETA1 0.14
+#define BETA2 0.08
+#define GAMMA 0.9
+#define QMIN 0.2
+#define QMAX 10.0
+#define QOLDINIT 1e-4
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ bool gate_ok(const Ctl* c, int gate) {
+  if (gate == GATE_ALWAYS) return true;
+  if (c->done) return false;
+  if (gate == GATE_FSAL) return c->fsal_reeval != 0;
+  return true;
+}
+
+__device__ __forceinline__ float act_f(int a, float z) {
+  switch (a) {
+    case 1: return tanhf(z);
+    case 2: return log1pf(expf(-fabsf(z))) + fmaxf(z, 0.f);
+    case 3: return fmaxf(z, 0.f);
+    case 4: return 1.f / (1.f + expf(-z));
+    default: return z;
+  }
+}
+__device__ __forceinline__ float act_d(int a, float z, float y) {
+  switch (a) {
+    case 1: return 1.f - y * y;
+    case 2: return 1.f / (1.f + expf(-z));
+    case 3: return z > 0.f ? 1.f : 0.f;
+    case 4: return y * (1.f - y);
+    default: return 1.f;
+  }
+}
+__device__ __forceinline__ float map_f(int m, float u) { return m == 1 ? u * u * u : u; }
+
+template <typename T> __device__ __forceinline__ T cvt(float x);
+template <> __device__ __forceinline__ float cvt<float>(float x) { return x; }
+template <> __device__ __forceinline__ bf16 cvt<bf16>(float x) { return (bf16)x; }
+template <typename T> __device__ __forceinline__ float tof(T x);
+template <> __device__ __forceinline__ float tof<float>(float x) { return x; }
+template <> __device__ __forceinline__ float tof<bf16>(bf16 x) { return (float)x; }
+
+// store 4 consecutive values (8 B for bf16, 16 B for fp32; caller guarantees alignment)
+__device__ __forceinline__ void store4(float* p, const float v[4]) {
+  *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void store4(bf16* p, const float v[4]) {
+  typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  bf16x4 o = {(bf16)v[0], (bf16)v[1], (bf16)v[2], (bf16)v[3]};
+  *(bf16x4*)p = o;
+}
+
+// Block-wide reduction of (sum, max-flag); thread 0 writes part[2*slot..].
+__device__ void block_reduce_store(double s, double f, double* part, int slot) {
+  __shared__ double red[2][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off);
+    f = fmax(f, __shfl_xor(f, off));
+  }
+  if (lane == 0) { red[0][wave] = s; red[1][wave] = f; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ss = 0.0, ff = 0.0;
+    for (int w = 0; w < nw; ++w) { ss += red[0][w]; ff = fmax(ff, red[1][w]); }
+    part[2 * slot] = ss;
+    part[2 * slot + 1] = ff;
+  }
+}
+
+__device__ __forceinline__ double dspacing(double t) {
+  double a = fabs(t);
+  return nextafter(a, INFINITY) - a;
+}
+
+// ---------------------------------------------------------------- MFMA
+template <typename T> struct Mma;
+template <> struct Mma<bf16> {
+  static __device__ __forceinline__ void run(f32x16& c, const uint4& a, const uint4& b) {
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  }
+};
+// FP32: 4 MFMA 32x32x2 per 16-B fragment.  Lane l supplies k = 4*(l/32) + j to
+// step j for both operands, so the four steps cover the 8 k of the chunk.
+template <> struct Mma<float> {
+  static __device__ __forceinline__ void run(f32x16& c, const uint4& a, const uint4& b) {
+    const float4 fa = __builtin_bit_cast(float4, a), fb = __builtin_bit_cast(float4, b);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa.x, fb.x, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa.y, fb.y, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa.z, fb.z, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa.w, fb.w, c, 0, 0, 0);
+  }
+};
+
+// Accumulator layout of a 32x32 MFMA tile: lane l, register v holds
+// row 8*(v/4) + 4*(l/32) + v%4, column l%32.
+#define EPI_LOOP_BEGIN                                                     \
+  _Pragma("unroll") for (int mi = 0; mi < MR; ++mi)                       \
+  _Pragma("unroll") for (int mj = 0; mj < MC; ++mj) {                     \
+    const int c = cb + mj * 32 + (lane & 31);                              \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                        \
+      const int r0 = rb + mi * 32 + q * 8 + (lane >> 5) * 4;
+#define EPI_LOOP_END }}
+
+// ---------------------------------------------------------------- epilogues
+template <typename T, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiHidden& e, f32x16 (&acc)[MR][MC], int rb,
+                                          int cb, int lane, const Ctl*) {
+  T* Y = (T*)e.Y;
+  T* YT = (T*)e.YT;
+  T* S = (T*)e.S;
+  EPI_LOOP_BEGIN
+    const bool cv = c < e.C;
+    const float bias = cv ? e.bias[c] : 0.f;
+    float yv[4], dv[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const float z = acc[mi][mj][q * 4 + p] + bias;
+      const float y = act_f(e.act, z);
+      yv[p] = cv ? y : 0.f;
+      dv[p] = cv ? act_d(e.act, z, y) : 0.f;
+      Y[(int64_t)(r0 + p) * e.ldo + c] = cvt<T>(yv[p]);
+      if (S) S[(int64_t)(r0 + p) * e.ldo + c] = cvt<T>(dv[p]);
+    }
+    if (YT) store4(YT + (int64_t)c * e.ldt + r0, yv);
+  EPI_LOOP_END
+}
+
+template <typename T, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC], int rb,
+                                          int cb, int lane, const Ctl*) {
+  const T* S = (const T*)e.S;
+  T* Dl = (T*)e.Dl;
+  T* DlT = (T*)e.DlT;
+  EPI_LOOP_BEGIN
+    const bool cv = c < e.C;
+    float dv[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
+      dv[p] = cv ? acc[mi][mj][q * 4 + p] * tof<T>(S[o]) : 0.f;
+      Dl[o] = cvt<T>(dv[p]);
+    }
+    store4(DlT + (int64_t)c * e.ldt + r0, dv);
+  EPI_LOOP_END
+}
+
+template <typename T, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR][MC], int rb,
+                                          int cb, int lane, const Ctl* ctl) {
+  const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
+  float* ent = e.tape + n * e.ent;
+  const float dt = (float)ctl->dt;
+  const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
+  const int s = e.stage;
+  T* X0 = (T*)e.X0;
+  double sum = 0.0, flag = 0.0;
+  EPI_LOOP_BEGIN
+    if (c < e.D) {
+      const float bias = e.bias[c];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p;
+        const int64_t idx = (int64_t)r * e.D + c;
+        const float k = act_f(e.act, acc[mi][mj][q * 4 + p] + bias);
+        const bool live = r < e.B;
+        if (e.mode == SM_STEP) {
+          ent[(1 + s) * e.sv + idx] = k;
+          if (s < 6) {
+            float sm = cA[s + 1][s] * k;
+            for (int j = 0; j < s; ++j) sm += cA[s + 1][j] * ent[(1 + j) * e.sv + idx];
+            const float u = ent[idx] + dt * sm;
+            X0[(int64_t)r * e.ldx + c] = cvt<T>(map_f(e.map, u));
+            if (s == 5) ent[e.ent + idx] = u;            // candidate tape[n+1].uprev
+          } else {
+            ent[e.ent + e.sv + idx] = k;                  // FSAL: tape[n+1].k1
+            float ut = cBT[6] * k;
+            for (int j = 0; j < 6; ++j) ut += cBT[j] * ent[(1 + j) * e.sv + idx];
+            ut *= dt;
+            const float up = ent[idx], un = ent[e.ent + idx];
+            const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
+            if (live) {
+              const double a = (double)(ut / sk);
+              sum += a * a;
+              if (isnan(un)) flag = 1.0;
+            }
+          }
+        } else if (e.mode == SM_INIT_F0) {
+          ent[e.sv + idx] = k;
+          const float u0 = ent[idx];
+          const float sk = atol + fabsf(u0) * rtol;
+          if (live) { const double a = (double)(k / sk); sum += a * a; }
+        } else if (e.mode == SM_INIT_F1) {
+          e.scratch[idx] = k;
+          const float u0 = ent[idx], f0 = ent[e.sv + idx];
+          const float sk = atol + fabsf(u0) * rtol;
+          if (live) {
+            const double a = (double)((k - f0) / sk);
+            sum += a * a;
+            if (k != f0) flag = 1.0;
+          }
+        } else {
+          e.scratch[idx] = k;
+        }
+      }
+    }
+  EPI_LOOP_END
+  if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
+}
+
+template <typename T, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR][MC], int rb,
+                                          int cb, int lane, const Ctl* ctl) {
+  const float dt = (float)ctl->dt;
+  const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
+  const int s = e.stage;
+  T* L = (T*)e.L;
+  T* LT = (T*)e.LT;
+  T* X0 = (T*)e.X0;
+  T* X0T = (T*)e.X0T;
+  double sum = 0.0, flag = 0.0;
+  // interpolation data of the next stage's time
+  const int slot = s + 1;
+  int tj = 0, ex = 1;
+  float tdt = 0.f, w[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (e.mode == SM_STEP && s < 6) {
+    tj = ctl->st_step[slot];
+    ex = ctl->st_exact[slot];
+    tdt = ctl->st_dt[slot];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) w[i] = ctl->st_w[slot][i];
+  }
+  EPI_LOOP_BEGIN
+    if (c < e.D) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p;
+        const int64_t idx = (int64_t)r * e.D + c;
+        float v = acc[mi][mj][q * 4 + p];
+        if (e.map == 1) { const float u = e.Ucur[idx]; v *= 3.f * u * u; }
+        const float k = -v;
+        const bool live = r < e.B;
+        if (e.mode == SM_PLAIN) {
+          e.plain_out[idx] = v;
+        } else if (e.mode == SM_INIT_F0) {
+          e.kl[idx] = k;
+          const float sk = atol + fabsf(e.lam[idx]) * rtol;
+          if (live) { const double a = (double)(k / sk); sum += a * a; }
+        } else if (e.mode == SM_INIT_F1) {
+          e.kl[7 * e.sv + idx] = k;
+          const float f0 = e.kl[idx];
+          const float sk = atol + fabsf(e.lam[idx]) * rtol;
+          if (live) {
+            const double a = (double)((k - f0) / sk);
+            sum += a * a;
+            if (k != f0) flag = 1.0;
+          }
+        } else {
+          e.kl[s * e.sv + idx] = k;
+          if (s < 6) {
+            float sm = cA[s + 1][s] * k;
+            for (int j = 0; j < s; ++j) sm += cA[s + 1][j] * e.kl[j * e.sv + idx];
+            const float ls = e.lam[idx] + dt * sm;
+            if (s == 5) e.lam_new[idx] = ls;
+            L[(int64_t)r * e.ldl + c] = cvt<T>(ls);
+            LT[(int64_t)c * e.ldlt + r] = cvt<T>(ls);
+            const float* te = e.tape + (int64_t)tj * e.ent;
+            float u = te[idx];
+            if (!ex) {
+              float sm2 = 0.f;
+#pragma unroll
+              for (int i = 0; i < 7; ++i) sm2 += w[i] * te[(1 + i) * e.sv + idx];
+              u += tdt * sm2;
+            }
+            const float x = map_f(e.map, u);
+            X0[(int64_t)r * e.ldx + c] = cvt<T>(x);
+            X0T[(int64_t)c * e.ldxt + r] = cvt<T>(x);
+            e.Ucur[idx] = u;
+          } else {
+            float ut = cBT[6] * k;
+            for (int j = 0; j < 6; ++j) ut += cBT[j] * e.kl[j * e.sv + idx];
+            ut *= dt;
+            const float lp = e.lam[idx], ln = e.lam_new[idx];
+            const float sk = atol + fmaxf(fabsf(lp), fabsf(ln)) * rtol;
+            if (live) {
+              const double a = (double)(ut / sk);
+              sum += a * a;
+              if (isnan(ln)) flag = 1.0;
+            }
+          }
+        }
+      }
+    }
+  EPI_LOOP_END
+  if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
+}
+
+template <typename T, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiWgrad& e, f32x16 (&acc)[MR][MC], int rb,
+                                          int cb, int lane, const Ctl* ctl) {
+  const int gsel = ctl->gsel;
+  float* gcur = gsel ? e.gf1 : e.gf0;
+  float* gnext = gsel ? e.gf0 : e.gf1;
+  EPI_LOOP_BEGIN
+    if (c < e.out) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int i = r0 + p;
+        if (i < e.in) {
+          const int64_t idx = e.w0 + (int64_t)i * e.out + c;
+          const float g = -acc[mi][mj][q * 4 + p];
+          if (e.flags & WG_STORE_CUR) gcur[idx] = g;
+          if (e.flags & WG_STORE_PTR) e.gptr[idx] = g;
+          if (e.flags & WG_INIT) {
+            const float g1 = gcur[idx];
+            e.accb[idx] = e.cb1 * g1 + e.cb * g;
+            e.acce[idx] = e.ce1 * g1 + e.ce * g;
+          } else {
+            if (e.flags & WG_B) e.accb[idx] += e.cb * g;
+            if (e.flags & WG_E) e.acce[idx] += e.ce * g;
+          }
+          if (e.flags & WG_STORE_NEXT) gnext[idx] = g;
+        }
+      }
+    }
+  EPI_LOOP_END
+}
+
+// ---------------------------------------------------------------- GEMM
+// C[r][c] = sum_k A[r][k] * B[c][k]; rows = batch (or input feature for
+// weight gradients), cols = output feature.  256-thread workgroups of WRxWC
+// waves, each wave a (BR/WR)x(BC/WC) block of 32x32 MFMA tiles; K is staged
+// through LDS in 128-byte rows (64 bf16 / 32 fp32), double-buffered, one
+// barrier per K tile.  Tile order is XCD-aware: consecutive tiles of the
+// row-major tile grid go to the same XCD so that its L2 holds their operands.
+template <typename T, int BR, int BC, int WR, int WC, typename Epi>
+__global__ void __launch_bounds__(WR * WC * 64)
+    gemm_tn(GemmArgs g, Epi epi, const Ctl* __restrict__ ctl, int gate) {
+  if (!gate_ok(ctl, gate)) return;
+  constexpr int NT = WR * WC * 64;
+  constexpr int PITCH = 144;
+  constexpr int BKB = 128;  // bytes of K per tile row
+  constexpr int MR = BR / WR / 32, MC = BC / WC / 32;
+  constexpr int RP = NT / 8;  // rows per load pass
+  constexpr int A_LOADS = BR / RP, B_LOADS = BC / RP;
+  static_assert(BR % RP == 0 && BC % RP == 0, "tile/thread mismatch");
+  __shared__ __attribute__((aligned(16))) unsigned char lds[2 * (BR + BC) * PITCH];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WC, wc = wave % WC;
+  const int nt = g.row_tiles * g.col_tiles;
+  int bid = blockIdx.x;
+  if ((nt & 7) == 0) bid = (bid & 7) * (nt >> 3) + (bid >> 3);
+  const int ct = bid % g.col_tiles, rt = bid / g.col_tiles;
+  const int row0 = rt * BR, col0 = ct * BC;
+
+  const int64_t ldab = g.lda * (int64_t)sizeof(T), ldbb = g.ldb * (int64_t)sizeof(T);
+  const unsigned char* Ag = (const unsigned char*)g.A + (int64_t)row0 * ldab;
+  const unsigned char* Bg = (const unsigned char*)g.B + (int64_t)col0 * ldbb;
+  const int lr = tid >> 3, lc = tid & 7;
+
+  uint4 ra[A_LOADS], rbv[B_LOADS];
+  f32x16 acc[MR][MC];
+#pragma unroll
+  for (int i = 0; i < MR; ++i)
+#pragma unroll
+    for (int j = 0; j < MC; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+  const int nk = (int)((int64_t)g.K * sizeof(T) / BKB);
+  auto gload = [&](int kt) {
+    const int64_t kb = (int64_t)kt * BKB + lc * 16;
+#pragma unroll
+    for (int i = 0; i < A_LOADS; ++i)
+      ra[i] = *(const uint4*)(Ag + (int64_t)(lr + i * RP) * ldab + kb);
+#pragma unroll
+    for (int i = 0; i < B_LOADS; ++i)
+      rbv[i] = *(const uint4*)(Bg + (int64_t)(lr + i * RP) * ldbb + kb);
+  };
+  auto sstore = [&](int buf) {
+    unsigned char* base = lds + buf * (BR + BC) * PITCH;
+#pragma unroll
+    for (int i = 0; i < A_LOADS; ++i) *(uint4*)(base + (lr + i * RP) * PITCH + lc * 16) = ra[i];
+#pragma unroll
+    for (int i = 0; i < B_LOADS; ++i)
+      *(uint4*)(base + (BR + lr + i * RP) * PITCH + lc * 16) = rbv[i];
+  };
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int arow = wr * (BR / WR) + (lane & 31);
+  const int brow = BR + wc * (BC / WC) + (lane & 31);
+  const int koff = (lane >> 5) * 16;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload(kt + 1);
+    const unsigned char* base = lds + cur * (BR + BC) * PITCH;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      uint4 fa[MR], fb[MC];
+#pragma unroll
+      for (int mi = 0; mi < MR; ++mi)
+        fa[mi] = *(const uint4*)(base + (arow + mi * 32) * PITCH + kk * 32 + koff);
+#pragma unroll
+      for (int mj = 0; mj < MC; ++mj)
+        fb[mj] = *(const uint4*)(base + (brow + mj * 32) * PITCH + kk * 32 + koff);
+#pragma unroll
+      for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+        for (int mj = 0; mj < MC; ++mj) Mma<T>::run(acc[mi][mj], fa[mi], fb[mj]);
+    }
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+  epi_apply<T, MR, MC>(epi, acc, row0 + wr * (BR / WR), col0 + wc * (BC / WC), lane, ctl);
+}
+
+// tile configurations: {BR, BC, WR, WC}
+static const int kCfg[4][4] = {{128, 128, 2, 2}, {128, 64, 2, 2}, {64, 64, 2, 2}, {32, 64, 1, 2}};
+int gemm_cfg_count() { return 4; }
+int gemm_cfg_tile_rows(int cfg) { return kCfg[cfg][0]; }
+int gemm_cfg_tile_cols(int cfg) { return kCfg[cfg][1]; }
+
+template <typename T, typename Epi>
+static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, int gate,
+                     hipStream_t s) {
+  const int nt = g.row_tiles * g.col_tiles;
+  switch (cfg) {
+    case 0: hipLaunchKernelGGL((gemm_tn<T, 128, 128, 2, 2, Epi>), dim3(nt), dim3(256), 0, s, g, e, ctl, gate); break;
+    case 1: hipLaunchKernelGGL((gemm_tn<T, 128, 64, 2, 2, Epi>), dim3(nt), dim3(256), 0, s, g, e, ctl, gate); break;
+    case 2: hipLaunchKernelGGL((gemm_tn<T, 64, 64, 2, 2, Epi>), dim3(nt), dim3(256), 0, s, g, e, ctl, gate); break;
+    default: hipLaunchKernelGGL((gemm_tn<T, 32, 64, 1, 2, Epi>), dim3(nt), dim3(128), 0, s, g, e, ctl, gate); break;
+  }
+}
+template <typename Epi>
+static void launch_p(int prec, int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl,
+                     int gate, hipStream_t s) {
+  if (prec == 1) launch_t<bf16>(cfg, g, e, ctl, gate, s);
+  else launch_t<float>(cfg, g, e, ctl, gate, s);
+}
+void launch_gemm_hidden(int prec, int cfg, const GemmArgs& g, const EpiHidden& e,
+                        const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+void launch_gemm_fwdstage(int prec, int cfg, const GemmArgs& g, const EpiFwdStage& e,
+                          const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+void launch_gemm_bwddata(int prec, int cfg, const GemmArgs& g, const EpiBwdData& e,
+                         const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+void launch_gemm_bwdlast(int prec, int cfg, const GemmArgs& g, const EpiBwdLast& e,
+                         const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
+                       const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+
+// ---------------------------------------------------------------- elementwise
+constexpr int EW_THREADS = 256;
+constexpr int EW_MAX_BLOCKS = 1024;
+int state_blocks(const StateGeom& sg) {
+  int64_t n = (int64_t)sg.Bp * sg.D;
+  int64_t b = (n + EW_THREADS - 1) / EW_THREADS;
+  return (int)(b < EW_MAX_BLOCKS ? b : EW_MAX_BLOCKS);
+}
+
+template <typename T>
+__global__ void k_fwd_init(StateGeom sg, const float* __restrict__ u0, float* tape, T* X0,
+                           const Ctl* ctl, double* part) {
+  const double atol = ctl->abstol, rtol = ctl->reltol;
+  double sum = 0.0;
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / sg.D), c = (int)(i % sg.D);
+    const float u = r < sg.B ? u0[i] : 0.f;
+    tape[i] = u;
+    X0[(int64_t)r * sg.Dp + c] = cvt<T>(map_f(sg.map, u));
+    if (r < sg.B) { const double a = (double)u / (atol + fabs((double)u) * rtol); sum += a * a; }
+  }
+  block_reduce_store(sum, 0.0, part, blockIdx.x);
+}
+void launch_fwd_init(const StateGeom& sg, const float* u0, float* tape, void* X0,
+                     const Ctl* ctl, double* part, int nblk, hipStream_t s) {
+  if (sg.prec == 1) hipLaunchKernelGGL(k_fwd_init<bf16>, dim3(nblk), dim3(EW_THREADS), 0, s, sg, u0, tape, (bf16*)X0, ctl, part);
+  else hipLaunchKernelGGL(k_fwd_init<float>, dim3(nblk), dim3(EW_THREADS), 0, s, sg, u0, tape, (float*)X0, ctl, part);
+}
+
+template <typename T>
+__global__ void k_fwd_initdt_prep(StateGeom sg, const float* __restrict__ tape, T* X0,
+                                  const Ctl* ctl) {
+  const float h = (float)(ctl->tdir * ctl->dt0);
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / sg.D), c = (int)(i % sg.D);
+    const float u = tape[i] + h * tape[sg.sv + i];
+    X0[(int64_t)r * sg.Dp + c] = cvt<T>(map_f(sg.map, u));
+  }
+}
+void launch_fwd_initdt_prep(const StateGeom& sg, const float* tape, void* X0, const Ctl* ctl,
+                            hipStream_t s) {
+  int nb = state_blocks(sg);
+  if (sg.prec == 1) hipLaunchKernelGGL(k_fwd_initdt_prep<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (bf16*)X0, ctl);
+  else hipLaunchKernelGGL(k_fwd_initdt_prep<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (float*)X0, ctl);
+}
+
+// forward step begin: pending saves of the last accepted step, then the
+// stage-2 input of the next attempt
+template <typename T>
+__global__ void k_fwd_step_begin(StateGeom sg, const float* __restrict__ tape, T* X0,
+                                 const double* __restrict__ save_times, float* out,
+                                 const Ctl* ctl) {
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  const int64_t lo = ctl->save_lo, hi = ctl->save_hi;
+  const double t = ctl->t;
+  const int64_t cur = ctl->n;
+  const float* ecur = tape + cur * sg.ent;
+  const float* es = tape + ctl->save_step * sg.ent;
+  const double tprev = ctl->tprev, dts = ctl->dt_step;
+  const int64_t nreal = (int64_t)sg.B * sg.D;
+  for (int64_t si = lo; si < hi; ++si) {
+    const double ts = save_times[si];
+    float w[7];
+    const bool exact = (ts == t);
+    if (!exact) {
+      const double th = (ts - tprev) / dts, th2 = th * th;
+      w[0] = (float)(th * (dR1[0] + th * (dR1[1] + th * (dR1[2] + th * dR1[3]))));
+      for (int i = 0; i < 6; ++i) w[i + 1] = (float)(th2 * (dRI[i][0] + th * (dRI[i][1] + th * dRI[i][2])));
+    }
+    float* o = out + si * nreal;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nreal;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      float v;
+      if (exact) v = ecur[i];
+      else {
+        float sm = 0.f;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) sm += w[j] * es[(1 + j) * sg.sv + i];
+        v = es[i] + (float)dts * sm;
+      }
+      o[i] = v;
+    }
+  }
+  if (ctl->done) return;
+  const float h = (float)ctl->dt * cA[1][0];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / sg.D), c = (int)(i % sg.D);
+    const float u = ecur[i] + h * ecur[sg.sv + i];
+    X0[(int64_t)r * sg.Dp + c] = cvt<T>(map_f(sg.map, u));
+  }
+}
+void launch_fwd_step_begin(const StateGeom& sg, const float* tape, void* X0,
+                           const double* save_times, float* out, const Ctl* ctl,
+                           hipStream_t s) {
+  int nb = state_blocks(sg);
+  if (sg.prec == 1) hipLaunchKernelGGL(k_fwd_step_begin<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (bf16*)X0, save_times, out, ctl);
+  else hipLaunchKernelGGL(k_fwd_step_begin<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (float*)X0, save_times, out, ctl);
+}
+
+// ---------------------------------------------------------------- controllers
+__device__ double block_sum_parts(const double* part, int n, double* flag_out) {
+  __shared__ double rs[256], rf[256];
+  double s = 0.0, f = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) { s += part[2 * i]; f = fmax(f, part[2 * i + 1]); }
+  rs[threadIdx.x] = s;
+  rf[threadIdx.x] = f;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rs[threadIdx.x] += rs[threadIdx.x + w];
+      rf[threadIdx.x] = fmax(rf[threadIdx.x], rf[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  const double out = rs[0], fo = rf[0];
+  __syncthreads();
+  if (flag_out) *flag_out = fo;
+  return out;
+}
+
+__device__ void interp_weights(double th, float* w) {
+  const double th2 = th * th;
+  w[0] = (float)(th * (dR1[0] + th * (dR1[1] + th * (dR1[2] + th * dR1[3]))));
+  for (int i = 0; i < 6; ++i) w[i + 1] = (float)(th2 * (dRI[i][0] + th * (dRI[i][1] + th * dRI[i][2])));
+}
+
+// SciMLBase sol(tau) on the forward record: exact stored state at a step
+// time, else the interpolant of the bracketing step with dt = t_i - t_{i-1}
+__device__ void set_interp(Ctl* c, int slot, double tau, const double* ft) {
+  const int64_t nf = c->n_fwd;
+  int64_t lo = 0, hi = nf + 1;  // first index with ft[i] >= tau
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (ft[mid] < tau) lo = mid + 1; else hi = mid;
+  }
+  int64_t i = lo;
+  if (i <= nf && ft[i] == tau) { c->st_step[slot] = (int)i; c->st_exact[slot] = 1; return; }
+  if (i > 0 && ft[i - 1] == tau) { c->st_step[slot] = (int)(i - 1); c->st_exact[slot] = 1; return; }
+  if (i < 1) i = 1;
+  if (i > nf) i = nf;
+  const double dtj = ft[i] - ft[i - 1];
+  c->st_step[slot] = (int)(i - 1);
+  c->st_exact[slot] = 0;
+  c->st_dt[slot] = (float)dtj;
+  interp_weights((tau - ft[i - 1]) / dtj, c->st_w[slot]);
+}
+
+// loopheader + check_error for the next attempt; dt already holds the
+// proposal (after accept) or the shrunk step (after reject)
+__device__ void loopheader(Ctl* c, double tstop) {
+  c->iter += 1;
+  double dt = c->dt;
+  if (fabs(dt) > c->dtmax_abs) dt = c->tdir * c->dtmax_abs;
+  const double dtmin_t = fmax(dspacing(c->t), c->dtmin);
+  if (fabs(dt) < dtmin_t) dt = c->tdir * dtmin_t;
+  if (c->adaptive) dt = c->tdir * fmin(fabs(dt), fabs(tstop - c->t));
+  else dt = c->tdir * fmin(c->dt_fixed, fabs(tstop - c->t));
+  c->dt = dt;
+  if (isnan(dt)) { c->retcode = 3; c->done = 1; }
+  else if (c->iter > c->maxiters) { c->retcode = 4; c->done = 1; }
+  else if (c->adaptive && fabs(dt) <= c->dtmin &&
+           (!c->last_accept || c->tdir * (c->t + dt) < c->tdir * tstop)) {
+    c->retcode = 5; c->done = 1;
+  }
+}
+
+__device__ double snap_t(double t, double ttmp, double tstop) {
+  return fabs(ttmp - tstop) < 100.0 * dspacing(fmax(t, tstop)) ? tstop : ttmp;
+}
+
+// accept/reject decision common to forward and adjoint; returns accepted
+__device__ bool pi_control(Ctl* c, double eest) {
+  c->eest = eest;
+  if (!c->adaptive) return true;
+  double q;
+  if (eest == 0.0) q = 1.0 / QMAX;
+  else {
+    const double q11 = pow(eest, BETA1);
+    c->q11 = q11;
+    q = q11 / pow(c->qold, BETA2);
+    q = fmax(1.0 / QMAX, fmin(1.0 / QMIN, q / GAMMA));
+  }
+  const bool acc = eest <= 1.0;
+  if (acc) {
+    if (q >= 1.0 && q <= 1.0) q = 1.0;
+    c->qold = fmax(eest, QOLDINIT);
+    double dtnew = c->dt / q;
+    dtnew = c->tdir * fmin(c->dtmax_abs, fabs(dtnew));
+    c->dtpropose = c->tdir * fmax(fabs(dtnew), c->dtmin);
+  }
+  return acc;
+}
+
+__global__ void k_initdt_a(Ctl* c, const double* p0, int n0, const double* px, int nx,
+                           const double* pm, int nm, double nel) {
+  double s0 = block_sum_parts(p0, n0, nullptr);
+  double s1 = block_sum_parts(px, nx, nullptr);
+  double sm = pm ? block_sum_parts(pm, nm, nullptr) : 0.0;
+  if (threadIdx.x != 0) return;
+  const double d0 = sqrt(s0 / nel), d1 = sqrt((s1 + sm) / nel);
+  const double dtmin = nextafter(fmax(c->dtmin, dspacing(c->t)), INFINITY);
+  const double smalldt = fmax(dtmin, 1e-6);
+  double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : 0.01 * (d0 / d1);
+  dt0 = fmin(dt0, c->dtmax_abs);
+  c->d0 = d0;
+  c->d1 = d1;
+  c->pad_ = 0;
+  if (!(dt0 >= 10 * 2.220446049250313e-16)) { dt0 = smalldt; c->pad_ = 1; }
+  c->dt0 = dt0;
+}
+void launch_initdt_a(Ctl* ctl, const double* p0, int n0, const double* px, int nx,
+                     const double* pm, int nm, double n_elems, hipStream_t s) {
+  hipLaunchKernelGGL(k_initdt_a, dim3(1), dim3(256), 0, s, ctl, p0, n0, px, nx, pm, nm, n_elems);
+}
+
+__global__ void k_initdt_b(Ctl* c, const double* px, int nx, const double* pm, int nm,
+                           double nel, const double* save_times, int is_adj,
+                           const double* ft) {
+  double fl1 = 0.0, fl2 = 0.0;
+  double s2 = block_sum_parts(px, nx, &fl1);
+  double sm = pm ? block_sum_parts(pm, nm, &fl2) : 0.0;
+  if (threadIdx.x != 0) return;
+  if (c->has_dt_) {
+    // dt given by the caller (adaptive initial dt or fixed step)
+  } else if (c->pad_) {
+    c->dt = c->tdir * c->dt0;
+  } else {
+    const double dt0 = c->dt0;
+    const double dtmin = nextafter(fmax(c->dtmin, dspacing(c->t)), INFINITY);
+    if (fmax(fl1, fl2) == 0.0) {
+      c->dt = c->tdir * fmax(dtmin, 100 * dt0);
+    } else {
+      const double d2 = sqrt((s2 + sm) / nel) / dt0;
+      const double m = fmax(c->d1, d2);
+      double dt1;
+      if (m <= 1e-15) dt1 = fmax(1e-6, dt0 * 1e-3);
+      else dt1 = pow(10.0, -(2.0 + log10(m)) / 5.0);
+      c->dt = c->tdir * fmax(dtmin, fmin(fmin(100 * dt0, dt1), c->dtmax_abs));
+    }
+  }
+  c->nf += c->has_dt_ ? 0 : 1;
+  c->dtpropose = c->dt;
+  c->last_accept = 0;
+  double tstop = c->t_end;
+  if (!is_adj) {
+    // saves at t0 (save_start)
+    int64_t k = 0;
+    while (k < c->n_save && c->tdir * save_times[k] <= c->tdir * c->t) ++k;
+    c->save_lo = 0;
+    c->save_hi = k;
+    c->save_next = k;
+    c->save_step = 0;
+  } else {
+    if (c->save_next >= 0) tstop = save_times[c->save_next];
+  }
+  loopheader(c, tstop);
+  if (is_adj) {
+    for (int s = 0; s < 7; ++s) set_interp(c, s, c->t + dC[s] * c->dt, ft);
+  }
+}
+void launch_initdt_b(Ctl* ctl, const double* px, int nx, const double* pm, int nm,
+                     double n_elems, const double* save_times, int is_adjoint,
+                     const double* fwd_times, hipStream_t s) {
+  hipLaunchKernelGGL(k_initdt_b, dim3(1), dim3(256), 0, s, ctl, px, nx, pm, nm, n_elems,
+                     save_times, is_adjoint, fwd_times);
+}
+
+__global__ void k_fwd_controller(Ctl* c, const double* part, int np, const double* save_times,
+                                 double* ft, double nel) {
+  if (c->done) return;
+  double flag = 0.0;
+  const double s = block_sum_parts(part, np, &flag);
+  if (threadIdx.x != 0) return;
+  c->nf += 6;
+  const double eest = c->adaptive ? sqrt(s / nel) : 0.0;
+  if (c->adaptive && isnan(eest)) { c->retcode = 3; c->done = 1; c->save_lo = c->save_hi = c->save_next; return; }
+  const bool acc = pi_control(c, eest);
+  c->accepted = acc;
+  if (acc) {
+    c->naccept += 1;
+    const double ttmp = c->t + c->dt;
+    c->tprev = c->t;
+    c->t = snap_t(c->t, ttmp, c->t_end);
+    c->dt_step = c->dt;
+    if (!c->adaptive) c->dtpropose = c->dt;
+    c->save_step = c->n;
+    c->n += 1;
+    ft[c->n] = c->t;
+    int64_t k = c->save_next;
+    c->save_lo = k;
+    while (k < c->n_save && c->tdir * save_times[k] <= c->tdir * c->t) ++k;
+    c->save_hi = k;
+    c->save_next = k;
+    c->last_accept = 1;
+    if (flag != 0.0) { c->retcode = 6; c->done = 1; return; }
+    if (c->t == c->t_end) { c->retcode = 1; c->done = 1; return; }
+    c->dt = c->dtpropose;
+  } else {
+    c->nreject += 1;
+    c->last_accept = 0;
+    c->save_lo = c->save_hi = c->save_next;
+    c->dt = c->dt / fmin(1.0 / QMIN, c->q11 / GAMMA);
+  }
+  loopheader(c, c->t_end);
+}
+void launch_fwd_controller(Ctl* ctl, const double* part, int n_part, const double* save_times,
+                           double* fwd_times, double n_elems, hipStream_t s) {
+  hipLaunchKernelGGL(k_fwd_controller, dim3(1), dim3(256), 0, s, ctl, part, n_part, save_times,
+                     fwd_times, n_elems);
+}
+
+// ---------------------------------------------------------------- adjoint
+template <typename T>
+__device__ __forceinline__ void adj_prep_stage(const StateGeom& sg, const AdjBufs& ab,
+                                               const Ctl* c, int slot, int64_t i, float lam_s) {
+  const int r = (int)(i / sg.D), col = (int)(i % sg.D);
+  ((T*)ab.L)[(int64_t)r * ab.ldl + col] = cvt<T>(lam_s);
+  ((T*)ab.LT)[(int64_t)col * ab.ldlt + r] = cvt<T>(lam_s);
+  const float* te = ab.tape + (int64_t)c->st_step[slot] * sg.ent;
+  float u = te[i];
+  if (!c->st_exact[slot]) {
+    float sm = 0.f;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) sm += c->st_w[slot][j] * te[(1 + j) * sg.sv + i];
+    u += c->st_dt[slot] * sm;
+  }
+  const float x = map_f(sg.map, u);
+  ((T*)ab.X0)[(int64_t)r * ab.ldx + col] = cvt<T>(x);
+  ((T*)ab.X0T)[(int64_t)col * ab.ldxt + r] = cvt<T>(x);
+  ab.U[i] = u;
+}
+
+template <typename T>
+__global__ void k_adj_init(StateGeom sg, AdjBufs ab, const Ctl* c, double* part) {
+  const double atol = c->abstol, rtol = c->reltol;
+  const int jidx = c->jump_idx;
+  double sum = 0.0;
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / sg.D);
+    float l = 0.f;
+    if (jidx >= 0 && r < sg.B) l = ab.dLdout[(int64_t)jidx * sg.B * sg.D + i];
+    ab.lam[i] = l;
+    adj_prep_stage<T>(sg, ab, c, 0, i, l);
+    if (r < sg.B) { const double a = (double)l / (atol + fabs((double)l) * rtol); sum += a * a; }
+  }
+  block_reduce_store(sum, 0.0, part, blockIdx.x);
+}
+void launch_adj_init(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl, double* part,
+                     hipStream_t s) {
+  int nb = state_blocks(sg);
+  if (sg.prec == 1) hipLaunchKernelGGL(k_adj_init<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl, part);
+  else hipLaunchKernelGGL(k_adj_init<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl, part);
+}
+
+template <typename T>
+__global__ void k_adj_initdt_prep(StateGeom sg, AdjBufs ab, const Ctl* c) {
+  const float h = (float)(c->tdir * c->dt0);
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    adj_prep_stage<T>(sg, ab, c, 7, i, ab.lam[i] + h * ab.kl[i]);
+}
+void launch_adj_initdt_prep(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl,
+                            hipStream_t s) {
+  int nb = state_blocks(sg);
+  if (sg.prec == 1) hipLaunchKernelGGL(k_adj_initdt_prep<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl);
+  else hipLaunchKernelGGL(k_adj_initdt_prep<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl);
+}
+
+// adjoint step begin (idempotent, so speculative launches after the end are
+// harmless): lambda = lambda_new + jump on accept, FSAL copy, mu commit into
+// the other mu buffer, and the first stage input of the next attempt
+template <typename T>
+__global__ void k_adj_step_begin(StateGeom sg, AdjBufs ab, const Ctl* c, int nbs) {
+  if ((int)blockIdx.x >= nbs) {
+    if (!c->commit_mu) return;
+    const int ms = c->gsel_mu_;
+    const double* src = ms ? ab.mu : ab.mu + ab.P;
+    double* dst = ms ? ab.mu + ab.P : ab.mu;
+    const double h = c->dt_step;
+    for (int64_t i = (blockIdx.x - nbs) * (int64_t)blockDim.x + threadIdx.x; i < ab.P;
+         i += (int64_t)(gridDim.x - nbs) * blockDim.x)
+      dst[i] = src[i] + h * (double)ab.accb[i];
+    return;
+  }
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  const int acc = c->last_accept, jidx = c->jump_idx, done = c->done, re = c->fsal_reeval;
+  const float h = (float)c->dt * cA[1][0];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)nbs * blockDim.x) {
+    const int r = (int)(i / sg.D);
+    float l = acc ? ab.lam_new[i] : ab.lam[i];
+    if (acc && jidx >= 0 && r < sg.B) l += ab.dLdout[(int64_t)jidx * sg.B * sg.D + i];
+    if (acc) { ab.lam[i] = l; ab.kl[i] = ab.kl[6 * sg.sv + i]; }
+    if (done) continue;
+    if (re) adj_prep_stage<T>(sg, ab, c, 0, i, l);
+    else adj_prep_stage<T>(sg, ab, c, 1, i, l + h * (acc ? ab.kl[6 * sg.sv + i] : ab.kl[i]));
+  }
+}
+int mu_blocks(int64_t P) {
+  int64_t b = (P + EW_THREADS * 4 - 1) / (EW_THREADS * 4);
+  return (int)(b < EW_MAX_BLOCKS ? (b < 1 ? 1 : b) : EW_MAX_BLOCKS);
+}
+void launch_adj_step_begin(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl, int gate,
+                           hipStream_t s) {
+  (void)gate;
+  int nbs = state_blocks(sg), nbm = mu_blocks(ab.P);
+  if (sg.prec == 1) hipLaunchKernelGGL(k_adj_step_begin<bf16>, dim3(nbs + nbm), dim3(EW_THREADS), 0, s, sg, ab, ctl, nbs);
+  else hipLaunchKernelGGL(k_adj_step_begin<float>, dim3(nbs + nbm), dim3(EW_THREADS), 0, s, sg, ab, ctl, nbs);
+}
+
+// mu part of the error norm: modes 0 step error, 1 initdt d1, 2 initdt d2
+__global__ void k_mu_norm(int mode, AdjBufs ab, const Ctl* c, double* part, int gate) {
+  if (!gate_ok(c, gate)) return;
+  const double atol = c->abstol, rtol = c->reltol, dt = c->dt;
+  const int ms = c->gsel_mu_;
+  const double* mu = ms ? ab.mu + ab.P : ab.mu;
+  const float* gcur = c->gsel ? ab.gf1 : ab.gf0;
+  double sum = 0.0, flag = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ab.P;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double m = mu[i];
+    double a;
+    if (mode == 0) {
+      const double mn = m + dt * (double)ab.accb[i];
+      a = dt * (double)ab.acce[i] / (atol + fmax(fabs(m), fabs(mn)) * rtol);
+    } else if (mode == 1) {
+      a = (double)gcur[i] / (atol + fabs(m) * rtol);
+    } else {
+      a = ((double)ab.accb[i] - (double)gcur[i]) / (atol + fabs(m) * rtol);
+      if (ab.accb[i] != gcur[i]) flag = 1.0;
+    }
+    sum += a * a;
+  }
+  block_reduce_store(sum, flag, part, blockIdx.x);
+}
+void launch_mu_norm(int mode, const AdjBufs& ab, const Ctl* ctl, double* part, int nblk,
+                    int gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_mu_norm, dim3(nblk), dim3(EW_THREADS), 0, s, mode, ab, ctl, part, gate);
+}
+
+__global__ void k_adj_controller(Ctl* c, const double* pl, int nl, const double* pm, int nm,
+                                 double nel, const double* cbt, const double* ft) {
+  if (c->done) return;
+  double flag = 0.0;
+  const double sl = block_sum_parts(pl, nl, &flag);
+  const double sm = block_sum_parts(pm, nm, nullptr);
+  if (threadIdx.x != 0) return;
+  c->nf += c->fsal_reeval ? 7 : 6;
+  const double eest = c->adaptive ? sqrt((sl + sm) / nel) : 0.0;
+  if (c->adaptive && isnan(eest)) { c->retcode = 3; c->done = 1; c->commit_mu = 0; c->last_accept = 0; c->jump_idx = -1; return; }
+  const bool acc = pi_control(c, eest);
+  c->accepted = acc;
+  c->fsal_reeval = 0;
+  c->jump_idx = -1;
+  double tstop = c->save_next >= 0 ? cbt[c->save_next] : c->t_end;
+  if (acc) {
+    c->naccept += 1;
+    const double ttmp = c->t + c->dt;
+    c->tprev = c->t;
+    c->t = snap_t(c->t, ttmp, tstop);
+    c->dt_step = c->dt;
+    if (!c->adaptive) c->dtpropose = c->dt;
+    c->n += 1;
+    c->last_accept = 1;
+    c->commit_mu = 1;
+    c->gsel_mu_ ^= 1;
+    c->gsel ^= 1;
+    if (c->save_next >= 0 && c->t == cbt[c->save_next]) {
+      c->jump_idx = (int)c->save_next;
+      c->save_next -= 1;
+      c->fsal_reeval = 1;
+    }
+    if (flag != 0.0) { c->retcode = 6; c->done = 1; return; }
+    if (c->t == c->t_end) { c->retcode = 1; c->done = 1; return; }
+    c->dt = c->dtpropose;
+  } else {
+    c->nreject += 1;
+    c->last_accept = 0;
+    c->commit_mu = 0;
+    c->dt = c->dt / fmin(1.0 / QMIN, c->q11 / GAMMA);
+  }
+  tstop = c->save_next >= 0 ? cbt[c->save_next] : c->t_end;
+  loopheader(c, tstop);
+  for (int s = 0; s < 7; ++s) set_interp(c, s, c->t + dC[s] * c->dt, ft);
+}
+void launch_adj_controller(Ctl* ctl, const double* part, int n_lam, const double* part_mu,
+                           int n_mu, double n_elems, const double* cb_times,
+                           const double* fwd_times, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_controller, dim3(1), dim3(256), 0, s, ctl, part, n_lam, part_mu, n_mu,
+                     n_elems, cb_times, fwd_times);
+}
+
+__global__ void k_set_stage_interp(Ctl* c, int slot, double tau, const double* ft) {
+  if (threadIdx.x == 0) {
+    double t = tau;
+    if (slot == 7) t = c->t + c->tdir * c->dt0;
+    set_interp(c, slot, t, ft);
+  }
+}
+void launch_set_stage_interp(Ctl* ctl, int slot, double tau, const double* fwd_times,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_set_stage_interp, dim3(1), dim3(64), 0, s, ctl, slot, tau, fwd_times);
+}
+
+// bias gradients: g_b[o] = -sum_b delta^T[o][b], folded like the weights
+template <typename T>
+__global__ void k_bias_fold(int nl, const void* const* dT, const int* outs, const int64_t* b0s,
+                            int ldt, int Bp, EpiWgrad e, const Ctl* c, int gate, int rows_total) {
+  if (!gate_ok(c, gate)) return;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (wave >= rows_total) return;
+  int l = 0, row = wave;
+  while (l < nl && row >= outs[l]) { row -= outs[l]; ++l; }
+  const T* src = (const T*)dT[l] + (int64_t)row * ldt;
+  float s = 0.f;
+  for (int b = lane; b < Bp; b += 64) s += tof<T>(src[b]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane != 0) return;
+  const int64_t idx = b0s[l] + row;
+  const float g = -s;
+  float* gcur = c->gsel ? e.gf1 : e.gf0;
+  float* gnext = c->gsel ? e.gf0 : e.gf1;
+  if (e.flags & WG_STORE_CUR) gcur[idx] = g;
+  if (e.flags & WG_STORE_PTR) e.gptr[idx] = g;
+  if (e.flags & WG_INIT) {
+    const float g1 = gcur[idx];
+    e.accb[idx] = e.cb1 * g1 + e.cb * g;
+    e.acce[idx] = e.ce1 * g1 + e.ce * g;
+  } else {
+    if (e.flags & WG_B) e.accb[idx] += e.cb * g;
+    if (e.flags & WG_E) e.acce[idx] += e.ce * g;
+  }
+  if (e.flags & WG_STORE_NEXT) gnext[idx] = g;
+}
+void launch_bias_fold(int prec, int nl, const void* const* dT, const int* outs,
+                      const int64_t* b0s, int ldt, int Bp, int rows_total,
+                      const EpiWgrad& tmpl, const Ctl* ctl, int gate, hipStream_t s) {
+  // dT / outs / b0s are device arrays (per layer)
+  const int nb = (rows_total * 64 + 255) / 256;
+  if (prec == 1) hipLaunchKernelGGL(k_bias_fold<bf16>, dim3(nb), dim3(256), 0, s, nl, dT, outs, b0s, ldt, Bp, tmpl, ctl, gate, rows_total);
+  else hipLaunchKernelGGL(k_bias_fold<float>, dim3(nb), dim3(256), 0, s, nl, dT, outs, b0s, ldt, Bp, tmpl, ctl, gate, rows_total);
+}
+
+// plain input preparation for node_rhs / node_vjp
+template <typename T>
+__global__ void k_prep_plain(StateGeom sg, const float* u, T* X0, T* X0T, float* Ucur, int ldx,
+                             int ldxt) {
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / sg.D), c = (int)(i % sg.D);
+    const float v = r < sg.B ? u[i] : 0.f;
+    const float x = map_f(sg.map, v);
+    X0[(int64_t)r * ldx + c] = cvt<T>(x);
+    if (X0T) X0T[(int64_t)c * ldxt + r] = cvt<T>(x);
+    if (Ucur) Ucur[i] = v;
+  }
+}
+void launch_prep_plain(const StateGeom& sg, const float* u, void* X0, void* X0T, float* Ucur,
+                       int ldx, int ldxt, hipStream_t s) {
+  int nb = state_blocks(sg);
+  if (sg.prec == 1) hipLaunchKernelGGL(k_prep_plain<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, u, (bf16*)X0, (bf16*)X0T, Ucur, ldx, ldxt);
+  else hipLaunchKernelGGL(k_prep_plain<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, u, (float*)X0, (float*)X0T, Ucur, ldx, ldxt);
+}
+template <typename T>
+__global__ void k_prep_lambda(StateGeom sg, const float* lam, T* L, T* LT, int ldl, int ldlt) {
+  const int64_t n = (int64_t)sg.Bp * sg.D;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / sg.D), c = (int)(i % sg.D);
+    const float v = r < sg.B ? lam[i] : 0.f;
+    L[(int64_t)r * ldl + c] = cvt<T>(v);
+    LT[(int64_t)c * ldlt + r] = cvt<T>(v);
+  }
+}
+void launch_prep_lambda(const StateGeom& sg, const float* lam, void* L, void* LT, int ldl,
+                        int ldlt, hipStream_t s) {
+  int nb = state_blocks(sg);
+  if (sg.prec == 1) hipLaunchKernelGGL(k_prep_lambda<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, lam, (bf16*)L, (bf16*)LT, ldl, ldlt);
+  else hipLaunchKernelGGL(k_prep_lambda<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, lam, (float*)L, (float*)LT, ldl, ldlt);
+}
+
+// gather real rows of n state vectors (stride src_stride floats) into a dense
+// D x B x n array
+__global__ void k_gather(StateGeom sg, const float* src, int64_t stride, float* dst, int64_t n) {
+  const int64_t per = (int64_t)sg.B * sg.D;
+  const int64_t tot = per * n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = i / per, k = i % per;
+    dst[i] = src[j * stride + k];
+  }
+}
+void launch_gather(const StateGeom& sg, const float* src, int64_t src_stride_floats, float* dst,
+                   int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather, dim3(EW_MAX_BLOCKS), dim3(EW_THREADS), 0, s, sg, src,
+                     src_stride_floats, dst, n);
+}
+
+}  // namespace nodehip

@@ -1,0 +1,942 @@
+// Host driver and C ABI of the NeuralODE hot path (see include/node_hip.h).
+//
+// The solve loop never waits on the GPU per step: the step controller runs on
+// the device (a single-workgroup kernel after each attempt) and every kernel
+// reads t/dt/accept state from the device control block, so the host enqueues
+// chunks of step attempts and only polls the control block between chunks.
+// Kernels of attempts enqueued after the solve has finished return at once.
+#include "node_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "node_internal.h"
+
+using namespace nodehip;
+
+namespace {
+
+constexpr int KPAD = 64;    // feature padding (K tile of the BF16 GEMM)
+constexpr int BPAD = 128;   // batch padding (largest row tile)
+constexpr int SEC = 2 * MAX_PART;  // doubles per partial-sum section
+
+inline int64_t rup(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+const double kB[7] = {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742,
+                      -3.290069515436081, 2.324710524099774, 0.0};
+const double kBT[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                       -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
+                       0.015151515151515152};
+
+struct Layer {
+  int in, out, inp, outp, act;
+  int64_t w0, b0;
+  void* Wr = nullptr;     // [outp][inp]
+  void* Wc = nullptr;     // [inp][outp]
+  float* bias = nullptr;  // [outp]
+};
+
+struct GemmPlan {
+  int cfg, rt, ct;
+};
+
+}  // namespace
+
+struct node_handle {
+  node_desc desc{};
+  int L = 0, D = 0, Dp = 0, B = 0, Bp = 0, prec = 0, esz = 4;
+  int64_t P = 0;
+  std::vector<Layer> layers;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::mutex mtx;
+
+  Ctl* ctl = nullptr;
+  Ctl* hctl = nullptr;  // pinned mirror
+  double* part = nullptr;
+  float* tape = nullptr;
+  int64_t tape_cap = 0;
+  double* fwd_times = nullptr;
+  double* save_times = nullptr;
+  int64_t save_cap = 0;
+  std::vector<double> h_save;  // full list of saved times of the last forward
+  float* out_dev = nullptr;
+  int64_t out_dev_cap = 0;
+  float* scratch = nullptr;
+  float* ucur = nullptr;
+
+  void* X0 = nullptr;
+  void* X0T = nullptr;
+  std::vector<void*> H, HT, S, Dl, DlT;  // per hidden layer (L-1)
+
+  // adjoint
+  float* lam = nullptr;
+  float* lam_new = nullptr;
+  float* kl = nullptr;
+  void* Lm = nullptr;
+  void* LmT = nullptr;
+  double* mu = nullptr;
+  float* accb = nullptr;
+  float* acce = nullptr;
+  float* gf0 = nullptr;
+  float* gf1 = nullptr;
+  float* dLdout_dev = nullptr;
+  int64_t dLdout_cap = 0;
+  void** d_dT = nullptr;
+  int* d_outs = nullptr;
+  int64_t* d_b0s = nullptr;
+  int bias_rows = 0;
+  float* dp32 = nullptr;
+
+  bool have_params = false, have_forward = false;
+  node_solve_opts fwd_opts{};
+  node_stats fwd_stats{};
+  int64_t n_fwd = 0;
+  int64_t nsaved = 0;
+  bool everystep = false;
+
+  std::vector<void*> allocs;
+
+  // profiling (node_prof_enable)
+  uint64_t prof_mask = 0;
+  struct ProfRec { int key; hipEvent_t a, b; };
+  std::vector<ProfRec> prof;
+  std::vector<hipEvent_t> ev_pool;
+};
+
+namespace {
+
+int fail(node_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(h, NODE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(node_handle* h, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(h, NODE_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemsetAsync(*p, 0, bytes, h->stream);
+  if (e != hipSuccess)
+    return fail(h, NODE_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  h->allocs.push_back(*p);
+  return 0;
+}
+template <typename P>
+int dalloc_t(node_handle* h, P** p, size_t bytes) {
+  void* v = nullptr;
+  int rc = dalloc(h, &v, bytes);
+  *p = (P*)v;
+  return rc;
+}
+
+void dfree(node_handle* h, void* p) {
+  if (!p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+GemmPlan plan(int rows, int cols) {
+  int best = -1, best_tiles = -1;
+  for (int c = 0; c < gemm_cfg_count(); ++c) {
+    const int br = gemm_cfg_tile_rows(c), bc = gemm_cfg_tile_cols(c);
+    if (rows % br || cols % bc) continue;
+    const int t = (rows / br) * (cols / bc);
+    if (t >= 240) return {c, rows / br, cols / bc};
+    if (t > best_tiles) { best = c; best_tiles = t; }
+  }
+  return {best, rows / gemm_cfg_tile_rows(best), cols / gemm_cfg_tile_cols(best)};
+}
+
+enum ProfRole { PR_FWD_HIDDEN = 0, PR_FWD_STAGE = 1, PR_ADJ_HIDDEN = 2, PR_BWD_DATA = 3,
+                PR_WGRAD = 4, PR_BWD_LAST = 5 };
+
+hipEvent_t prof_event(node_handle* h) {
+  if (!h->ev_pool.empty()) {
+    hipEvent_t e = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// RAII pair of events around one launch when its key is selected
+struct ProfScope {
+  node_handle* h;
+  int key;
+  hipEvent_t a = nullptr;
+  ProfScope(node_handle* hh, int role, int layer) : h(hh), key(role * 8 + layer) {
+    if (key < 64 && ((h->prof_mask >> key) & 1ull)) {
+      a = prof_event(h);
+      (void)hipEventRecord(a, h->stream);
+    }
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = prof_event(h);
+    (void)hipEventRecord(b, h->stream);
+    h->prof.push_back({key, a, b});
+  }
+};
+
+StateGeom geom(const node_handle* h) {
+  StateGeom g;
+  g.B = h->B; g.Bp = h->Bp; g.D = h->D; g.Dp = h->Dp;
+  g.map = h->desc.input_map; g.prec = h->prec;
+  g.sv = (int64_t)h->Bp * h->D;
+  g.ent = 8 * g.sv;
+  return g;
+}
+
+int validate_desc(const node_desc* d, std::string& msg) {
+  if (!d) { msg = "desc is NULL"; return NODE_ERR_INVALID; }
+  if (d->n_layers < 1 || d->n_layers > NODE_HIP_MAX_LAYERS) { msg = "n_layers must be 1..8"; return NODE_ERR_INVALID; }
+  for (int i = 0; i <= d->n_layers; ++i)
+    if (d->widths[i] < 1 || d->widths[i] > 65536) { msg = "widths must be 1..65536"; return NODE_ERR_INVALID; }
+  if (d->widths[0] != d->widths[d->n_layers]) { msg = "output width must equal input width (ODE state dimension)"; return NODE_ERR_INVALID; }
+  for (int i = 0; i < d->n_layers; ++i)
+    if (d->acts[i] < 0 || d->acts[i] > 4) { msg = "unknown activation"; return NODE_ERR_INVALID; }
+  if (d->input_map != NODE_MAP_IDENTITY && d->input_map != NODE_MAP_CUBE) { msg = "unknown input_map"; return NODE_ERR_INVALID; }
+  if (d->precision != NODE_PREC_FP32 && d->precision != NODE_PREC_BF16) { msg = "unknown precision"; return NODE_ERR_INVALID; }
+  if (d->batch < 1 || d->batch > (int64_t)1 << 24) { msg = "batch must be 1..2^24"; return NODE_ERR_INVALID; }
+  return 0;
+}
+
+// ---------------------------------------------------------------- RHS / VJP
+// One batched right-hand side: X0 -> layers -> last-layer epilogue.
+void rhs_eval(node_handle* h, int stage, int mode, int gate, double* part) {
+  const int L = h->L;
+  for (int l = 0; l < L; ++l) {
+    const Layer& ly = h->layers[l];
+    GemmArgs g;
+    g.A = l == 0 ? h->X0 : h->H[l - 1];
+    g.lda = ly.inp;
+    g.B = ly.Wr;
+    g.ldb = ly.inp;
+    g.K = ly.inp;
+    if (l < L - 1) {
+      GemmPlan p = plan(h->Bp, ly.outp);
+      g.row_tiles = p.rt; g.col_tiles = p.ct;
+      EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], nullptr, nullptr};
+      ProfScope ps(h, PR_FWD_HIDDEN, l);
+      launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+    } else {
+      GemmPlan p = plan(h->Bp, ly.outp);
+      g.row_tiles = p.rt; g.col_tiles = p.ct;
+      EpiFwdStage e{};
+      e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
+      e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D; e.sv = (int64_t)h->Bp * h->D;
+      e.X0 = h->X0; e.ldx = h->Dp; e.map = h->desc.input_map;
+      e.scratch = h->scratch; e.part = part;
+      ProfScope ps(h, PR_FWD_STAGE, l);
+      launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+    }
+  }
+}
+
+int rhs_last_tiles(node_handle* h) {
+  GemmPlan p = plan(h->Bp, h->layers[h->L - 1].outp);
+  return p.rt * p.ct;
+}
+
+// One adjoint stage: forward recompute at the interpolated u, data and weight
+// gradients, bias gradients, and the layer-1 epilogue (k^lambda + next stage).
+void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float cb, float ce,
+              float cb1, float ce1, double* part, float* plain_out, float* gptr) {
+  const int L = h->L;
+  // forward recompute of the hidden layers (activations, transposes, act')
+  for (int l = 0; l < L - 1; ++l) {
+    const Layer& ly = h->layers[l];
+    GemmArgs g;
+    g.A = l == 0 ? h->X0 : h->H[l - 1];
+    g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
+    GemmPlan p = plan(h->Bp, ly.outp);
+    g.row_tiles = p.rt; g.col_tiles = p.ct;
+    EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], h->HT[l], h->S[l]};
+    ProfScope ps(h, PR_ADJ_HIDDEN, l);
+    launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  }
+  EpiWgrad wt{};
+  wt.accb = h->accb; wt.acce = h->acce; wt.gf0 = h->gf0; wt.gf1 = h->gf1; wt.gptr = gptr;
+  wt.cb = cb; wt.ce = ce; wt.cb1 = cb1; wt.ce1 = ce1; wt.flags = wflags;
+  auto wgrad = [&](int l) {
+    const Layer& ly = h->layers[l];
+    GemmArgs g;
+    g.A = l == 0 ? h->X0T : h->HT[l - 1];
+    g.lda = h->Bp;
+    g.B = l == L - 1 ? h->LmT : h->DlT[l];
+    g.ldb = h->Bp;
+    g.K = h->Bp;
+    GemmPlan p = plan(ly.inp, ly.outp);
+    g.row_tiles = p.rt; g.col_tiles = p.ct;
+    EpiWgrad e = wt;
+    e.in = ly.in; e.out = ly.out; e.w0 = ly.w0;
+    ProfScope ps(h, PR_WGRAD, l);
+    launch_gemm_wgrad(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  };
+  // data gradients of layers L..2 (each followed by that layer's weight gradient)
+  for (int l = L - 1; l >= 1; --l) {
+    const Layer& ly = h->layers[l];
+    GemmArgs g;
+    g.A = l == L - 1 ? h->Lm : h->Dl[l];
+    g.lda = ly.outp; g.B = ly.Wc; g.ldb = ly.outp; g.K = ly.outp;
+    GemmPlan p = plan(h->Bp, ly.inp);
+    g.row_tiles = p.rt; g.col_tiles = p.ct;
+    EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1]};
+    {
+      ProfScope ps(h, PR_BWD_DATA, l);
+      launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+    }
+    wgrad(l);
+  }
+  wgrad(0);
+  launch_bias_fold(h->prec, L, (const void* const*)h->d_dT, h->d_outs, h->d_b0s, h->Bp, h->Bp,
+                   h->bias_rows, wt, h->ctl, gate, h->stream);
+  // layer-1 data gradient with the stage epilogue
+  const Layer& l0 = h->layers[0];
+  GemmArgs g;
+  g.A = L == 1 ? h->Lm : h->Dl[0];
+  g.lda = l0.outp; g.B = l0.Wc; g.ldb = l0.outp; g.K = l0.outp;
+  GemmPlan p = plan(h->Bp, h->Dp);
+  g.row_tiles = p.rt; g.col_tiles = p.ct;
+  EpiBwdLast e{};
+  e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
+  e.kl = h->kl; e.sv = (int64_t)h->Bp * h->D;
+  e.lam = h->lam; e.lam_new = h->lam_new; e.Ucur = h->ucur;
+  e.L = h->Lm; e.LT = h->LmT; e.ldl = h->Dp; e.ldlt = h->Bp;
+  e.X0 = h->X0; e.X0T = h->X0T; e.ldx = h->Dp; e.ldxt = h->Bp; e.map = h->desc.input_map;
+  e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D;
+  e.part = part; e.plain_out = plain_out;
+  ProfScope ps(h, PR_BWD_LAST, 0);
+  launch_gemm_bwdlast(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+}
+
+int vjp_last_tiles(node_handle* h) {
+  GemmPlan p = plan(h->Bp, h->Dp);
+  return p.rt * p.ct;
+}
+
+int ensure_tape(node_handle* h, int64_t entries) {
+  if (entries <= h->tape_cap) return 0;
+  int64_t cap = std::max<int64_t>(entries, std::max<int64_t>(16, 2 * h->tape_cap));
+  const int64_t ent_bytes = 8 * (int64_t)h->Bp * h->D * sizeof(float);
+  float* nt = nullptr;
+  double* nft = nullptr;
+  if (int rc = dalloc_t(h, &nt, (size_t)(cap * ent_bytes))) return rc;
+  if (int rc = dalloc_t(h, &nft, (size_t)((cap + 1) * sizeof(double)))) return rc;
+  if (h->tape) {
+    HIPCHK(h, hipMemcpyAsync(nt, h->tape, h->tape_cap * ent_bytes, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(nft, h->fwd_times, (h->tape_cap + 1) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(h, h->tape);
+    dfree(h, h->fwd_times);
+  }
+  h->tape = nt;
+  h->fwd_times = nft;
+  h->tape_cap = cap;
+  return 0;
+}
+
+int ensure_adjoint(node_handle* h) {
+  if (h->mu) return 0;
+  const int64_t sv = (int64_t)h->Bp * h->D;
+  const size_t act = (size_t)h->Bp * h->Dp * h->esz;
+  if (int rc = dalloc_t(h, &h->lam, sv * 4)) return rc;
+  if (int rc = dalloc_t(h, &h->lam_new, sv * 4)) return rc;
+  if (int rc = dalloc_t(h, &h->kl, 8 * sv * 4)) return rc;
+  if (int rc = dalloc(h, &h->Lm, act)) return rc;
+  if (int rc = dalloc(h, &h->LmT, act)) return rc;
+  if (int rc = dalloc_t(h, &h->mu, 2 * h->P * 8)) return rc;
+  if (int rc = dalloc_t(h, &h->accb, h->P * 4)) return rc;
+  if (int rc = dalloc_t(h, &h->acce, h->P * 4)) return rc;
+  if (int rc = dalloc_t(h, &h->gf0, h->P * 4)) return rc;
+  if (int rc = dalloc_t(h, &h->gf1, h->P * 4)) return rc;
+  // per-layer transposed delta pointers for the bias fold (layer L uses lambda^T)
+  std::vector<void*> dT(h->L);
+  std::vector<int> outs(h->L);
+  std::vector<int64_t> b0s(h->L);
+  int rows = 0;
+  for (int l = 0; l < h->L; ++l) {
+    dT[l] = l == h->L - 1 ? h->LmT : h->DlT[l];
+    outs[l] = h->layers[l].out;
+    b0s[l] = h->layers[l].b0;
+    rows += outs[l];
+  }
+  h->bias_rows = rows;
+  if (int rc = dalloc_t(h, &h->d_dT, h->L * sizeof(void*))) return rc;
+  if (int rc = dalloc_t(h, &h->d_outs, h->L * sizeof(int))) return rc;
+  if (int rc = dalloc_t(h, &h->d_b0s, h->L * sizeof(int64_t))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->d_dT, dT.data(), h->L * sizeof(void*), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_outs, outs.data(), h->L * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_b0s, b0s.data(), h->L * sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+AdjBufs adj_bufs(node_handle* h) {
+  AdjBufs a{};
+  a.lam = h->lam; a.lam_new = h->lam_new; a.kl = h->kl; a.U = h->ucur;
+  a.L = h->Lm; a.LT = h->LmT; a.X0 = h->X0; a.X0T = h->X0T;
+  a.ldl = h->Dp; a.ldlt = h->Bp; a.ldx = h->Dp; a.ldxt = h->Bp;
+  a.tape = h->tape; a.fwd_times = h->fwd_times; a.cb_times = h->save_times;
+  a.dLdout = h->dLdout_dev;
+  a.mu = h->mu; a.accb = h->accb; a.acce = h->acce; a.gf0 = h->gf0; a.gf1 = h->gf1;
+  a.P = h->P;
+  return a;
+}
+
+void init_ctl(node_handle* h, Ctl& c, const node_solve_opts& o, double t0, double t1) {
+  std::memset(&c, 0, sizeof(c));
+  c.t = t0;
+  c.tprev = t0;
+  c.tdir = t1 > t0 ? 1.0 : -1.0;
+  c.t_end = t1;
+  const double span = std::fabs(t1 - t0);
+  c.dtmax_abs = o.dtmax > 0 ? o.dtmax : span;
+  auto spacing = [](double x) { x = std::fabs(x); return std::nextafter(x, INFINITY) - x; };
+  c.dtmin = o.dtmin > 0 ? o.dtmin : std::max(spacing(t0), spacing(t1));
+  c.abstol = o.abstol > 0 ? o.abstol : 1e-6;
+  c.reltol = o.reltol > 0 ? o.reltol : 1e-3;
+  c.adaptive = o.adaptive ? 1 : 0;
+  c.dt_fixed = std::fabs(o.dt);
+  c.has_dt_ = (!o.adaptive || o.dt != 0.0) ? 1 : 0;
+  c.dt = c.has_dt_ ? c.tdir * std::fabs(o.dt) : 0.0;
+  c.qold = 1e-4;
+  c.q11 = 1.0;
+  c.maxiters = o.maxiters > 0 ? o.maxiters : 100000;
+  c.nf = 1;
+  c.jump_idx = -1;
+  (void)h;
+}
+
+int poll_ctl(node_handle* h) {
+  HIPCHK(h, hipMemcpyAsync(h->hctl, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int upload_save_times(node_handle* h, const std::vector<double>& ts) {
+  if ((int64_t)ts.size() + 1 > h->save_cap) {
+    dfree(h, h->save_times);
+    h->save_cap = std::max<int64_t>(ts.size() + 1, 64);
+    if (int rc = dalloc_t(h, &h->save_times, h->save_cap * sizeof(double))) return rc;
+  }
+  if (!ts.empty())
+    HIPCHK(h, hipMemcpyAsync(h->save_times, ts.data(), ts.size() * sizeof(double),
+                             hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+int ensure_out(node_handle* h, int64_t n) {
+  const int64_t need = std::max<int64_t>(n, 1) * h->B * h->D;
+  if (need <= h->out_dev_cap) return 0;
+  dfree(h, h->out_dev);
+  h->out_dev = nullptr;
+  if (int rc = dalloc_t(h, &h->out_dev, need * sizeof(float))) return rc;
+  h->out_dev_cap = need;
+  return 0;
+}
+
+// ---------------------------------------------------------------- forward
+int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
+                 const double* saveat, int64_t n_saveat, node_stats* st) {
+  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  if (!o) return fail(h, NODE_ERR_INVALID, "opts is NULL");
+  if (!(o->t1 != o->t0) || !std::isfinite(o->t0) || !std::isfinite(o->t1))
+    return fail(h, NODE_ERR_INVALID, "tspan must be finite with t1 != t0");
+  if (!o->adaptive && !(o->dt != 0.0)) return fail(h, NODE_ERR_INVALID, "fixed-step solve needs dt");
+  if (o->save_everystep && n_saveat > 0)
+    return fail(h, NODE_ERR_UNSUPPORTED, "save_everystep together with saveat is not supported");
+  const double tdir = o->t1 > o->t0 ? 1.0 : -1.0;
+  std::vector<double> ts;
+  if (o->save_start) ts.push_back(o->t0);
+  for (int64_t i = 0; i < n_saveat; ++i) {
+    const double s = saveat[i];
+    if (!(tdir * s > tdir * o->t0 && tdir * s < tdir * o->t1))
+      return fail(h, NODE_ERR_INVALID, "saveat entries must lie strictly inside tspan");
+    if (i > 0 && !(tdir * s > tdir * saveat[i - 1]))
+      return fail(h, NODE_ERR_INVALID, "saveat must be strictly monotone in the direction of time");
+    ts.push_back(s);
+  }
+  if (o->save_end && !o->save_everystep) ts.push_back(o->t1);
+  h->everystep = o->save_everystep != 0;
+  if (int rc = upload_save_times(h, ts)) return rc;
+  if (int rc = ensure_out(h, (int64_t)ts.size())) return rc;
+  if (int rc = ensure_tape(h, 16)) return rc;
+
+  const StateGeom sg = geom(h);
+  Ctl c;
+  init_ctl(h, c, *o, o->t0, o->t1);
+  c.n_save = h->everystep ? 0 : (int64_t)ts.size();
+  HIPCHK(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->fwd_times, &o->t0, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  double* p0 = h->part;
+  double* px = h->part + SEC;
+  const int nb = state_blocks(sg);
+  launch_fwd_init(sg, d_u0, h->tape, h->X0, h->ctl, p0, nb, h->stream);
+  rhs_eval(h, 0, SM_INIT_F0, GATE_ALWAYS, px);
+  const int nx = rhs_last_tiles(h);
+  const double nel = (double)h->B * h->D;
+  if (!c.has_dt_) {
+    launch_initdt_a(h->ctl, p0, nb, px, nx, nullptr, 0, nel, h->stream);
+    launch_fwd_initdt_prep(sg, h->tape, h->X0, h->ctl, h->stream);
+    rhs_eval(h, 0, SM_INIT_F1, GATE_ALWAYS, px);
+  }
+  launch_initdt_b(h->ctl, px, nx, nullptr, 0, nel, h->save_times, 0, h->fwd_times, h->stream);
+  HIPCHK(h, hipGetLastError());
+
+  int64_t chunk = 2, enqueued = 0;
+  for (;;) {
+    if (int rc = poll_ctl(h)) return rc;
+    if (h->hctl->done) break;
+    if (enqueued > c.maxiters + 64)
+      return fail(h, NODE_ERR_STATE, "step controller did not terminate (internal error)");
+    enqueued += chunk;
+    if (int rc = ensure_tape(h, h->hctl->n + chunk + 2)) return rc;
+    for (int64_t k = 0; k < chunk; ++k) {
+      launch_fwd_step_begin(sg, h->tape, h->X0, h->save_times, h->out_dev, h->ctl, h->stream);
+      for (int s = 1; s < 7; ++s) rhs_eval(h, s, SM_STEP, GATE_ACTIVE, px);
+      launch_fwd_controller(h->ctl, px, nx, h->save_times, h->fwd_times, nel, h->stream);
+    }
+    HIPCHK(h, hipGetLastError());
+    chunk = std::min<int64_t>(chunk * 2, 16);
+  }
+  // pending saves of the final step
+  launch_fwd_step_begin(sg, h->tape, h->X0, h->save_times, h->out_dev, h->ctl, h->stream);
+  if (int rc = poll_ctl(h)) return rc;
+  const Ctl& r = *h->hctl;
+  h->n_fwd = r.n;
+  if (h->everystep) {
+    // every accepted step: states are tape[j].uprev, j = 0..n
+    std::vector<double> ft(r.n + 1);
+    HIPCHK(h, hipMemcpy(ft.data(), h->fwd_times, (r.n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    ts.clear();
+    int64_t j0 = o->save_start ? 0 : 1;
+    for (int64_t j = j0; j <= r.n; ++j) ts.push_back(ft[j]);
+    if (int rc = ensure_out(h, (int64_t)ts.size())) return rc;
+    launch_gather(sg, h->tape + j0 * sg.ent, sg.ent, h->out_dev, (int64_t)ts.size(), h->stream);
+    if (int rc = upload_save_times(h, ts)) return rc;
+    h->nsaved = (int64_t)ts.size();
+  } else {
+    h->nsaved = r.save_next;
+  }
+  ts.resize(h->nsaved);
+  h->h_save = ts;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipGetLastError());
+  node_stats s{};
+  s.nf = r.nf; s.naccept = r.naccept; s.nreject = r.nreject; s.nsaved = h->nsaved;
+  s.retcode = r.retcode; s.t_final = r.t;
+  h->fwd_stats = s;
+  h->fwd_opts = *o;
+  h->have_forward = true;
+  if (st) *st = s;
+  return 0;
+}
+
+// ---------------------------------------------------------------- backward
+int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
+  if (!h->have_forward) return fail(h, NODE_ERR_STATE, "no forward solve to differentiate");
+  if (h->fwd_stats.retcode != NODE_RET_SUCCESS)
+    return fail(h, NODE_ERR_STATE, "forward solve did not succeed (retcode != Success)");
+  if (!(h->fwd_opts.t1 > h->fwd_opts.t0))
+    return fail(h, NODE_ERR_UNSUPPORTED, "adjoint needs a forward solve with t1 > t0");
+  if (int rc = ensure_adjoint(h)) return rc;
+  const node_solve_opts& o = h->fwd_opts;
+  const StateGeom sg = geom(h);
+  const int64_t ncb = (int64_t)h->h_save.size();
+  Ctl c;
+  init_ctl(h, c, o, o.t1, o.t0);
+  c.n_fwd = h->n_fwd;
+  c.n_save = ncb;
+  int64_t next = ncb - 1;
+  if (next >= 0 && h->h_save[next] == o.t1) { c.jump_idx = (int)next; --next; }
+  c.save_next = next;
+  HIPCHK(h, hipMemsetAsync(h->mu, 0, 2 * h->P * sizeof(double), h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+  AdjBufs ab = adj_bufs(h);
+  ab.dLdout = d_dLdout;
+  double* p0 = h->part;
+  double* px = h->part + SEC;
+  double* pm = h->part + 2 * SEC;
+  const int nb = state_blocks(sg);
+  const int nmb = mu_blocks(h->P);
+  const int nx = vjp_last_tiles(h);
+  const double nel = (double)h->B * h->D + (double)h->P;
+
+  launch_set_stage_interp(h->ctl, 0, o.t1, h->fwd_times, h->stream);
+  launch_adj_init(sg, ab, h->ctl, p0, h->stream);
+  vjp_eval(h, 0, SM_INIT_F0, GATE_ALWAYS, WG_STORE_CUR, 0, 0, 0, 0, px, nullptr, nullptr);
+  launch_mu_norm(1, ab, h->ctl, pm, nmb, GATE_ALWAYS, h->stream);
+  if (!c.has_dt_) {
+    launch_initdt_a(h->ctl, p0, nb, px, nx, pm, nmb, nel, h->stream);
+    launch_set_stage_interp(h->ctl, 7, 0.0, h->fwd_times, h->stream);
+    launch_adj_initdt_prep(sg, ab, h->ctl, h->stream);
+    vjp_eval(h, 0, SM_INIT_F1, GATE_ALWAYS, WG_STORE_PTR, 0, 0, 0, 0, px, nullptr, h->accb);
+    launch_mu_norm(2, ab, h->ctl, pm, nmb, GATE_ALWAYS, h->stream);
+  }
+  launch_initdt_b(h->ctl, px, nx, pm, nmb, nel, h->save_times, 1, h->fwd_times, h->stream);
+  HIPCHK(h, hipGetLastError());
+
+  int64_t chunk = 2, enqueued = 0;
+  for (;;) {
+    if (int rc = poll_ctl(h)) return rc;
+    if (h->hctl->done) break;
+    if (enqueued > c.maxiters + 64)
+      return fail(h, NODE_ERR_STATE, "adjoint step controller did not terminate (internal error)");
+    enqueued += chunk;
+    for (int64_t k = 0; k < chunk; ++k) {
+      launch_adj_step_begin(sg, ab, h->ctl, GATE_ALWAYS, h->stream);
+      vjp_eval(h, 0, SM_STEP, GATE_FSAL, WG_STORE_CUR, 0, 0, 0, 0, px, nullptr, nullptr);
+      for (int s = 1; s < 7; ++s) {
+        int fl;
+        float cb = (float)kB[s], ce = (float)kBT[s], cb1 = 0, ce1 = 0;
+        if (s == 1) { fl = WG_INIT; cb1 = (float)kB[0]; ce1 = (float)kBT[0]; }
+        else if (s < 6) fl = WG_B | WG_E;
+        else fl = WG_E | WG_STORE_NEXT;
+        vjp_eval(h, s, SM_STEP, GATE_ACTIVE, fl, cb, ce, cb1, ce1, px, nullptr, nullptr);
+      }
+      launch_mu_norm(0, ab, h->ctl, pm, nmb, GATE_ACTIVE, h->stream);
+      launch_adj_controller(h->ctl, px, nx, pm, nmb, nel, h->save_times, h->fwd_times, h->stream);
+    }
+    HIPCHK(h, hipGetLastError());
+    chunk = std::min<int64_t>(chunk * 2, 16);
+  }
+  launch_adj_step_begin(sg, ab, h->ctl, GATE_ALWAYS, h->stream);
+  if (int rc = poll_ctl(h)) return rc;
+  HIPCHK(h, hipGetLastError());
+  const Ctl& r = *h->hctl;
+  if (st) {
+    node_stats s{};
+    s.nf = r.nf; s.naccept = r.naccept; s.nreject = r.nreject; s.nsaved = 0;
+    s.retcode = r.retcode; s.t_final = r.t;
+    *st = s;
+  }
+  return 0;
+}
+
+int copy_grads(node_handle* h, float* dLdu0, double* dLdp, hipMemcpyKind kind) {
+  // lambda real rows are the first B*D floats (row stride D)
+  if (dLdu0)
+    HIPCHK(h, hipMemcpyAsync(dLdu0, h->lam, (size_t)h->B * h->D * sizeof(float), kind, h->stream));
+  if (dLdp) {
+    const double* mu = h->hctl->gsel_mu_ ? h->mu + h->P : h->mu;
+    HIPCHK(h, hipMemcpyAsync(dLdp, mu, (size_t)h->P * sizeof(double), kind, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+int32_t node_abi_version(void) { return NODE_HIP_ABI_VERSION; }
+
+int64_t node_num_params(const node_desc* d) {
+  std::string m;
+  if (validate_desc(d, m)) return -1;
+  int64_t p = 0;
+  for (int l = 0; l < d->n_layers; ++l)
+    p += (int64_t)d->widths[l] * d->widths[l + 1] + d->widths[l + 1];
+  return p;
+}
+
+const char* node_last_error(const node_handle_t* h) { return h ? h->err.c_str() : "null handle"; }
+
+int32_t node_create(const node_desc* desc, node_handle_t** out) {
+  if (!out) return NODE_ERR_INVALID;
+  *out = nullptr;
+  std::string msg;
+  if (int rc = validate_desc(desc, msg)) { std::fprintf(stderr, "node_create: %s\n", msg.c_str()); return rc; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    std::fprintf(stderr, "node_create: no HIP device\n");
+    return NODE_ERR_HIP;
+  }
+  node_handle* h = new node_handle();
+  h->desc = *desc;
+  h->L = desc->n_layers;
+  h->D = desc->widths[0];
+  h->Dp = (int)rup(h->D, KPAD);
+  h->B = (int)desc->batch;
+  h->Bp = (int)rup(h->B, BPAD);
+  h->prec = desc->precision;
+  h->esz = h->prec == NODE_PREC_BF16 ? 2 : 4;
+  h->P = node_num_params(desc);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return NODE_ERR_HIP;
+  }
+  upload_tableau();
+  int64_t off = 0;
+  int rc = 0;
+  for (int l = 0; l < h->L && !rc; ++l) {
+    Layer ly;
+    ly.in = desc->widths[l];
+    ly.out = desc->widths[l + 1];
+    ly.inp = (int)rup(ly.in, KPAD);
+    ly.outp = (int)rup(ly.out, KPAD);
+    ly.act = desc->acts[l];
+    ly.w0 = off;
+    ly.b0 = off + (int64_t)ly.in * ly.out;
+    off = ly.b0 + ly.out;
+    rc = dalloc(h, &ly.Wr, (size_t)ly.outp * ly.inp * h->esz);
+    if (!rc) rc = dalloc(h, &ly.Wc, (size_t)ly.outp * ly.inp * h->esz);
+    if (!rc) rc = dalloc_t(h, &ly.bias, (size_t)ly.outp * sizeof(float));
+    h->layers.push_back(ly);
+  }
+  const size_t x0b = (size_t)h->Bp * h->Dp * h->esz;
+  if (!rc) rc = dalloc(h, &h->X0, x0b);
+  if (!rc) rc = dalloc(h, &h->X0T, x0b);
+  for (int l = 0; l < h->L - 1 && !rc; ++l) {
+    const size_t hb = (size_t)h->Bp * h->layers[l].outp * h->esz;
+    void *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
+    rc = dalloc(h, &a, hb);
+    if (!rc) rc = dalloc(h, &b, hb);
+    if (!rc) rc = dalloc(h, &c, hb);
+    if (!rc) rc = dalloc(h, &d, hb);
+    if (!rc) rc = dalloc(h, &e, hb);
+    h->H.push_back(a); h->HT.push_back(b); h->S.push_back(c); h->Dl.push_back(d); h->DlT.push_back(e);
+  }
+  if (!rc) rc = dalloc_t(h, &h->ctl, sizeof(Ctl));
+  if (!rc) rc = dalloc_t(h, &h->part, 4 * SEC * sizeof(double));
+  if (!rc) rc = dalloc_t(h, &h->scratch, (size_t)h->Bp * h->D * sizeof(float));
+  if (!rc) rc = dalloc_t(h, &h->ucur, (size_t)h->Bp * h->D * sizeof(float));
+  if (!rc && hipHostMalloc((void**)&h->hctl, sizeof(Ctl), 0) != hipSuccess) rc = NODE_ERR_HIP;
+  if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = NODE_ERR_HIP;
+  if (rc) {
+    std::fprintf(stderr, "node_create: %s\n", h->err.c_str());
+    node_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int32_t node_destroy(node_handle_t* h) {
+  if (!h) return 0;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  for (auto& r : h->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  if (h->hctl) (void)hipHostFree(h->hctl);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+static int set_params_host(node_handle* h, const float* p, int64_t n) {
+  if (!p) return fail(h, NODE_ERR_INVALID, "params is NULL");
+  if (n != h->P) return fail(h, NODE_ERR_INVALID, "parameter count mismatch: expected " + std::to_string(h->P));
+  for (const Layer& ly : h->layers) {
+    const size_t cnt = (size_t)ly.outp * ly.inp;
+    std::vector<float> wr(cnt, 0.f), wc(cnt, 0.f), b(ly.outp, 0.f);
+    for (int i = 0; i < ly.in; ++i)
+      for (int o = 0; o < ly.out; ++o) {
+        const float w = p[ly.w0 + (int64_t)i * ly.out + o];  // Julia W[o, i]
+        wr[(size_t)o * ly.inp + i] = w;
+        wc[(size_t)i * ly.outp + o] = w;
+      }
+    for (int o = 0; o < ly.out; ++o) b[o] = p[ly.b0 + o];
+    if (h->prec == NODE_PREC_BF16) {
+      std::vector<uint16_t> r16(cnt), c16(cnt);
+      auto tobf = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+        return (uint16_t)(u >> 16);
+      };
+      for (size_t k = 0; k < cnt; ++k) { r16[k] = tobf(wr[k]); c16[k] = tobf(wc[k]); }
+      HIPCHK(h, hipMemcpy(ly.Wr, r16.data(), cnt * 2, hipMemcpyHostToDevice));
+      HIPCHK(h, hipMemcpy(ly.Wc, c16.data(), cnt * 2, hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(h, hipMemcpy(ly.Wr, wr.data(), cnt * 4, hipMemcpyHostToDevice));
+      HIPCHK(h, hipMemcpy(ly.Wc, wc.data(), cnt * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(h, hipMemcpy(ly.bias, b.data(), ly.outp * 4, hipMemcpyHostToDevice));
+  }
+  h->have_params = true;
+  h->have_forward = false;
+  return 0;
+}
+
+int32_t node_set_params(node_handle_t* h, const float* p, int64_t n) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return set_params_host(h, p, n);
+}
+
+int32_t node_set_params_device(node_handle_t* h, const float* d_p, int64_t n) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (n != h->P) return fail(h, NODE_ERR_INVALID, "parameter count mismatch");
+  std::vector<float> p(n);
+  HIPCHK(h, hipMemcpy(p.data(), d_p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return set_params_host(h, p.data(), n);
+}
+
+int32_t node_forward_device(node_handle_t* h, const node_solve_opts* opts, const float* d_u0,
+                            const double* saveat, int64_t n_saveat, float* d_out,
+                            int64_t out_cap, node_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!d_u0) return fail(h, NODE_ERR_INVALID, "u0 is NULL");
+  if (n_saveat < 0 || (n_saveat > 0 && !saveat)) return fail(h, NODE_ERR_INVALID, "bad saveat");
+  int rc = forward_impl(h, opts, d_u0, saveat, n_saveat, stats);
+  if (rc) return rc;
+  if (d_out) {
+    if (h->nsaved > out_cap) return fail(h, NODE_ERR_CAPACITY, "out_cap smaller than the number of saved states");
+    HIPCHK(h, hipMemcpyAsync(d_out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float),
+                             hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
+int32_t node_forward(node_handle_t* h, const node_solve_opts* opts, const float* u0,
+                     const double* saveat, int64_t n_saveat, float* out, int64_t out_cap,
+                     node_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!u0) return fail(h, NODE_ERR_INVALID, "u0 is NULL");
+  if (n_saveat < 0 || (n_saveat > 0 && !saveat)) return fail(h, NODE_ERR_INVALID, "bad saveat");
+  float* du0 = h->scratch;  // u0 staged through the scratch state buffer
+  HIPCHK(h, hipMemcpyAsync(du0, u0, (size_t)h->B * h->D * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  int rc = forward_impl(h, opts, du0, saveat, n_saveat, stats);
+  if (rc) return rc;
+  if (out) {
+    if (h->nsaved > out_cap) return fail(h, NODE_ERR_CAPACITY, "out_cap smaller than the number of saved states");
+    HIPCHK(h, hipMemcpy(out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int32_t node_saved_times(node_handle_t* h, double* t_out, int64_t cap) {
+  if (!h || !t_out) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->have_forward) return fail(h, NODE_ERR_STATE, "no forward solve");
+  if (cap < (int64_t)h->h_save.size()) return fail(h, NODE_ERR_CAPACITY, "cap too small");
+  std::copy(h->h_save.begin(), h->h_save.end(), t_out);
+  return 0;
+}
+
+int32_t node_copy_saved(node_handle_t* h, float* out, int64_t cap) {
+  if (!h || !out) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->have_forward) return fail(h, NODE_ERR_STATE, "no forward solve");
+  if (cap < h->nsaved) return fail(h, NODE_ERR_CAPACITY, "cap too small");
+  HIPCHK(h, hipMemcpy(out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int32_t node_backward_device(node_handle_t* h, const float* d_dLdout, float* d_dLdu0,
+                             double* d_dLdp, node_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!d_dLdout) return fail(h, NODE_ERR_INVALID, "dLdout is NULL");
+  int rc = backward_impl(h, d_dLdout, stats);
+  if (rc) return rc;
+  return copy_grads(h, d_dLdu0, d_dLdp, hipMemcpyDeviceToDevice);
+}
+
+int32_t node_backward(node_handle_t* h, const float* dLdout, float* dLdu0, double* dLdp,
+                      node_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!dLdout) return fail(h, NODE_ERR_INVALID, "dLdout is NULL");
+  if (!h->have_forward) return fail(h, NODE_ERR_STATE, "no forward solve to differentiate");
+  const int64_t n = std::max<int64_t>(h->nsaved, 1) * h->B * h->D;
+  if (n > h->dLdout_cap) {
+    dfree(h, h->dLdout_dev);
+    h->dLdout_dev = nullptr;
+    if (int rc = dalloc_t(h, &h->dLdout_dev, n * sizeof(float))) return rc;
+    h->dLdout_cap = n;
+  }
+  HIPCHK(h, hipMemcpyAsync(h->dLdout_dev, dLdout, (size_t)h->nsaved * h->B * h->D * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+  int rc = backward_impl(h, h->dLdout_dev, stats);
+  if (rc) return rc;
+  return copy_grads(h, dLdu0, dLdp, hipMemcpyDeviceToHost);
+}
+
+int32_t node_rhs(node_handle_t* h, const float* u, float* du) {
+  if (!h || !u || !du) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  const StateGeom sg = geom(h);
+  HIPCHK(h, hipMemcpyAsync(h->ucur, u, (size_t)h->B * h->D * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  launch_prep_plain(sg, h->ucur, h->X0, nullptr, nullptr, h->Dp, h->Bp, h->stream);
+  rhs_eval(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(du, h->scratch, (size_t)h->B * h->D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda, float* du, double* dp) {
+  if (!h || !u || !lambda || !du || !dp) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  if (int rc = ensure_adjoint(h)) return rc;
+  const StateGeom sg = geom(h);
+  const size_t sb = (size_t)h->B * h->D * sizeof(float);
+  HIPCHK(h, hipMemcpyAsync(h->ucur, u, sb, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->lam, lambda, sb, hipMemcpyHostToDevice, h->stream));
+  launch_prep_plain(sg, h->ucur, h->X0, h->X0T, nullptr, h->Dp, h->Bp, h->stream);
+  launch_prep_lambda(sg, h->lam, h->Lm, h->LmT, h->Dp, h->Bp, h->stream);
+  if (!h->dp32 && dalloc_t(h, &h->dp32, h->P * sizeof(float))) return NODE_ERR_HIP;
+  vjp_eval(h, 0, SM_PLAIN, GATE_ALWAYS, WG_STORE_PTR, 0, 0, 0, 0, nullptr, h->scratch, h->dp32);
+  HIPCHK(h, hipGetLastError());
+  std::vector<float> p32(h->P);
+  HIPCHK(h, hipMemcpyAsync(du, h->scratch, sb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(p32.data(), h->dp32, h->P * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int64_t i = 0; i < h->P; ++i) dp[i] = -(double)p32[i];  // epilogue stores -g
+  return 0;
+}
+
+int32_t node_prof_enable(node_handle_t* h, uint64_t key_mask) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  h->prof_mask = key_mask;
+  return 0;
+}
+
+int32_t node_prof_collect(node_handle_t* h, int32_t* keys, float* ms, int64_t cap,
+                          int64_t* n_out) {
+  if (!h || !n_out) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int64_t n = 0;
+  for (auto& r : h->prof) {
+    float t = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&t, r.a, r.b));
+    if (n < cap && keys && ms) { keys[n] = r.key; ms[n] = t; }
+    ++n;
+    h->ev_pool.push_back(r.a);
+    h->ev_pool.push_back(r.b);
+  }
+  h->prof.clear();
+  *n_out = n;
+  return 0;
+}
+
+}  // extern "C"

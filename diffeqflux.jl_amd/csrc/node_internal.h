@@ -1,0 +1,177 @@
+// Internal declarations shared by the device kernels (kernels.hip) and the
+// host driver (node_hip.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nodehip {
+
+constexpr int MAXL = 8;
+constexpr int NSTAGE = 7;
+constexpr int MAX_PART = 8192;   // partial-sum slots per section
+
+// Gates: a kernel launched in a speculative step returns at once when its
+// gate does not hold (the solve is finished, or no FSAL re-evaluation).
+enum Gate { GATE_ALWAYS = 0, GATE_ACTIVE = 1, GATE_FSAL = 2 };
+
+// Epilogue modes of the last forward layer (k = model(u))
+enum StageMode { SM_STEP = 0, SM_INIT_F0 = 1, SM_INIT_F1 = 2, SM_PLAIN = 3 };
+
+// Device-resident solver control block.  FP64 time / step / controller.
+struct Ctl {
+  double t, dt, tprev, dtpropose, qold, q11, eest;
+  double tdir, dtmax_abs, dtmin, abstol, reltol, t_end;
+  double dt0, d0, d1;          // initial-dt heuristic scratch
+  double dt_step;              // dt of the last accepted step (saves / mu commit)
+  double dt_fixed;
+  int64_t n;                   // accepted steps = tape index of the current step
+  int64_t iter, naccept, nreject, nf, maxiters;
+  int64_t n_save;              // number of save (forward) / callback (adjoint) times
+  int64_t save_next;           // forward: next save index; adjoint: next callback (descending)
+  int64_t save_lo, save_hi;    // forward saves pending at the next step_begin
+  int64_t save_step;           // tape index the pending saves interpolate in
+  int64_t n_fwd;               // adjoint: number of forward steps on the tape
+  int32_t done, retcode, accepted, fsal_reeval, adaptive, last_accept;
+  int32_t jump_idx;            // adjoint: cotangent index to add at step_begin (-1 none)
+  int32_t gsel;                // adjoint: current FSAL parameter-gradient buffer
+  int32_t commit_mu;           // adjoint: step_begin adds dt_step*acc_b to mu
+  int32_t pad_;                // initial-dt heuristic returned early (small dt)
+  int32_t has_dt_;             // dt given by the caller: skip the heuristic
+  int32_t gsel_mu_;            // adjoint: current mu buffer (0/1)
+  // adjoint: forward-interpolation data for each stage time of the next
+  // attempt (slots 0..6) and for the initial-dt probe (slot 7)
+  int32_t st_step[8];
+  int32_t st_exact[8];         // 1: exact stored state tape[st_step].uprev
+  float st_dt[8];
+  float st_w[8][7];
+};
+
+struct GemmArgs {
+  const void* A; int64_t lda;  // rows x K, K contiguous
+  const void* B; int64_t ldb;  // cols x K, K contiguous
+  int K;                       // multiple of the K tile
+  int row_tiles, col_tiles;
+};
+
+// hidden layer forward: Y = act(acc + bias) (+ transposed copy, + act')
+struct EpiHidden {
+  const float* bias; int act; int C; int ldo; int ldt;
+  void* Y; void* YT; void* S;
+};
+
+// last forward layer: k_s = acc + bias, then the next Tsit5 stage input
+struct EpiFwdStage {
+  const float* bias; int act; int D; int B; int stage; int mode;
+  float* tape; int64_t ent; int64_t sv;   // floats per tape entry / per state
+  void* X0; int ldx; int map;
+  float* scratch;                          // initdt f1 / plain rhs output
+  double* part;
+};
+
+// hidden data gradient: delta = acc * act'(z)  (row-major + transposed)
+struct EpiBwdData {
+  int C; int ldo; int ldt; const void* S; void* Dl; void* DlT;
+};
+
+// layer-1 data gradient: k^lambda_s = -(acc * map'(u)), next stage inputs
+struct EpiBwdLast {
+  int D; int B; int stage; int mode;
+  float* kl; int64_t sv;                   // 8 stage slots of B_pad x D floats
+  float* lam; float* lam_new; float* Ucur;
+  void* L; void* LT; int ldl; int ldlt;    // lambda stage input (row-major + transposed)
+  void* X0; void* X0T; int ldx; int ldxt; int map;
+  const float* tape; int64_t ent;
+  double* part;
+  float* plain_out;                        // node_vjp: write -k (= lambda^T df/du)
+};
+
+// weight gradient (K = batch): fold into the mu accumulators
+enum WgradFlags { WG_INIT = 1, WG_B = 2, WG_E = 4, WG_STORE_CUR = 8,
+                  WG_STORE_NEXT = 16, WG_STORE_PTR = 32 };
+struct EpiWgrad {
+  int in, out; int64_t w0;
+  float* accb; float* acce; float* gf0; float* gf1; float* gptr;
+  float cb, ce, cb1, ce1; int flags;
+};
+
+// tableau in constant memory
+void upload_tableau();
+
+// ---- launchers (kernels.hip) ----
+template <typename T> struct TypeTag {};
+
+void launch_gemm_hidden(int prec, int cfg, const GemmArgs& g, const EpiHidden& e,
+                        const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_fwdstage(int prec, int cfg, const GemmArgs& g, const EpiFwdStage& e,
+                          const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_bwddata(int prec, int cfg, const GemmArgs& g, const EpiBwdData& e,
+                         const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_bwdlast(int prec, int cfg, const GemmArgs& g, const EpiBwdLast& e,
+                         const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
+                       const Ctl* ctl, int gate, hipStream_t s);
+int gemm_cfg_tile_rows(int cfg);
+int gemm_cfg_tile_cols(int cfg);
+int gemm_cfg_count();
+
+// elementwise / control kernels
+struct StateGeom {
+  int B, Bp, D, Dp;          // real / padded batch, state dim, padded state dim
+  int map, prec;
+  int64_t sv;                // floats per state vector (Bp * D)
+  int64_t ent;               // floats per tape entry (8 * sv)
+};
+
+void launch_fwd_init(const StateGeom& sg, const float* u0, float* tape, void* X0,
+                     const Ctl* ctl, double* part, int nblk, hipStream_t s);
+void launch_initdt_a(Ctl* ctl, const double* p0, int n0, const double* px, int nx,
+                     const double* pm, int nm, double n_elems, hipStream_t s);
+void launch_initdt_b(Ctl* ctl, const double* px, int nx, const double* pm, int nm,
+                     double n_elems, const double* save_times, int is_adjoint,
+                     const double* fwd_times, hipStream_t s);
+void launch_fwd_initdt_prep(const StateGeom& sg, const float* tape, void* X0,
+                            const Ctl* ctl, hipStream_t s);
+void launch_fwd_step_begin(const StateGeom& sg, const float* tape, void* X0,
+                           const double* save_times, float* out, const Ctl* ctl,
+                           hipStream_t s);
+void launch_fwd_controller(Ctl* ctl, const double* part, int n_part,
+                           const double* save_times, double* fwd_times, double n_elems,
+                           hipStream_t s);
+int state_blocks(const StateGeom& sg);
+
+// adjoint
+struct AdjBufs {
+  float* lam; float* lam_new; float* kl; float* U;
+  void* L; void* LT; void* X0; void* X0T;
+  int ldl, ldlt, ldx, ldxt;
+  const float* tape; const double* fwd_times;
+  const double* cb_times; const float* dLdout;
+  double* mu; float* accb; float* acce; float* gf0; float* gf1;
+  int64_t P;
+};
+void launch_adj_init(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl,
+                     double* part, hipStream_t s);
+void launch_adj_initdt_prep(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl,
+                            hipStream_t s);
+void launch_adj_step_begin(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl,
+                           int gate, hipStream_t s);
+void launch_mu_norm(int mode, const AdjBufs& ab, const Ctl* ctl, double* part,
+                    int nblk, int gate, hipStream_t s);
+int mu_blocks(int64_t P);
+void launch_adj_controller(Ctl* ctl, const double* part, int n_lam,
+                           const double* part_mu, int n_mu, double n_elems,
+                           const double* cb_times, const double* fwd_times,
+                           hipStream_t s);
+void launch_bias_fold(int prec, int nl, const void* const* dT, const int* outs,
+                      const int64_t* b0s, int ldt, int Bp, int rows_total,
+                      const EpiWgrad& tmpl, const Ctl* ctl, int gate, hipStream_t s);
+void launch_set_stage_interp(Ctl* ctl, int slot, double tau, const double* fwd_times,
+                             hipStream_t s);
+void launch_prep_plain(const StateGeom& sg, const float* u, void* X0, void* X0T,
+                       float* Ucur, int ldx, int ldxt, hipStream_t s);
+void launch_prep_lambda(const StateGeom& sg, const float* lam, void* L, void* LT,
+                        int ldl, int ldlt, hipStream_t s);
+void launch_gather(const StateGeom& sg, const float* src, int64_t src_stride_floats,
+                   float* dst, int64_t n, hipStream_t s);
+
+}  // namespace nodehip

@@ -1,0 +1,164 @@
+/*
+ * node_hip.h -- C ABI of the MI355X (gfx950) NeuralODE hot path.
+ *
+ * Replaces, for supported models, what the reference reaches through
+ *
+ *   (n::NeuralODE)(x, p, st)                      src/neural_de.jl:47-61
+ *     -> solve(prob, Tsit5(); sensealg = InterpolatingAdjoint(
+ *              autojacvec = ZygoteVJP()), kwargs...)  src/neural_de.jl:55-58
+ *
+ * i.e. the OrdinaryDiffEq Tsit5 forward solve (tableau, embedded error, PI
+ * controller, initial dt, tstops, dense output, saveat) and the
+ * SciMLSensitivity InterpolatingAdjoint backward solve with the Lux Dense
+ * chain as right-hand side (model built at src/neural_de.jl:48-50).
+ *
+ * Conventions
+ *  - State arrays use Julia's column-major D x B layout (one contiguous
+ *    column of D values per trajectory); saved states are D x B x Nt.
+ *  - Parameters are the flat Lux/ComponentArray vector: per Dense layer the
+ *    out x in column-major weight, then the bias
+ *    (test/BasicNeuralDE/neural_de_tests.jl:45).
+ *  - Time, step size and controller state are FP64; the returned parameter
+ *    gradient is FP64 (the batch reduction of dL/dp is accumulated in FP64).
+ *  - Every entry point returns 0 on success or a negative NODE_ERR_* code;
+ *    node_last_error() gives the message.  Solver outcomes (MaxIters,
+ *    DtLessThanMin, Unstable, DtNaN) are NOT call failures: they come back in
+ *    node_stats.retcode with SciMLBase.ReturnCode numbering.
+ *  - A handle is single-threaded (one HIP stream per handle); calls are
+ *    synchronous with respect to the host.  Distinct handles are independent.
+ *  - The library never calls back into the caller.
+ */
+#ifndef NODE_HIP_H
+#define NODE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NODE_HIP_ABI_VERSION 1
+#define NODE_HIP_MAX_LAYERS 8
+
+/* activations of a Dense layer (NNlib names) */
+enum { NODE_ACT_IDENTITY = 0, NODE_ACT_TANH = 1, NODE_ACT_SOFTPLUS = 2,
+       NODE_ACT_RELU = 3, NODE_ACT_SIGMOID = 4 };
+/* parameter-free first layer: identity or x -> x.^3 (WrappedFunction,
+ * docs/src/examples/neural_ode.md:32) */
+enum { NODE_MAP_IDENTITY = 0, NODE_MAP_CUBE = 1 };
+/* GEMM precision of the right-hand side and its VJPs.  State, stage sums and
+ * adjoint state stay FP32 in both; BF16 runs the GEMMs on BF16 MFMA with FP32
+ * accumulation. */
+enum { NODE_PREC_FP32 = 0, NODE_PREC_BF16 = 1 };
+
+/* SciMLBase.ReturnCode numbering */
+enum { NODE_RET_DEFAULT = 0, NODE_RET_SUCCESS = 1, NODE_RET_TERMINATED = 2,
+       NODE_RET_DTNAN = 3, NODE_RET_MAXITERS = 4, NODE_RET_DTLESSTHANMIN = 5,
+       NODE_RET_UNSTABLE = 6 };
+
+/* call failures */
+enum { NODE_OK = 0, NODE_ERR_INVALID = -1, NODE_ERR_HIP = -2,
+       NODE_ERR_STATE = -3, NODE_ERR_CAPACITY = -4, NODE_ERR_UNSUPPORTED = -5 };
+
+typedef struct node_handle node_handle_t;
+
+/* Model + batch description.  Mirrors a Lux Chain:
+ *   Chain([x -> x.^3,] Dense(w0 => w1, act0), ..., Dense(w_{L-1} => w_L, act_{L-1}))
+ * with widths[0] == widths[n_layers] == D (the ODE state dimension). */
+typedef struct node_desc {
+    int32_t n_layers;                          /* number of Dense layers, 1..8 */
+    int32_t widths[NODE_HIP_MAX_LAYERS + 1];
+    int32_t acts[NODE_HIP_MAX_LAYERS];
+    int32_t input_map;                         /* NODE_MAP_* */
+    int32_t precision;                         /* NODE_PREC_* */
+    int64_t batch;                             /* B: trajectories (columns) */
+} node_desc;
+
+/* Solver options: the kwargs of solve(prob, Tsit5(); ...) that this path
+ * honours.  Saving follows OrdinaryDiffEq's internal form: `saveat` holds the
+ * interior save times (strictly between t0 and t1, increasing), plus the
+ * save_start / save_end / save_everystep flags. */
+typedef struct node_solve_opts {
+    double t0, t1;            /* tspan */
+    double abstol, reltol;    /* 0 -> 1e-6 / 1e-3 (OrdinaryDiffEq defaults) */
+    int32_t adaptive;         /* 1: PI-controlled steps; 0: fixed dt */
+    int32_t save_start, save_end, save_everystep;
+    double dt;                /* adaptive: initial dt (0 -> heuristic); fixed: the step */
+    double dtmax;             /* 0 -> |t1 - t0| */
+    double dtmin;             /* 0 -> eps of the span */
+    int64_t maxiters;         /* 0 -> 100000 */
+} node_solve_opts;
+
+typedef struct node_stats {
+    int64_t nf;               /* batched RHS (forward) or VJP (adjoint) evaluations */
+    int64_t naccept, nreject; /* steps */
+    int64_t nsaved;           /* saved states written */
+    int32_t retcode;          /* NODE_RET_* */
+    int32_t reserved;
+    double t_final;
+} node_stats;
+
+int32_t node_abi_version(void);
+
+/* Number of flat parameters P for a description (no handle needed). */
+int64_t node_num_params(const node_desc* desc);
+
+int32_t node_create(const node_desc* desc, node_handle_t** out);
+int32_t node_destroy(node_handle_t* h);
+const char* node_last_error(const node_handle_t* h);
+
+/* Parameters in flat Lux order (P FP32 values). Host or device pointer. */
+int32_t node_set_params(node_handle_t* h, const float* p, int64_t n);
+int32_t node_set_params_device(node_handle_t* h, const float* d_p, int64_t n);
+
+/* Forward solve.  u0: D x B.  out: D x B x n_out with capacity out_cap states
+ * (n_out is returned in stats->nsaved; with save_everystep it is only known
+ * after the solve: NODE_ERR_CAPACITY is returned when out_cap is too small and
+ * node_copy_saved() can then fetch everything).  The forward tape needed by
+ * node_backward stays in the handle.
+ * Replaces: the forward half of src/neural_de.jl:55-58. */
+int32_t node_forward(node_handle_t* h, const node_solve_opts* opts,
+                     const float* u0, const double* saveat, int64_t n_saveat,
+                     float* out, int64_t out_cap, node_stats* stats);
+/* Same with u0/out/saveat in device memory (HIP pointers); no host copies. */
+int32_t node_forward_device(node_handle_t* h, const node_solve_opts* opts,
+                            const float* d_u0, const double* saveat, int64_t n_saveat,
+                            float* d_out, int64_t out_cap, node_stats* stats);
+
+/* Times of the states saved by the last forward (n = stats->nsaved). */
+int32_t node_saved_times(node_handle_t* h, double* t_out, int64_t cap);
+/* Copy the saved states of the last forward (D x B x n) to host memory. */
+int32_t node_copy_saved(node_handle_t* h, float* out, int64_t cap);
+
+/* InterpolatingAdjoint backward solve of the last forward.
+ * dLdout: cotangent of the saved states (D x B x nsaved).  Writes dL/du0
+ * (D x B, FP32) and dL/dp (P, FP64).
+ * Replaces: the pullback SciMLSensitivity attaches to the solve at
+ * src/neural_de.jl:55-58 with sensealg InterpolatingAdjoint(ZygoteVJP()). */
+int32_t node_backward(node_handle_t* h, const float* dLdout, float* dLdu0,
+                      double* dLdp, node_stats* stats);
+int32_t node_backward_device(node_handle_t* h, const float* d_dLdout,
+                             float* d_dLdu0, double* d_dLdp, node_stats* stats);
+
+/* One batched right-hand-side evaluation du = model(u, p) (D x B), and one
+ * VJP (lambda^T df/du, lambda^T df/dp) -- the Lux Chain and its Zygote
+ * pullback (src/neural_de.jl:50, ZygoteVJP at :57).  Host pointers. */
+int32_t node_rhs(node_handle_t* h, const float* u, float* du);
+int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda,
+                 float* du, double* dp);
+
+/* Profiling hook (no counterpart in the reference; for measurement tools).
+ * Records a HIP-event pair on the handle's stream around every GEMM launch
+ * whose key (role * 8 + layer) has its bit set in key_mask; 0 disables.
+ * Roles: 0 forward hidden layer, 1 forward last layer (stage epilogue),
+ * 2 adjoint forward recompute, 3 adjoint data gradient, 4 weight gradient,
+ * 5 adjoint layer-1 data gradient (stage epilogue). */
+int32_t node_prof_enable(node_handle_t* h, uint64_t key_mask);
+/* Per-launch durations (ms) and keys recorded since enabling; clears them. */
+int32_t node_prof_collect(node_handle_t* h, int32_t* keys, float* ms, int64_t cap,
+                          int64_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NODE_HIP_H */
