@@ -33,7 +33,7 @@ for _p in (ROOT, os.path.join(ROOT, "diffeqflux.jl_amd")):
 import numpy as np  # noqa: E402
 
 METRIC = ("fwd+adjoint RHS evals/sec, D=64 H=1024 B=4096; BF16-MFMA vs FP32 at grad rel err")
-PEAK = {"bf16": 2500.0, "fp32": 157.3}   # MI355X dense TFLOP/s
+PEAK = {"bf16": 2500.0, "bf16x3": 2500.0, "fp32": 157.3}   # MI355X dense TFLOP/s
 ROLE_NAMES = {0: "fwd_hidden", 1: "fwd_stage", 2: "adj_hidden", 3: "bwd_data", 4: "wgrad",
               5: "bwd_last"}
 
@@ -84,9 +84,12 @@ def make_inputs(B, seed=1):
     return u0, ts, teacher_targets(u0, ts)
 
 
-def flops_per_launch(role, layer, B):
+def flops_per_launch(role, layer, B, precision):
+    """Algorithmic FLOPs of one launch of the (role, layer) GEMM: 2*B*in*out.
+    For split BF16 the MFMA executes three BF16 passes; the roofline counts
+    the MFMA work actually issued (3x) against the BF16 peak."""
     i, o = WIDTHS[layer], WIDTHS[layer + 1]
-    return 2.0 * B * i * o
+    return 2.0 * B * i * o * (3 if precision == "bf16x3" else 1)
 
 
 # ---------------------------------------------------------------- GPU legs
@@ -152,7 +155,7 @@ def roofline_for(run, dom, precision, durations):
     thr = 0.25 * np.percentile(d, 90)
     real = d[d >= thr]
     avg_ms = float(real.mean())
-    fl = flops_per_launch(dom[0], dom[1], run.B)
+    fl = flops_per_launch(dom[0], dom[1], run.B, precision)
     achieved = fl / (avg_ms * 1e-3) / 1e12
     peak = PEAK[precision]
     info = {"kernel": f"{ROLE_NAMES[dom[0]]}[layer {dom[1] + 1}]",
@@ -210,8 +213,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison leg")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the pure-BF16 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-batch", type=int, default=256)
     args = ap.parse_args()
@@ -243,28 +247,39 @@ def main():
         f"{sa['nreject']} steps nf {sa['nf']}; dominant {rinfo}")
 
     extra = {}
-    if not args.no_fp32 and args.precision == "bf16":
-        r32 = Runner("fp32", args.batch, p, u0, ts, tgt)
-        dt32, nf32, nfa32, _ = r32.timed(max(2, args.steps // 5), 1)
-        v32 = (nf32 + nfa32) / dt32
-        extra["fp32_evals_per_s"] = round(v32, 2)
-        extra["speedup_vs_fp32"] = round(value / v32, 3)
-        log(f"[bench] fp32: {v32:.1f} evals/s -> bf16 speedup {value / v32:.2f}x")
-        del r32
+    legs = []
+    if not args.no_fp32 and args.precision != "fp32":
+        legs.append(("fp32", max(2, args.steps // 5)))
+    if not args.no_bf16 and args.precision != "bf16":
+        legs.append(("bf16", 1))
+    for prec, nsteps in legs:
+        r2 = Runner(prec, args.batch, p, u0, ts, tgt)
+        dt2, nf2, nfa2, l2 = r2.timed(nsteps, 1)
+        v2 = (nf2 + nfa2) / dt2
+        extra[f"{prec}_evals_per_s"] = round(v2, 2)
+        extra[f"{prec}_ms_per_step"] = round(1e3 * dt2 / nsteps, 2)
+        extra[f"{prec}_steps_fwd_adj"] = (f"{l2[0]['naccept']}+{l2[0]['nreject']}/"
+                                          f"{l2[1]['naccept']}+{l2[1]['nreject']}")
+        log(f"[bench] {prec}: {v2:.1f} evals/s, {1e3 * dt2 / nsteps:.1f} ms/step "
+            f"(steps fwd/adj {extra[f'{prec}_steps_fwd_adj']})")
+        del r2
+    if "fp32_evals_per_s" in extra:
+        extra["speedup_vs_fp32"] = round(value / extra["fp32_evals_per_s"], 3)
+        extra["time_speedup_vs_fp32"] = round(extra["fp32_ms_per_step"] / (1e3 * dt / args.steps), 3)
 
     cpu = None
     if not args.no_cpu:
         Bs = args.cpu_batch
         us, tss, tgs = make_inputs(Bs)
         grads = {}
-        for prec in ("bf16", "fp32"):
+        for prec in ("bf16x3", "bf16", "fp32"):
             rs = Runner(prec, Bs, p, us, tss, tgs)
             _, _, du0, dp = rs.step()
             grads[prec] = (du0.cpu().numpy(), dp.cpu().numpy())
         cpu, errs = cpu_baseline_leg(p, Bs, grads)
-        extra["grad_rel_err_bf16"] = round(errs["bf16"], 6)
-        extra["grad_rel_err_fp32"] = round(errs["fp32"], 8)
-        extra["grad_rel_err_du0_bf16"] = round(errs["bf16_du0"], 6)
+        for prec in ("bf16x3", "bf16", "fp32"):
+            extra[f"grad_rel_err_{prec}"] = float(f"{errs[prec]:.3e}")
+            extra[f"grad_rel_err_du0_{prec}"] = float(f"{errs[prec + '_du0']:.3e}")
         extra["grad_rel_err_sample"] = f"||dL/dp - FP64 oracle|| / ||oracle||, B={Bs} sample"
         log(f"[bench] cpu baseline {cpu}; grad rel err {errs}")
 
@@ -273,6 +288,10 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+        "dtype_note": {"bf16x3": "BF16 MFMA operands split hi+lo (3 passes), FP32 accumulate; "
+                                 "FP32 state, FP64 time/controller/dL/dp quadrature",
+                       "bf16": "BF16 MFMA operands, FP32 accumulate; FP32 state",
+                       "fp32": "FP32 MFMA"}[args.precision],
         "data": "synthetic (u0 ~ N(0,1), teacher-ODE targets, Glorot weights)",
         "config": {"workload": "NeuralODE fwd+InterpolatingAdjoint, D=64, MLP "
                                "64-1024-1024-1024-64 softplus, B=4096, tspan (0,1), 32 saveat, "

@@ -9,15 +9,22 @@
 // * single-workgroup controllers: FP64 PI step-size control, initial-dt
 //   heuristic, tstop landing, saveat bookkeeping, retcodes.
 // * elementwise kernels: solve initialisation, step begin (saves, FSAL,
-//   cotangent jumps, mu commit), mu error norm, bias-gradient fold.
+//   cotangent jumps, mu commit), mu error norm, bias / split-K fold.
+//
+// Precision policies: FP32 (fp32 operands), BF16 (bf16 operands) and split
+// BF16 (every operand stored as two bf16 planes hi + lo with hi = bf16(x),
+// lo = bf16(x - hi); a GEMM runs three BF16 passes hi*hi + lo*hi + hi*lo over
+// the same accumulators, which keeps ~16 mantissa bits of both operands).
 //
 // Layouts (all device buffers are allocated zero-filled; padded regions are
 // never written with non-zeros):
 //   state vectors      FP32 [Bp][D]            (Julia D x B column-major bytes)
-//   GEMM activations   T    [Bp][Fp]           (K contiguous for the next layer)
-//   transposed copies  T    [Fp][Bp]           (batch contiguous: K of weight grads)
-//   weights            T    Wr[out_p][in_p] (forward), Wc[in_p][out_p] (backward)
+//   GEMM activations   T    [planes][Bp][Fp]   (K contiguous for the next layer)
+//   transposed copies  T    [planes][Fp][Bp]   (batch contiguous: K of weight grads)
+//   weights            T    Wr[planes][out_p][in_p] (forward), Wc[planes][in_p][out_p]
 // with Bp = B rounded up to 128 and Fp = F rounded up to 64.
+#include <type_traits>
+
 #include "node_internal.h"
 
 namespace nodehip {
@@ -25,6 +32,10 @@ namespace nodehip {
 typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+// native 16-byte vector (HIP's uint4 is a struct wrapper that blocks SROA of
+// the register-staged tiles)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __constant__ float cA[7][7];
 __constant__ float cBT[7];
@@ -120,6 +131,55 @@ __device__ __forceinline__ void store4(bf16* p, const float v[4]) {
   *(bf16x4*)p = o;
 }
 
+// precision policy: operand type, split (hi/lo planes), act' storage type
+template <typename T_, bool SPLIT_>
+struct Prec {
+  using T = T_;
+  static constexpr bool SPLIT = SPLIT_;
+  using ST = typename std::conditional<SPLIT_, float, T_>::type;
+};
+using PF32 = Prec<float, false>;
+using PBF16 = Prec<bf16, false>;
+using PBF16X3 = Prec<bf16, true>;
+
+template <class PP>
+__device__ __forceinline__ void put(typename PP::T* p, int64_t plane, float v) {
+  if constexpr (PP::SPLIT) {
+    const bf16 h = (bf16)v;
+    p[0] = h;
+    p[plane] = (bf16)(v - (float)h);
+  } else {
+    p[0] = cvt<typename PP::T>(v);
+  }
+}
+template <class PP>
+__device__ __forceinline__ float get(const typename PP::T* p, int64_t plane) {
+  if constexpr (PP::SPLIT) return (float)p[0] + (float)p[plane];
+  else return tof<typename PP::T>(p[0]);
+}
+template <class PP>
+__device__ __forceinline__ void put4(typename PP::T* p, int64_t plane, const float v[4]) {
+  if constexpr (PP::SPLIT) {
+    float hi[4], lo[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      hi[i] = (float)(bf16)v[i];
+      lo[i] = v[i] - hi[i];
+    }
+    store4(p, hi);
+    store4(p + plane, lo);
+  } else {
+    store4(p, v);
+  }
+}
+
+#define DISPATCH_PREC(prec, ...)                   \
+  do {                                             \
+    if ((prec) == 1) { using PP = PBF16; __VA_ARGS__; }   \
+    else if ((prec) == 2) { using PP = PBF16X3; __VA_ARGS__; } \
+    else { using PP = PF32; __VA_ARGS__; }          \
+  } while (0)
+
 // Block-wide reduction of (sum, max-flag); thread 0 writes part[2*slot..].
 __device__ void block_reduce_store(double s, double f, double* part, int slot) {
   __shared__ double red[2][16];
@@ -148,7 +208,7 @@ __device__ __forceinline__ double dspacing(double t) {
 // ---------------------------------------------------------------- MFMA
 template <typename T> struct Mma;
 template <> struct Mma<bf16> {
-  static __device__ __forceinline__ void run(f32x16& c, const uint4& a, const uint4& b) {
+  static __device__ __forceinline__ void run(f32x16& c, const u32x4& a, const u32x4& b) {
     c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
   }
@@ -156,8 +216,8 @@ template <> struct Mma<bf16> {
 // FP32: 4 MFMA 32x32x2 per 16-B fragment.  Lane l supplies k = 4*(l/32) + j to
 // step j for both operands, so the four steps cover the 8 k of the chunk.
 template <> struct Mma<float> {
-  static __device__ __forceinline__ void run(f32x16& c, const uint4& a, const uint4& b) {
-    const float4 fa = __builtin_bit_cast(float4, a), fb = __builtin_bit_cast(float4, b);
+  static __device__ __forceinline__ void run(f32x16& c, const u32x4& a, const u32x4& b) {
+    const f32x4v fa = __builtin_bit_cast(f32x4v, a), fb = __builtin_bit_cast(f32x4v, b);
     c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa.x, fb.x, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa.y, fb.y, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa.z, fb.z, c, 0, 0, 0);
@@ -176,33 +236,37 @@ template <> struct Mma<float> {
 #define EPI_LOOP_END }}
 
 // ---------------------------------------------------------------- epilogues
-template <typename T, int MR, int MC>
+template <class PP, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiHidden& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl*) {
+                                          int cb, int lane, const Ctl*, int) {
+  using T = typename PP::T;
+  using ST = typename PP::ST;
   T* Y = (T*)e.Y;
   T* YT = (T*)e.YT;
-  T* S = (T*)e.S;
+  ST* S = (ST*)e.S;
   EPI_LOOP_BEGIN
     const bool cv = c < e.C;
     const float bias = cv ? e.bias[c] : 0.f;
-    float yv[4], dv[4];
+    float yv[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       const float z = acc[mi][mj][q * 4 + p] + bias;
       const float y = act_f(e.act, z);
       yv[p] = cv ? y : 0.f;
-      dv[p] = cv ? act_d(e.act, z, y) : 0.f;
-      Y[(int64_t)(r0 + p) * e.ldo + c] = cvt<T>(yv[p]);
-      if (S) S[(int64_t)(r0 + p) * e.ldo + c] = cvt<T>(dv[p]);
+      const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
+      put<PP>(Y + o, e.py, yv[p]);
+      if (S) S[o] = cvt<ST>(cv ? act_d(e.act, z, y) : 0.f);
     }
-    if (YT) store4(YT + (int64_t)c * e.ldt + r0, yv);
+    if (YT) put4<PP>(YT + (int64_t)c * e.ldt + r0, e.pyt, yv);
   EPI_LOOP_END
 }
 
-template <typename T, int MR, int MC>
+template <class PP, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl*) {
-  const T* S = (const T*)e.S;
+                                          int cb, int lane, const Ctl*, int) {
+  using T = typename PP::T;
+  using ST = typename PP::ST;
+  const ST* S = (const ST*)e.S;
   T* Dl = (T*)e.Dl;
   T* DlT = (T*)e.DlT;
   EPI_LOOP_BEGIN
@@ -211,16 +275,17 @@ __device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR]
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
-      dv[p] = cv ? acc[mi][mj][q * 4 + p] * tof<T>(S[o]) : 0.f;
-      Dl[o] = cvt<T>(dv[p]);
+      dv[p] = cv ? acc[mi][mj][q * 4 + p] * tof<ST>(S[o]) : 0.f;
+      put<PP>(Dl + o, e.pd, dv[p]);
     }
-    store4(DlT + (int64_t)c * e.ldt + r0, dv);
+    put4<PP>(DlT + (int64_t)c * e.ldt + r0, e.pdt, dv);
   EPI_LOOP_END
 }
 
-template <typename T, int MR, int MC>
+template <class PP, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl* ctl) {
+                                          int cb, int lane, const Ctl* ctl, int) {
+  using T = typename PP::T;
   const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
   float* ent = e.tape + n * e.ent;
   const float dt = (float)ctl->dt;
@@ -241,13 +306,16 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
           ent[(1 + s) * e.sv + idx] = k;
           if (s < 6) {
             float sm = cA[s + 1][s] * k;
-            for (int j = 0; j < s; ++j) sm += cA[s + 1][j] * ent[(1 + j) * e.sv + idx];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+              if (j < s) sm += cA[s + 1][j] * ent[(1 + j) * e.sv + idx];
             const float u = ent[idx] + dt * sm;
-            X0[(int64_t)r * e.ldx + c] = cvt<T>(map_f(e.map, u));
+            put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, map_f(e.map, u));
             if (s == 5) ent[e.ent + idx] = u;            // candidate tape[n+1].uprev
           } else {
             ent[e.ent + e.sv + idx] = k;                  // FSAL: tape[n+1].k1
             float ut = cBT[6] * k;
+#pragma unroll
             for (int j = 0; j < 6; ++j) ut += cBT[j] * ent[(1 + j) * e.sv + idx];
             ut *= dt;
             const float up = ent[idx], un = ent[e.ent + idx];
@@ -281,9 +349,10 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
   if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
 }
 
-template <typename T, int MR, int MC>
+template <class PP, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl* ctl) {
+                                          int cb, int lane, const Ctl* ctl, int) {
+  using T = typename PP::T;
   const float dt = (float)ctl->dt;
   const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
   const int s = e.stage;
@@ -332,11 +401,13 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
           e.kl[s * e.sv + idx] = k;
           if (s < 6) {
             float sm = cA[s + 1][s] * k;
-            for (int j = 0; j < s; ++j) sm += cA[s + 1][j] * e.kl[j * e.sv + idx];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+              if (j < s) sm += cA[s + 1][j] * e.kl[j * e.sv + idx];
             const float ls = e.lam[idx] + dt * sm;
             if (s == 5) e.lam_new[idx] = ls;
-            L[(int64_t)r * e.ldl + c] = cvt<T>(ls);
-            LT[(int64_t)c * e.ldlt + r] = cvt<T>(ls);
+            put<PP>(L + (int64_t)r * e.ldl + c, e.pl, ls);
+            put<PP>(LT + (int64_t)c * e.ldlt + r, e.plt, ls);
             const float* te = e.tape + (int64_t)tj * e.ent;
             float u = te[idx];
             if (!ex) {
@@ -346,11 +417,12 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
               u += tdt * sm2;
             }
             const float x = map_f(e.map, u);
-            X0[(int64_t)r * e.ldx + c] = cvt<T>(x);
-            X0T[(int64_t)c * e.ldxt + r] = cvt<T>(x);
+            put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, x);
+            put<PP>(X0T + (int64_t)c * e.ldxt + r, e.pxt, x);
             e.Ucur[idx] = u;
           } else {
             float ut = cBT[6] * k;
+#pragma unroll
             for (int j = 0; j < 6; ++j) ut += cBT[j] * e.kl[j * e.sv + idx];
             ut *= dt;
             const float lp = e.lam[idx], ln = e.lam_new[idx];
@@ -368,31 +440,37 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
   if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
 }
 
-template <typename T, int MR, int MC>
+__device__ __forceinline__ void fold_one(const EpiWgrad& e, float* gcur, float* gnext,
+                                         int64_t idx, float g) {
+  if (e.flags & WG_STORE_CUR) gcur[idx] = g;
+  if (e.flags & WG_STORE_PTR) e.gptr[idx] = g;
+  if (e.flags & WG_INIT) {
+    const float g1 = gcur[idx];
+    e.accb[idx] = e.cb1 * g1 + e.cb * g;
+    e.acce[idx] = e.ce1 * g1 + e.ce * g;
+  } else {
+    if (e.flags & WG_B) e.accb[idx] += e.cb * g;
+    if (e.flags & WG_E) e.acce[idx] += e.ce * g;
+  }
+  if (e.flags & WG_STORE_NEXT) gnext[idx] = g;
+}
+
+template <class PP, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiWgrad& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl* ctl) {
+                                          int cb, int lane, const Ctl* ctl, int split) {
   const int gsel = ctl->gsel;
   float* gcur = gsel ? e.gf1 : e.gf0;
   float* gnext = gsel ? e.gf0 : e.gf1;
+  float* wp = e.wpart ? e.wpart + (int64_t)split * e.in * e.out : nullptr;
   EPI_LOOP_BEGIN
     if (c < e.out) {
 #pragma unroll
       for (int p = 0; p < 4; ++p) {
         const int i = r0 + p;
         if (i < e.in) {
-          const int64_t idx = e.w0 + (int64_t)i * e.out + c;
           const float g = -acc[mi][mj][q * 4 + p];
-          if (e.flags & WG_STORE_CUR) gcur[idx] = g;
-          if (e.flags & WG_STORE_PTR) e.gptr[idx] = g;
-          if (e.flags & WG_INIT) {
-            const float g1 = gcur[idx];
-            e.accb[idx] = e.cb1 * g1 + e.cb * g;
-            e.acce[idx] = e.ce1 * g1 + e.ce * g;
-          } else {
-            if (e.flags & WG_B) e.accb[idx] += e.cb * g;
-            if (e.flags & WG_E) e.acce[idx] += e.ce * g;
-          }
-          if (e.flags & WG_STORE_NEXT) gnext[idx] = g;
+          if (wp) wp[(int64_t)i * e.out + c] = g;       // reduced by the fold kernel
+          else fold_one(e, gcur, gnext, e.w0 + (int64_t)i * e.out + c, g);
         }
       }
     }
@@ -401,14 +479,17 @@ __device__ __forceinline__ void epi_apply(const EpiWgrad& e, f32x16 (&acc)[MR][M
 
 // ---------------------------------------------------------------- GEMM
 // C[r][c] = sum_k A[r][k] * B[c][k]; rows = batch (or input feature for
-// weight gradients), cols = output feature.  256-thread workgroups of WRxWC
-// waves, each wave a (BR/WR)x(BC/WC) block of 32x32 MFMA tiles; K is staged
-// through LDS in 128-byte rows (64 bf16 / 32 fp32), double-buffered, one
-// barrier per K tile.  Tile order is XCD-aware: consecutive tiles of the
-// row-major tile grid go to the same XCD so that its L2 holds their operands.
-template <typename T, int BR, int BC, int WR, int WC, typename Epi>
+// weight gradients), cols = output feature.  WRxWC waves per workgroup, each
+// wave a (BR/WR)x(BC/WC) block of 32x32 MFMA tiles; K is staged through LDS in
+// 128-byte rows (64 bf16 / 32 fp32), double-buffered, one barrier per K tile.
+// Split BF16 runs three passes over K: (A hi, B hi), (A lo, B hi), (A hi, B lo).
+// Split-K (ksplit > 1): workgroup group s covers K tiles [s*nk, (s+1)*nk).
+// Tile order is XCD-aware: consecutive tiles of the row-major tile grid go to
+// the same XCD so that its L2 holds their operands.
+template <class PP, int BR, int BC, int WR, int WC, typename Epi>
 __global__ void __launch_bounds__(WR * WC * 64)
     gemm_tn(GemmArgs g, Epi epi, const Ctl* __restrict__ ctl, int gate) {
+  using T = typename PP::T;
   if (!gate_ok(ctl, gate)) return;
   constexpr int NT = WR * WC * 64;
   constexpr int PITCH = 144;
@@ -416,23 +497,26 @@ __global__ void __launch_bounds__(WR * WC * 64)
   constexpr int MR = BR / WR / 32, MC = BC / WC / 32;
   constexpr int RP = NT / 8;  // rows per load pass
   constexpr int A_LOADS = BR / RP, B_LOADS = BC / RP;
+  constexpr int NPASS = PP::SPLIT ? 3 : 1;
   static_assert(BR % RP == 0 && BC % RP == 0, "tile/thread mismatch");
   __shared__ __attribute__((aligned(16))) unsigned char lds[2 * (BR + BC) * PITCH];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave / WC, wc = wave % WC;
   const int nt = g.row_tiles * g.col_tiles;
-  int bid = blockIdx.x;
+  const int split = blockIdx.x / nt;
+  int bid = blockIdx.x - split * nt;
   if ((nt & 7) == 0) bid = (bid & 7) * (nt >> 3) + (bid >> 3);
   const int ct = bid % g.col_tiles, rt = bid / g.col_tiles;
   const int row0 = rt * BR, col0 = ct * BC;
 
   const int64_t ldab = g.lda * (int64_t)sizeof(T), ldbb = g.ldb * (int64_t)sizeof(T);
+  const int64_t pab = g.pa * (int64_t)sizeof(T), pbb = g.pb * (int64_t)sizeof(T);
   const unsigned char* Ag = (const unsigned char*)g.A + (int64_t)row0 * ldab;
   const unsigned char* Bg = (const unsigned char*)g.B + (int64_t)col0 * ldbb;
   const int lr = tid >> 3, lc = tid & 7;
 
-  uint4 ra[A_LOADS], rbv[B_LOADS];
+  u32x4 ra[A_LOADS], rbv[B_LOADS];
   f32x16 acc[MR][MC];
 #pragma unroll
   for (int i = 0; i < MR; ++i)
@@ -441,53 +525,61 @@ __global__ void __launch_bounds__(WR * WC * 64)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
-  const int nk = (int)((int64_t)g.K * sizeof(T) / BKB);
-  auto gload = [&](int kt) {
-    const int64_t kb = (int64_t)kt * BKB + lc * 16;
-#pragma unroll
-    for (int i = 0; i < A_LOADS; ++i)
-      ra[i] = *(const uint4*)(Ag + (int64_t)(lr + i * RP) * ldab + kb);
-#pragma unroll
-    for (int i = 0; i < B_LOADS; ++i)
-      rbv[i] = *(const uint4*)(Bg + (int64_t)(lr + i * RP) * ldbb + kb);
-  };
-  auto sstore = [&](int buf) {
-    unsigned char* base = lds + buf * (BR + BC) * PITCH;
-#pragma unroll
-    for (int i = 0; i < A_LOADS; ++i) *(uint4*)(base + (lr + i * RP) * PITCH + lc * 16) = ra[i];
-#pragma unroll
-    for (int i = 0; i < B_LOADS; ++i)
-      *(uint4*)(base + (BR + lr + i * RP) * PITCH + lc * 16) = rbv[i];
-  };
+  const int nk = (int)((int64_t)g.K * sizeof(T) / BKB) / g.ksplit;
+  const int k0 = split * nk;
+  const int ntot = NPASS * nk;
+  // global -> registers (tile t of the pass-major K sequence)
+#define GEMM_GLOAD(t)                                                              \
+  do {                                                                             \
+    const int pass_ = (t) / nk, kt_ = k0 + (t) - pass_ * nk;                       \
+    const int64_t kb_ = (int64_t)kt_ * BKB + lc * 16;                              \
+    const unsigned char* Ap_ = Ag + (pass_ == 1 ? pab : 0) + kb_;                  \
+    const unsigned char* Bp_ = Bg + (pass_ == 2 ? pbb : 0) + kb_;                  \
+    _Pragma("unroll") for (int i_ = 0; i_ < A_LOADS; ++i_)                         \
+      ra[i_] = *(const u32x4*)(Ap_ + (int64_t)(lr + i_ * RP) * ldab);              \
+    _Pragma("unroll") for (int i_ = 0; i_ < B_LOADS; ++i_)                         \
+      rbv[i_] = *(const u32x4*)(Bp_ + (int64_t)(lr + i_ * RP) * ldbb);             \
+  } while (0)
+  // registers -> LDS buffer
+#define GEMM_SSTORE(buf)                                                           \
+  do {                                                                             \
+    unsigned char* base_ = lds + (buf) * (BR + BC) * PITCH;                        \
+    _Pragma("unroll") for (int i_ = 0; i_ < A_LOADS; ++i_)                         \
+      *(u32x4*)(base_ + (lr + i_ * RP) * PITCH + lc * 16) = ra[i_];               \
+    _Pragma("unroll") for (int i_ = 0; i_ < B_LOADS; ++i_)                         \
+      *(u32x4*)(base_ + (BR + lr + i_ * RP) * PITCH + lc * 16) = rbv[i_];         \
+  } while (0)
 
-  gload(0);
-  sstore(0);
+  GEMM_GLOAD(0);
+  GEMM_SSTORE(0);
   __syncthreads();
   const int arow = wr * (BR / WR) + (lane & 31);
   const int brow = BR + wc * (BC / WC) + (lane & 31);
   const int koff = (lane >> 5) * 16;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) gload(kt + 1);
+  for (int t = 0; t < ntot; ++t) {
+    const int cur = t & 1;
+    if (t + 1 < ntot) GEMM_GLOAD(t + 1);
     const unsigned char* base = lds + cur * (BR + BC) * PITCH;
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
-      uint4 fa[MR], fb[MC];
+      u32x4 fa[MR], fb[MC];
 #pragma unroll
       for (int mi = 0; mi < MR; ++mi)
-        fa[mi] = *(const uint4*)(base + (arow + mi * 32) * PITCH + kk * 32 + koff);
+        fa[mi] = *(const u32x4*)(base + (arow + mi * 32) * PITCH + kk * 32 + koff);
 #pragma unroll
       for (int mj = 0; mj < MC; ++mj)
-        fb[mj] = *(const uint4*)(base + (brow + mj * 32) * PITCH + kk * 32 + koff);
+        fb[mj] = *(const u32x4*)(base + (brow + mj * 32) * PITCH + kk * 32 + koff);
 #pragma unroll
       for (int mi = 0; mi < MR; ++mi)
 #pragma unroll
         for (int mj = 0; mj < MC; ++mj) Mma<T>::run(acc[mi][mj], fa[mi], fb[mj]);
     }
-    if (kt + 1 < nk) sstore(cur ^ 1);
+    if (t + 1 < ntot) GEMM_SSTORE(cur ^ 1);
     __syncthreads();
   }
-  epi_apply<T, MR, MC>(epi, acc, row0 + wr * (BR / WR), col0 + wc * (BC / WC), lane, ctl);
+#undef GEMM_GLOAD
+#undef GEMM_SSTORE
+  epi_apply<PP, MR, MC>(epi, acc, row0 + wr * (BR / WR), col0 + wc * (BC / WC), lane, ctl, split);
 }
 
 // tile configurations: {BR, BC, WR, WC}
@@ -496,33 +588,36 @@ int gemm_cfg_count() { return 4; }
 int gemm_cfg_tile_rows(int cfg) { return kCfg[cfg][0]; }
 int gemm_cfg_tile_cols(int cfg) { return kCfg[cfg][1]; }
 
-template <typename T, typename Epi>
+template <class PP, typename Epi>
 static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, int gate,
                      hipStream_t s) {
-  const int nt = g.row_tiles * g.col_tiles;
+  const int nb = g.row_tiles * g.col_tiles * (g.ksplit > 0 ? g.ksplit : 1);
+  GemmArgs ga = g;
+  if (ga.ksplit < 1) ga.ksplit = 1;
+  // the stage epilogues (per-element Tsit5 / adjoint stage work) are planned
+  // with at most 64 output columns per tile; their 128x128 form is not built
+  constexpr bool kStage = std::is_same<Epi, EpiFwdStage>::value || std::is_same<Epi, EpiBwdLast>::value;
+  if (kStage && cfg == 0) cfg = 1;
   switch (cfg) {
-    case 0: hipLaunchKernelGGL((gemm_tn<T, 128, 128, 2, 2, Epi>), dim3(nt), dim3(256), 0, s, g, e, ctl, gate); break;
-    case 1: hipLaunchKernelGGL((gemm_tn<T, 128, 64, 2, 2, Epi>), dim3(nt), dim3(256), 0, s, g, e, ctl, gate); break;
-    case 2: hipLaunchKernelGGL((gemm_tn<T, 64, 64, 2, 2, Epi>), dim3(nt), dim3(256), 0, s, g, e, ctl, gate); break;
-    default: hipLaunchKernelGGL((gemm_tn<T, 32, 64, 1, 2, Epi>), dim3(nt), dim3(128), 0, s, g, e, ctl, gate); break;
+    case 0:
+      if constexpr (!kStage)
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 2, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate);
+      break;
+    case 1: hipLaunchKernelGGL((gemm_tn<PP, 128, 64, 2, 2, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
+    case 2: hipLaunchKernelGGL((gemm_tn<PP, 64, 64, 2, 2, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
+    default: hipLaunchKernelGGL((gemm_tn<PP, 32, 64, 1, 2, Epi>), dim3(nb), dim3(128), 0, s, ga, e, ctl, gate); break;
   }
 }
-template <typename Epi>
-static void launch_p(int prec, int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl,
-                     int gate, hipStream_t s) {
-  if (prec == 1) launch_t<bf16>(cfg, g, e, ctl, gate, s);
-  else launch_t<float>(cfg, g, e, ctl, gate, s);
-}
 void launch_gemm_hidden(int prec, int cfg, const GemmArgs& g, const EpiHidden& e,
-                        const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+                        const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_fwdstage(int prec, int cfg, const GemmArgs& g, const EpiFwdStage& e,
-                          const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+                          const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_bwddata(int prec, int cfg, const GemmArgs& g, const EpiBwdData& e,
-                         const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+                         const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_bwdlast(int prec, int cfg, const GemmArgs& g, const EpiBwdLast& e,
-                         const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+                         const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
-                       const Ctl* ctl, int gate, hipStream_t s) { launch_p(prec, cfg, g, e, ctl, gate, s); }
+                       const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 
 // ---------------------------------------------------------------- elementwise
 constexpr int EW_THREADS = 256;
@@ -533,9 +628,9 @@ int state_blocks(const StateGeom& sg) {
   return (int)(b < EW_MAX_BLOCKS ? b : EW_MAX_BLOCKS);
 }
 
-template <typename T>
-__global__ void k_fwd_init(StateGeom sg, const float* __restrict__ u0, float* tape, T* X0,
-                           const Ctl* ctl, double* part) {
+template <class PP>
+__global__ void k_fwd_init(StateGeom sg, const float* __restrict__ u0, float* tape,
+                           typename PP::T* X0, int64_t px, const Ctl* ctl, double* part) {
   const double atol = ctl->abstol, rtol = ctl->reltol;
   double sum = 0.0;
   const int64_t n = (int64_t)sg.Bp * sg.D;
@@ -544,40 +639,41 @@ __global__ void k_fwd_init(StateGeom sg, const float* __restrict__ u0, float* ta
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float u = r < sg.B ? u0[i] : 0.f;
     tape[i] = u;
-    X0[(int64_t)r * sg.Dp + c] = cvt<T>(map_f(sg.map, u));
+    put<PP>(X0 + (int64_t)r * sg.Dp + c, px, map_f(sg.map, u));
     if (r < sg.B) { const double a = (double)u / (atol + fabs((double)u) * rtol); sum += a * a; }
   }
   block_reduce_store(sum, 0.0, part, blockIdx.x);
 }
-void launch_fwd_init(const StateGeom& sg, const float* u0, float* tape, void* X0,
+void launch_fwd_init(const StateGeom& sg, const float* u0, float* tape, void* X0, int64_t px,
                      const Ctl* ctl, double* part, int nblk, hipStream_t s) {
-  if (sg.prec == 1) hipLaunchKernelGGL(k_fwd_init<bf16>, dim3(nblk), dim3(EW_THREADS), 0, s, sg, u0, tape, (bf16*)X0, ctl, part);
-  else hipLaunchKernelGGL(k_fwd_init<float>, dim3(nblk), dim3(EW_THREADS), 0, s, sg, u0, tape, (float*)X0, ctl, part);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_fwd_init<PP>, dim3(nblk), dim3(EW_THREADS), 0, s,
+                                            sg, u0, tape, (typename PP::T*)X0, px, ctl, part));
 }
 
-template <typename T>
-__global__ void k_fwd_initdt_prep(StateGeom sg, const float* __restrict__ tape, T* X0,
-                                  const Ctl* ctl) {
+template <class PP>
+__global__ void k_fwd_initdt_prep(StateGeom sg, const float* __restrict__ tape,
+                                  typename PP::T* X0, int64_t px, const Ctl* ctl) {
   const float h = (float)(ctl->tdir * ctl->dt0);
   const int64_t n = (int64_t)sg.Bp * sg.D;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float u = tape[i] + h * tape[sg.sv + i];
-    X0[(int64_t)r * sg.Dp + c] = cvt<T>(map_f(sg.map, u));
+    put<PP>(X0 + (int64_t)r * sg.Dp + c, px, map_f(sg.map, u));
   }
 }
-void launch_fwd_initdt_prep(const StateGeom& sg, const float* tape, void* X0, const Ctl* ctl,
-                            hipStream_t s) {
+void launch_fwd_initdt_prep(const StateGeom& sg, const float* tape, void* X0, int64_t px,
+                            const Ctl* ctl, hipStream_t s) {
   int nb = state_blocks(sg);
-  if (sg.prec == 1) hipLaunchKernelGGL(k_fwd_initdt_prep<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (bf16*)X0, ctl);
-  else hipLaunchKernelGGL(k_fwd_initdt_prep<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (float*)X0, ctl);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_fwd_initdt_prep<PP>, dim3(nb), dim3(EW_THREADS), 0,
+                                            s, sg, tape, (typename PP::T*)X0, px, ctl));
 }
 
 // forward step begin: pending saves of the last accepted step, then the
 // stage-2 input of the next attempt
-template <typename T>
-__global__ void k_fwd_step_begin(StateGeom sg, const float* __restrict__ tape, T* X0,
+template <class PP>
+__global__ void k_fwd_step_begin(StateGeom sg, const float* __restrict__ tape,
+                                 typename PP::T* X0, int64_t px,
                                  const double* __restrict__ save_times, float* out,
                                  const Ctl* ctl) {
   const int64_t n = (int64_t)sg.Bp * sg.D;
@@ -617,15 +713,16 @@ __global__ void k_fwd_step_begin(StateGeom sg, const float* __restrict__ tape, T
        i += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float u = ecur[i] + h * ecur[sg.sv + i];
-    X0[(int64_t)r * sg.Dp + c] = cvt<T>(map_f(sg.map, u));
+    put<PP>(X0 + (int64_t)r * sg.Dp + c, px, map_f(sg.map, u));
   }
 }
-void launch_fwd_step_begin(const StateGeom& sg, const float* tape, void* X0,
+void launch_fwd_step_begin(const StateGeom& sg, const float* tape, void* X0, int64_t px,
                            const double* save_times, float* out, const Ctl* ctl,
                            hipStream_t s) {
   int nb = state_blocks(sg);
-  if (sg.prec == 1) hipLaunchKernelGGL(k_fwd_step_begin<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (bf16*)X0, save_times, out, ctl);
-  else hipLaunchKernelGGL(k_fwd_step_begin<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, tape, (float*)X0, save_times, out, ctl);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_fwd_step_begin<PP>, dim3(nb), dim3(EW_THREADS), 0,
+                                            s, sg, tape, (typename PP::T*)X0, px, save_times,
+                                            out, ctl));
 }
 
 // ---------------------------------------------------------------- controllers
@@ -839,14 +936,14 @@ void launch_fwd_controller(Ctl* ctl, const double* part, int n_part, const doubl
   hipLaunchKernelGGL(k_fwd_controller, dim3(1), dim3(256), 0, s, ctl, part, n_part, save_times,
                      fwd_times, n_elems);
 }
-
 // ---------------------------------------------------------------- adjoint
-template <typename T>
+template <class PP>
 __device__ __forceinline__ void adj_prep_stage(const StateGeom& sg, const AdjBufs& ab,
                                                const Ctl* c, int slot, int64_t i, float lam_s) {
+  using T = typename PP::T;
   const int r = (int)(i / sg.D), col = (int)(i % sg.D);
-  ((T*)ab.L)[(int64_t)r * ab.ldl + col] = cvt<T>(lam_s);
-  ((T*)ab.LT)[(int64_t)col * ab.ldlt + r] = cvt<T>(lam_s);
+  put<PP>((T*)ab.L + (int64_t)r * ab.ldl + col, ab.pl, lam_s);
+  put<PP>((T*)ab.LT + (int64_t)col * ab.ldlt + r, ab.plt, lam_s);
   const float* te = ab.tape + (int64_t)c->st_step[slot] * sg.ent;
   float u = te[i];
   if (!c->st_exact[slot]) {
@@ -856,12 +953,12 @@ __device__ __forceinline__ void adj_prep_stage(const StateGeom& sg, const AdjBuf
     u += c->st_dt[slot] * sm;
   }
   const float x = map_f(sg.map, u);
-  ((T*)ab.X0)[(int64_t)r * ab.ldx + col] = cvt<T>(x);
-  ((T*)ab.X0T)[(int64_t)col * ab.ldxt + r] = cvt<T>(x);
+  put<PP>((T*)ab.X0 + (int64_t)r * ab.ldx + col, ab.px, x);
+  put<PP>((T*)ab.X0T + (int64_t)col * ab.ldxt + r, ab.pxt, x);
   ab.U[i] = u;
 }
 
-template <typename T>
+template <class PP>
 __global__ void k_adj_init(StateGeom sg, AdjBufs ab, const Ctl* c, double* part) {
   const double atol = c->abstol, rtol = c->reltol;
   const int jidx = c->jump_idx;
@@ -873,7 +970,7 @@ __global__ void k_adj_init(StateGeom sg, AdjBufs ab, const Ctl* c, double* part)
     float l = 0.f;
     if (jidx >= 0 && r < sg.B) l = ab.dLdout[(int64_t)jidx * sg.B * sg.D + i];
     ab.lam[i] = l;
-    adj_prep_stage<T>(sg, ab, c, 0, i, l);
+    adj_prep_stage<PP>(sg, ab, c, 0, i, l);
     if (r < sg.B) { const double a = (double)l / (atol + fabs((double)l) * rtol); sum += a * a; }
   }
   block_reduce_store(sum, 0.0, part, blockIdx.x);
@@ -881,29 +978,29 @@ __global__ void k_adj_init(StateGeom sg, AdjBufs ab, const Ctl* c, double* part)
 void launch_adj_init(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl, double* part,
                      hipStream_t s) {
   int nb = state_blocks(sg);
-  if (sg.prec == 1) hipLaunchKernelGGL(k_adj_init<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl, part);
-  else hipLaunchKernelGGL(k_adj_init<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl, part);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_adj_init<PP>, dim3(nb), dim3(EW_THREADS), 0, s, sg,
+                                            ab, ctl, part));
 }
 
-template <typename T>
+template <class PP>
 __global__ void k_adj_initdt_prep(StateGeom sg, AdjBufs ab, const Ctl* c) {
   const float h = (float)(c->tdir * c->dt0);
   const int64_t n = (int64_t)sg.Bp * sg.D;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    adj_prep_stage<T>(sg, ab, c, 7, i, ab.lam[i] + h * ab.kl[i]);
+    adj_prep_stage<PP>(sg, ab, c, 7, i, ab.lam[i] + h * ab.kl[i]);
 }
 void launch_adj_initdt_prep(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl,
                             hipStream_t s) {
   int nb = state_blocks(sg);
-  if (sg.prec == 1) hipLaunchKernelGGL(k_adj_initdt_prep<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl);
-  else hipLaunchKernelGGL(k_adj_initdt_prep<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, ab, ctl);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_adj_initdt_prep<PP>, dim3(nb), dim3(EW_THREADS), 0,
+                                            s, sg, ab, ctl));
 }
 
 // adjoint step begin (idempotent, so speculative launches after the end are
 // harmless): lambda = lambda_new + jump on accept, FSAL copy, mu commit into
 // the other mu buffer, and the first stage input of the next attempt
-template <typename T>
+template <class PP>
 __global__ void k_adj_step_begin(StateGeom sg, AdjBufs ab, const Ctl* c, int nbs) {
   if ((int)blockIdx.x >= nbs) {
     if (!c->commit_mu) return;
@@ -926,8 +1023,8 @@ __global__ void k_adj_step_begin(StateGeom sg, AdjBufs ab, const Ctl* c, int nbs
     if (acc && jidx >= 0 && r < sg.B) l += ab.dLdout[(int64_t)jidx * sg.B * sg.D + i];
     if (acc) { ab.lam[i] = l; ab.kl[i] = ab.kl[6 * sg.sv + i]; }
     if (done) continue;
-    if (re) adj_prep_stage<T>(sg, ab, c, 0, i, l);
-    else adj_prep_stage<T>(sg, ab, c, 1, i, l + h * (acc ? ab.kl[6 * sg.sv + i] : ab.kl[i]));
+    if (re) adj_prep_stage<PP>(sg, ab, c, 0, i, l);
+    else adj_prep_stage<PP>(sg, ab, c, 1, i, l + h * (acc ? ab.kl[6 * sg.sv + i] : ab.kl[i]));
   }
 }
 int mu_blocks(int64_t P) {
@@ -938,8 +1035,8 @@ void launch_adj_step_begin(const StateGeom& sg, const AdjBufs& ab, const Ctl* ct
                            hipStream_t s) {
   (void)gate;
   int nbs = state_blocks(sg), nbm = mu_blocks(ab.P);
-  if (sg.prec == 1) hipLaunchKernelGGL(k_adj_step_begin<bf16>, dim3(nbs + nbm), dim3(EW_THREADS), 0, s, sg, ab, ctl, nbs);
-  else hipLaunchKernelGGL(k_adj_step_begin<float>, dim3(nbs + nbm), dim3(EW_THREADS), 0, s, sg, ab, ctl, nbs);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_adj_step_begin<PP>, dim3(nbs + nbm),
+                                            dim3(EW_THREADS), 0, s, sg, ab, ctl, nbs));
 }
 
 // mu part of the error norm: modes 0 step error, 1 initdt d1, 2 initdt d2
@@ -1036,83 +1133,91 @@ void launch_set_stage_interp(Ctl* ctl, int slot, double tau, const double* fwd_t
   hipLaunchKernelGGL(k_set_stage_interp, dim3(1), dim3(64), 0, s, ctl, slot, tau, fwd_times);
 }
 
-// bias gradients: g_b[o] = -sum_b delta^T[o][b], folded like the weights
-template <typename T>
-__global__ void k_bias_fold(int nl, const void* const* dT, const int* outs, const int64_t* b0s,
-                            int ldt, int Bp, EpiWgrad e, const Ctl* c, int gate, int rows_total) {
+// fold: bias gradients g_b[o] = -sum_b delta^T[o][b] of every layer (one wave
+// per bias row), and the fixed-order sum of the split-K partials of the
+// narrow layers' weight gradients (one thread per entry); both folded into
+// the mu accumulators like the weight-gradient epilogue does.
+template <class PP>
+__global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_bias) {
+  using T = typename PP::T;
   if (!gate_ok(c, gate)) return;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-  if (wave >= rows_total) return;
-  int l = 0, row = wave;
-  while (l < nl && row >= outs[l]) { row -= outs[l]; ++l; }
-  const T* src = (const T*)dT[l] + (int64_t)row * ldt;
-  float s = 0.f;
-  for (int b = lane; b < Bp; b += 64) s += tof<T>(src[b]);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane != 0) return;
-  const int64_t idx = b0s[l] + row;
-  const float g = -s;
   float* gcur = c->gsel ? e.gf1 : e.gf0;
   float* gnext = c->gsel ? e.gf0 : e.gf1;
-  if (e.flags & WG_STORE_CUR) gcur[idx] = g;
-  if (e.flags & WG_STORE_PTR) e.gptr[idx] = g;
-  if (e.flags & WG_INIT) {
-    const float g1 = gcur[idx];
-    e.accb[idx] = e.cb1 * g1 + e.cb * g;
-    e.acce[idx] = e.ce1 * g1 + e.ce * g;
-  } else {
-    if (e.flags & WG_B) e.accb[idx] += e.cb * g;
-    if (e.flags & WG_E) e.acce[idx] += e.ce * g;
+  if ((int)blockIdx.x < nb_bias) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (wave >= fa.bias_rows) return;
+    int l = 0, row = wave;
+    while (l < fa.nl - 1 && row >= fa.outs[l]) { row -= fa.outs[l]; ++l; }
+    const T* src = (const T*)fa.dT[l] + (int64_t)row * fa.ldt;
+    float s = 0.f;
+    for (int b = lane; b < fa.Bp; b += 64) s += get<PP>(src + b, fa.dplane[l]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) fold_one(e, gcur, gnext, fa.b0s[l] + row, -s);
+    return;
   }
-  if (e.flags & WG_STORE_NEXT) gnext[idx] = g;
+  int64_t i = (int64_t)(blockIdx.x - nb_bias) * blockDim.x + threadIdx.x;
+  for (int k = 0; k < fa.nsk; ++k) {
+    if (i < fa.sk_n[k]) {
+      float s = 0.f;
+      for (int j = 0; j < fa.sk_split[k]; ++j) s += fa.sk_part[k][(int64_t)j * fa.sk_n[k] + i];
+      fold_one(e, gcur, gnext, fa.sk_w0[k] + i, s);
+      return;
+    }
+    i -= fa.sk_n[k];
+  }
 }
-void launch_bias_fold(int prec, int nl, const void* const* dT, const int* outs,
-                      const int64_t* b0s, int ldt, int Bp, int rows_total,
-                      const EpiWgrad& tmpl, const Ctl* ctl, int gate, hipStream_t s) {
-  // dT / outs / b0s are device arrays (per layer)
-  const int nb = (rows_total * 64 + 255) / 256;
-  if (prec == 1) hipLaunchKernelGGL(k_bias_fold<bf16>, dim3(nb), dim3(256), 0, s, nl, dT, outs, b0s, ldt, Bp, tmpl, ctl, gate, rows_total);
-  else hipLaunchKernelGGL(k_bias_fold<float>, dim3(nb), dim3(256), 0, s, nl, dT, outs, b0s, ldt, Bp, tmpl, ctl, gate, rows_total);
+void launch_fold(int prec, const FoldArgs& fa, const EpiWgrad& tmpl, const Ctl* ctl, int gate,
+                 hipStream_t s) {
+  const int nb_bias = (fa.bias_rows * 64 + 255) / 256;
+  int64_t nsk = 0;
+  for (int k = 0; k < fa.nsk; ++k) nsk += fa.sk_n[k];
+  const int nb = nb_bias + (int)((nsk + 255) / 256);
+  DISPATCH_PREC(prec, hipLaunchKernelGGL(k_fold<PP>, dim3(nb), dim3(256), 0, s, fa, tmpl, ctl,
+                                         gate, nb_bias));
 }
 
 // plain input preparation for node_rhs / node_vjp
-template <typename T>
-__global__ void k_prep_plain(StateGeom sg, const float* u, T* X0, T* X0T, float* Ucur, int ldx,
-                             int ldxt) {
+template <class PP>
+__global__ void k_prep_plain(StateGeom sg, const float* u, typename PP::T* X0,
+                             typename PP::T* X0T, float* Ucur, int ldx, int ldxt, int64_t px,
+                             int64_t pxt) {
   const int64_t n = (int64_t)sg.Bp * sg.D;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float v = r < sg.B ? u[i] : 0.f;
     const float x = map_f(sg.map, v);
-    X0[(int64_t)r * ldx + c] = cvt<T>(x);
-    if (X0T) X0T[(int64_t)c * ldxt + r] = cvt<T>(x);
+    put<PP>(X0 + (int64_t)r * ldx + c, px, x);
+    if (X0T) put<PP>(X0T + (int64_t)c * ldxt + r, pxt, x);
     if (Ucur) Ucur[i] = v;
   }
 }
 void launch_prep_plain(const StateGeom& sg, const float* u, void* X0, void* X0T, float* Ucur,
-                       int ldx, int ldxt, hipStream_t s) {
+                       int ldx, int ldxt, int64_t px, int64_t pxt, hipStream_t s) {
   int nb = state_blocks(sg);
-  if (sg.prec == 1) hipLaunchKernelGGL(k_prep_plain<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, u, (bf16*)X0, (bf16*)X0T, Ucur, ldx, ldxt);
-  else hipLaunchKernelGGL(k_prep_plain<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, u, (float*)X0, (float*)X0T, Ucur, ldx, ldxt);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_prep_plain<PP>, dim3(nb), dim3(EW_THREADS), 0, s, sg,
+                                            u, (typename PP::T*)X0, (typename PP::T*)X0T, Ucur,
+                                            ldx, ldxt, px, pxt));
 }
-template <typename T>
-__global__ void k_prep_lambda(StateGeom sg, const float* lam, T* L, T* LT, int ldl, int ldlt) {
+template <class PP>
+__global__ void k_prep_lambda(StateGeom sg, const float* lam, typename PP::T* L,
+                              typename PP::T* LT, int ldl, int ldlt, int64_t pl, int64_t plt) {
   const int64_t n = (int64_t)sg.Bp * sg.D;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float v = r < sg.B ? lam[i] : 0.f;
-    L[(int64_t)r * ldl + c] = cvt<T>(v);
-    LT[(int64_t)c * ldlt + r] = cvt<T>(v);
+    put<PP>(L + (int64_t)r * ldl + c, pl, v);
+    put<PP>(LT + (int64_t)c * ldlt + r, plt, v);
   }
 }
 void launch_prep_lambda(const StateGeom& sg, const float* lam, void* L, void* LT, int ldl,
-                        int ldlt, hipStream_t s) {
+                        int ldlt, int64_t pl, int64_t plt, hipStream_t s) {
   int nb = state_blocks(sg);
-  if (sg.prec == 1) hipLaunchKernelGGL(k_prep_lambda<bf16>, dim3(nb), dim3(EW_THREADS), 0, s, sg, lam, (bf16*)L, (bf16*)LT, ldl, ldlt);
-  else hipLaunchKernelGGL(k_prep_lambda<float>, dim3(nb), dim3(EW_THREADS), 0, s, sg, lam, (float*)L, (float*)LT, ldl, ldlt);
+  DISPATCH_PREC(sg.prec, hipLaunchKernelGGL(k_prep_lambda<PP>, dim3(nb), dim3(EW_THREADS), 0, s,
+                                            sg, lam, (typename PP::T*)L, (typename PP::T*)LT,
+                                            ldl, ldlt, pl, plt));
 }
 
 // gather real rows of n state vectors (stride src_stride floats) into a dense

@@ -36,9 +36,12 @@ const double kBT[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880
 struct Layer {
   int in, out, inp, outp, act;
   int64_t w0, b0;
-  void* Wr = nullptr;     // [outp][inp]
-  void* Wc = nullptr;     // [inp][outp]
+  void* Wr = nullptr;     // [planes][outp][inp]
+  void* Wc = nullptr;     // [planes][inp][outp]
   float* bias = nullptr;  // [outp]
+  // weight-gradient GEMM plan (rows = inp, cols = outp, K = batch)
+  int wg_cfg = 0, wg_rt = 0, wg_ct = 0, wg_ks = 1;
+  float* wpart = nullptr; // split-K partials [wg_ks][in*out] when wg_ks > 1
 };
 
 struct GemmPlan {
@@ -49,7 +52,7 @@ struct GemmPlan {
 
 struct node_handle {
   node_desc desc{};
-  int L = 0, D = 0, Dp = 0, B = 0, Bp = 0, prec = 0, esz = 4;
+  int L = 0, D = 0, Dp = 0, B = 0, Bp = 0, prec = 0, esz = 4, planes = 1;
   int64_t P = 0;
   std::vector<Layer> layers;
   hipStream_t stream = nullptr;
@@ -87,10 +90,6 @@ struct node_handle {
   float* gf1 = nullptr;
   float* dLdout_dev = nullptr;
   int64_t dLdout_cap = 0;
-  void** d_dT = nullptr;
-  int* d_outs = nullptr;
-  int64_t* d_b0s = nullptr;
-  int bias_rows = 0;
   float* dp32 = nullptr;
 
   bool have_params = false, have_forward = false;
@@ -149,9 +148,12 @@ void dfree(node_handle* h, void* p) {
   (void)hipFree(p);
 }
 
-GemmPlan plan(int rows, int cols) {
+// Tile configuration with >= 240 workgroups if one exists (largest tile
+// first), else the one with the most workgroups.  min_cfg > 0 skips the
+// 128x128 tile (the stage epilogues use at most 64 columns per tile).
+GemmPlan plan(int rows, int cols, int min_cfg = 0) {
   int best = -1, best_tiles = -1;
-  for (int c = 0; c < gemm_cfg_count(); ++c) {
+  for (int c = min_cfg; c < gemm_cfg_count(); ++c) {
     const int br = gemm_cfg_tile_rows(c), bc = gemm_cfg_tile_cols(c);
     if (rows % br || cols % bc) continue;
     const int t = (rows / br) * (cols / bc);
@@ -212,7 +214,7 @@ int validate_desc(const node_desc* d, std::string& msg) {
   for (int i = 0; i < d->n_layers; ++i)
     if (d->acts[i] < 0 || d->acts[i] > 4) { msg = "unknown activation"; return NODE_ERR_INVALID; }
   if (d->input_map != NODE_MAP_IDENTITY && d->input_map != NODE_MAP_CUBE) { msg = "unknown input_map"; return NODE_ERR_INVALID; }
-  if (d->precision != NODE_PREC_FP32 && d->precision != NODE_PREC_BF16) { msg = "unknown precision"; return NODE_ERR_INVALID; }
+  if (d->precision < NODE_PREC_FP32 || d->precision > NODE_PREC_BF16X3) { msg = "unknown precision"; return NODE_ERR_INVALID; }
   if (d->batch < 1 || d->batch > (int64_t)1 << 24) { msg = "batch must be 1..2^24"; return NODE_ERR_INVALID; }
   return 0;
 }
@@ -223,26 +225,31 @@ void rhs_eval(node_handle* h, int stage, int mode, int gate, double* part) {
   const int L = h->L;
   for (int l = 0; l < L; ++l) {
     const Layer& ly = h->layers[l];
-    GemmArgs g;
+    GemmArgs g{};
     g.A = l == 0 ? h->X0 : h->H[l - 1];
     g.lda = ly.inp;
     g.B = ly.Wr;
     g.ldb = ly.inp;
     g.K = ly.inp;
+    g.pa = (int64_t)h->Bp * ly.inp;
+    g.pb = (int64_t)ly.outp * ly.inp;
+    g.ksplit = 1;
     if (l < L - 1) {
       GemmPlan p = plan(h->Bp, ly.outp);
       g.row_tiles = p.rt; g.col_tiles = p.ct;
-      EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], nullptr, nullptr};
+      EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], nullptr, nullptr,
+                  (int64_t)h->Bp * ly.outp, 0};
       ProfScope ps(h, PR_FWD_HIDDEN, l);
       launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
     } else {
-      GemmPlan p = plan(h->Bp, ly.outp);
+      GemmPlan p = plan(h->Bp, ly.outp, 1);
       g.row_tiles = p.rt; g.col_tiles = p.ct;
       EpiFwdStage e{};
       e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
       e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D; e.sv = (int64_t)h->Bp * h->D;
       e.X0 = h->X0; e.ldx = h->Dp; e.map = h->desc.input_map;
       e.scratch = h->scratch; e.part = part;
+      e.px = (int64_t)h->Bp * h->Dp;
       ProfScope ps(h, PR_FWD_STAGE, l);
       launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
     }
@@ -250,7 +257,7 @@ void rhs_eval(node_handle* h, int stage, int mode, int gate, double* part) {
 }
 
 int rhs_last_tiles(node_handle* h) {
-  GemmPlan p = plan(h->Bp, h->layers[h->L - 1].outp);
+  GemmPlan p = plan(h->Bp, h->layers[h->L - 1].outp, 1);
   return p.rt * p.ct;
 }
 
@@ -262,12 +269,14 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
   // forward recompute of the hidden layers (activations, transposes, act')
   for (int l = 0; l < L - 1; ++l) {
     const Layer& ly = h->layers[l];
-    GemmArgs g;
+    GemmArgs g{};
     g.A = l == 0 ? h->X0 : h->H[l - 1];
     g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
+    g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
     GemmPlan p = plan(h->Bp, ly.outp);
     g.row_tiles = p.rt; g.col_tiles = p.ct;
-    EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], h->HT[l], h->S[l]};
+    const int64_t ph = (int64_t)h->Bp * ly.outp;
+    EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], h->HT[l], h->S[l], ph, ph};
     ProfScope ps(h, PR_ADJ_HIDDEN, l);
     launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
   }
@@ -276,28 +285,32 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
   wt.cb = cb; wt.ce = ce; wt.cb1 = cb1; wt.ce1 = ce1; wt.flags = wflags;
   auto wgrad = [&](int l) {
     const Layer& ly = h->layers[l];
-    GemmArgs g;
+    GemmArgs g{};
     g.A = l == 0 ? h->X0T : h->HT[l - 1];
     g.lda = h->Bp;
     g.B = l == L - 1 ? h->LmT : h->DlT[l];
     g.ldb = h->Bp;
     g.K = h->Bp;
-    GemmPlan p = plan(ly.inp, ly.outp);
-    g.row_tiles = p.rt; g.col_tiles = p.ct;
+    g.pa = (int64_t)ly.inp * h->Bp;
+    g.pb = (int64_t)ly.outp * h->Bp;
+    g.row_tiles = ly.wg_rt; g.col_tiles = ly.wg_ct; g.ksplit = ly.wg_ks;
     EpiWgrad e = wt;
     e.in = ly.in; e.out = ly.out; e.w0 = ly.w0;
+    e.wpart = ly.wg_ks > 1 ? ly.wpart : nullptr;
     ProfScope ps(h, PR_WGRAD, l);
-    launch_gemm_wgrad(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+    launch_gemm_wgrad(h->prec, ly.wg_cfg, g, e, h->ctl, gate, h->stream);
   };
   // data gradients of layers L..2 (each followed by that layer's weight gradient)
   for (int l = L - 1; l >= 1; --l) {
     const Layer& ly = h->layers[l];
-    GemmArgs g;
+    GemmArgs g{};
     g.A = l == L - 1 ? h->Lm : h->Dl[l];
     g.lda = ly.outp; g.B = ly.Wc; g.ldb = ly.outp; g.K = ly.outp;
+    g.pa = (int64_t)h->Bp * ly.outp; g.pb = (int64_t)ly.inp * ly.outp; g.ksplit = 1;
     GemmPlan p = plan(h->Bp, ly.inp);
     g.row_tiles = p.rt; g.col_tiles = p.ct;
-    EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1]};
+    const int64_t pd = (int64_t)h->Bp * ly.inp;
+    EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd};
     {
       ProfScope ps(h, PR_BWD_DATA, l);
       launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
@@ -305,14 +318,35 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
     wgrad(l);
   }
   wgrad(0);
-  launch_bias_fold(h->prec, L, (const void* const*)h->d_dT, h->d_outs, h->d_b0s, h->Bp, h->Bp,
-                   h->bias_rows, wt, h->ctl, gate, h->stream);
+  {
+    FoldArgs fa{};
+    fa.nl = L;
+    for (int l = 0; l < L; ++l) {
+      const Layer& ly = h->layers[l];
+      fa.dT[l] = l == L - 1 ? h->LmT : h->DlT[l];
+      fa.dplane[l] = (int64_t)ly.outp * h->Bp;
+      fa.outs[l] = ly.out;
+      fa.b0s[l] = ly.b0;
+      fa.bias_rows += ly.out;
+      if (ly.wg_ks > 1) {
+        fa.sk_w0[fa.nsk] = ly.w0;
+        fa.sk_n[fa.nsk] = (int64_t)ly.in * ly.out;
+        fa.sk_part[fa.nsk] = ly.wpart;
+        fa.sk_split[fa.nsk] = ly.wg_ks;
+        fa.nsk++;
+      }
+    }
+    fa.ldt = h->Bp;
+    fa.Bp = h->Bp;
+    launch_fold(h->prec, fa, wt, h->ctl, gate, h->stream);
+  }
   // layer-1 data gradient with the stage epilogue
   const Layer& l0 = h->layers[0];
-  GemmArgs g;
+  GemmArgs g{};
   g.A = L == 1 ? h->Lm : h->Dl[0];
   g.lda = l0.outp; g.B = l0.Wc; g.ldb = l0.outp; g.K = l0.outp;
-  GemmPlan p = plan(h->Bp, h->Dp);
+  g.pa = (int64_t)h->Bp * l0.outp; g.pb = (int64_t)l0.inp * l0.outp; g.ksplit = 1;
+  GemmPlan p = plan(h->Bp, h->Dp, 1);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   EpiBwdLast e{};
   e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
@@ -322,12 +356,13 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
   e.X0 = h->X0; e.X0T = h->X0T; e.ldx = h->Dp; e.ldxt = h->Bp; e.map = h->desc.input_map;
   e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D;
   e.part = part; e.plain_out = plain_out;
+  e.pl = e.plt = e.px = e.pxt = (int64_t)h->Bp * h->Dp;
   ProfScope ps(h, PR_BWD_LAST, 0);
   launch_gemm_bwdlast(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
 }
 
 int vjp_last_tiles(node_handle* h) {
-  GemmPlan p = plan(h->Bp, h->Dp);
+  GemmPlan p = plan(h->Bp, h->Dp, 1);
   return p.rt * p.ct;
 }
 
@@ -355,7 +390,7 @@ int ensure_tape(node_handle* h, int64_t entries) {
 int ensure_adjoint(node_handle* h) {
   if (h->mu) return 0;
   const int64_t sv = (int64_t)h->Bp * h->D;
-  const size_t act = (size_t)h->Bp * h->Dp * h->esz;
+  const size_t act = (size_t)h->Bp * h->Dp * h->esz * h->planes;
   if (int rc = dalloc_t(h, &h->lam, sv * 4)) return rc;
   if (int rc = dalloc_t(h, &h->lam_new, sv * 4)) return rc;
   if (int rc = dalloc_t(h, &h->kl, 8 * sv * 4)) return rc;
@@ -366,24 +401,10 @@ int ensure_adjoint(node_handle* h) {
   if (int rc = dalloc_t(h, &h->acce, h->P * 4)) return rc;
   if (int rc = dalloc_t(h, &h->gf0, h->P * 4)) return rc;
   if (int rc = dalloc_t(h, &h->gf1, h->P * 4)) return rc;
-  // per-layer transposed delta pointers for the bias fold (layer L uses lambda^T)
-  std::vector<void*> dT(h->L);
-  std::vector<int> outs(h->L);
-  std::vector<int64_t> b0s(h->L);
-  int rows = 0;
-  for (int l = 0; l < h->L; ++l) {
-    dT[l] = l == h->L - 1 ? h->LmT : h->DlT[l];
-    outs[l] = h->layers[l].out;
-    b0s[l] = h->layers[l].b0;
-    rows += outs[l];
-  }
-  h->bias_rows = rows;
-  if (int rc = dalloc_t(h, &h->d_dT, h->L * sizeof(void*))) return rc;
-  if (int rc = dalloc_t(h, &h->d_outs, h->L * sizeof(int))) return rc;
-  if (int rc = dalloc_t(h, &h->d_b0s, h->L * sizeof(int64_t))) return rc;
-  HIPCHK(h, hipMemcpyAsync(h->d_dT, dT.data(), h->L * sizeof(void*), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->d_outs, outs.data(), h->L * sizeof(int), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->d_b0s, b0s.data(), h->L * sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+  for (Layer& ly : h->layers)
+    if (ly.wg_ks > 1 && !ly.wpart)
+      if (int rc = dalloc_t(h, &ly.wpart, (size_t)ly.wg_ks * ly.in * ly.out * sizeof(float)))
+        return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -393,6 +414,7 @@ AdjBufs adj_bufs(node_handle* h) {
   a.lam = h->lam; a.lam_new = h->lam_new; a.kl = h->kl; a.U = h->ucur;
   a.L = h->Lm; a.LT = h->LmT; a.X0 = h->X0; a.X0T = h->X0T;
   a.ldl = h->Dp; a.ldlt = h->Bp; a.ldx = h->Dp; a.ldxt = h->Bp;
+  a.pl = a.plt = a.px = a.pxt = (int64_t)h->Bp * h->Dp;
   a.tape = h->tape; a.fwd_times = h->fwd_times; a.cb_times = h->save_times;
   a.dLdout = h->dLdout_dev;
   a.mu = h->mu; a.accb = h->accb; a.acce = h->acce; a.gf0 = h->gf0; a.gf1 = h->gf1;
@@ -488,13 +510,14 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
   double* p0 = h->part;
   double* px = h->part + SEC;
   const int nb = state_blocks(sg);
-  launch_fwd_init(sg, d_u0, h->tape, h->X0, h->ctl, p0, nb, h->stream);
+  const int64_t xpl = (int64_t)h->Bp * h->Dp;  // X0 plane stride
+  launch_fwd_init(sg, d_u0, h->tape, h->X0, xpl, h->ctl, p0, nb, h->stream);
   rhs_eval(h, 0, SM_INIT_F0, GATE_ALWAYS, px);
   const int nx = rhs_last_tiles(h);
   const double nel = (double)h->B * h->D;
   if (!c.has_dt_) {
     launch_initdt_a(h->ctl, p0, nb, px, nx, nullptr, 0, nel, h->stream);
-    launch_fwd_initdt_prep(sg, h->tape, h->X0, h->ctl, h->stream);
+    launch_fwd_initdt_prep(sg, h->tape, h->X0, xpl, h->ctl, h->stream);
     rhs_eval(h, 0, SM_INIT_F1, GATE_ALWAYS, px);
   }
   launch_initdt_b(h->ctl, px, nx, nullptr, 0, nel, h->save_times, 0, h->fwd_times, h->stream);
@@ -509,7 +532,7 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
     enqueued += chunk;
     if (int rc = ensure_tape(h, h->hctl->n + chunk + 2)) return rc;
     for (int64_t k = 0; k < chunk; ++k) {
-      launch_fwd_step_begin(sg, h->tape, h->X0, h->save_times, h->out_dev, h->ctl, h->stream);
+      launch_fwd_step_begin(sg, h->tape, h->X0, xpl, h->save_times, h->out_dev, h->ctl, h->stream);
       for (int s = 1; s < 7; ++s) rhs_eval(h, s, SM_STEP, GATE_ACTIVE, px);
       launch_fwd_controller(h->ctl, px, nx, h->save_times, h->fwd_times, nel, h->stream);
     }
@@ -517,7 +540,7 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
     chunk = std::min<int64_t>(chunk * 2, 16);
   }
   // pending saves of the final step
-  launch_fwd_step_begin(sg, h->tape, h->X0, h->save_times, h->out_dev, h->ctl, h->stream);
+  launch_fwd_step_begin(sg, h->tape, h->X0, xpl, h->save_times, h->out_dev, h->ctl, h->stream);
   if (int rc = poll_ctl(h)) return rc;
   const Ctl& r = *h->hctl;
   h->n_fwd = r.n;
@@ -678,7 +701,8 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
   h->B = (int)desc->batch;
   h->Bp = (int)rup(h->B, BPAD);
   h->prec = desc->precision;
-  h->esz = h->prec == NODE_PREC_BF16 ? 2 : 4;
+  h->esz = h->prec == NODE_PREC_FP32 ? 4 : 2;
+  h->planes = h->prec == NODE_PREC_BF16X3 ? 2 : 1;
   h->P = node_num_params(desc);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
@@ -697,16 +721,24 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
     ly.w0 = off;
     ly.b0 = off + (int64_t)ly.in * ly.out;
     off = ly.b0 + ly.out;
-    rc = dalloc(h, &ly.Wr, (size_t)ly.outp * ly.inp * h->esz);
-    if (!rc) rc = dalloc(h, &ly.Wc, (size_t)ly.outp * ly.inp * h->esz);
+    rc = dalloc(h, &ly.Wr, (size_t)ly.outp * ly.inp * h->esz * h->planes);
+    if (!rc) rc = dalloc(h, &ly.Wc, (size_t)ly.outp * ly.inp * h->esz * h->planes);
+    {
+      // weight-gradient plan: narrow layers get split-K so that the grid fills the chip
+      GemmPlan p = plan(ly.inp, ly.outp);
+      ly.wg_cfg = p.cfg; ly.wg_rt = p.rt; ly.wg_ct = p.ct; ly.wg_ks = 1;
+      const int ktiles = (int)((int64_t)h->Bp * h->esz / 128);
+      while (p.rt * p.ct * ly.wg_ks < 256 && ly.wg_ks < 16 && ktiles % (2 * ly.wg_ks) == 0)
+        ly.wg_ks *= 2;
+    }
     if (!rc) rc = dalloc_t(h, &ly.bias, (size_t)ly.outp * sizeof(float));
     h->layers.push_back(ly);
   }
-  const size_t x0b = (size_t)h->Bp * h->Dp * h->esz;
+  const size_t x0b = (size_t)h->Bp * h->Dp * h->esz * h->planes;
   if (!rc) rc = dalloc(h, &h->X0, x0b);
   if (!rc) rc = dalloc(h, &h->X0T, x0b);
   for (int l = 0; l < h->L - 1 && !rc; ++l) {
-    const size_t hb = (size_t)h->Bp * h->layers[l].outp * h->esz;
+    const size_t hb = (size_t)h->Bp * h->layers[l].outp * h->esz * h->planes;
     void *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
     rc = dalloc(h, &a, hb);
     if (!rc) rc = dalloc(h, &b, hb);
@@ -755,8 +787,9 @@ static int set_params_host(node_handle* h, const float* p, int64_t n) {
         wc[(size_t)i * ly.outp + o] = w;
       }
     for (int o = 0; o < ly.out; ++o) b[o] = p[ly.b0 + o];
-    if (h->prec == NODE_PREC_BF16) {
-      std::vector<uint16_t> r16(cnt), c16(cnt);
+    if (h->prec != NODE_PREC_FP32) {
+      const int np = h->planes;
+      std::vector<uint16_t> r16(cnt * np), c16(cnt * np);
       auto tobf = [](float f) {
         uint32_t u;
         std::memcpy(&u, &f, 4);
@@ -764,9 +797,22 @@ static int set_params_host(node_handle* h, const float* p, int64_t n) {
         u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
         return (uint16_t)(u >> 16);
       };
-      for (size_t k = 0; k < cnt; ++k) { r16[k] = tobf(wr[k]); c16[k] = tobf(wc[k]); }
-      HIPCHK(h, hipMemcpy(ly.Wr, r16.data(), cnt * 2, hipMemcpyHostToDevice));
-      HIPCHK(h, hipMemcpy(ly.Wc, c16.data(), cnt * 2, hipMemcpyHostToDevice));
+      auto frombf = [](uint16_t b) {
+        uint32_t u = (uint32_t)b << 16;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+      };
+      for (size_t k = 0; k < cnt; ++k) {
+        r16[k] = tobf(wr[k]);
+        c16[k] = tobf(wc[k]);
+        if (np == 2) {  // lo plane: the rounding residual
+          r16[cnt + k] = tobf(wr[k] - frombf(r16[k]));
+          c16[cnt + k] = tobf(wc[k] - frombf(c16[k]));
+        }
+      }
+      HIPCHK(h, hipMemcpy(ly.Wr, r16.data(), cnt * 2 * np, hipMemcpyHostToDevice));
+      HIPCHK(h, hipMemcpy(ly.Wc, c16.data(), cnt * 2 * np, hipMemcpyHostToDevice));
     } else {
       HIPCHK(h, hipMemcpy(ly.Wr, wr.data(), cnt * 4, hipMemcpyHostToDevice));
       HIPCHK(h, hipMemcpy(ly.Wc, wc.data(), cnt * 4, hipMemcpyHostToDevice));
@@ -883,7 +929,8 @@ int32_t node_rhs(node_handle_t* h, const float* u, float* du) {
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
   const StateGeom sg = geom(h);
   HIPCHK(h, hipMemcpyAsync(h->ucur, u, (size_t)h->B * h->D * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  launch_prep_plain(sg, h->ucur, h->X0, nullptr, nullptr, h->Dp, h->Bp, h->stream);
+  const int64_t px = (int64_t)h->Bp * h->Dp;
+  launch_prep_plain(sg, h->ucur, h->X0, nullptr, nullptr, h->Dp, h->Bp, px, px, h->stream);
   rhs_eval(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(du, h->scratch, (size_t)h->B * h->D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
@@ -900,8 +947,9 @@ int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda, float* d
   const size_t sb = (size_t)h->B * h->D * sizeof(float);
   HIPCHK(h, hipMemcpyAsync(h->ucur, u, sb, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->lam, lambda, sb, hipMemcpyHostToDevice, h->stream));
-  launch_prep_plain(sg, h->ucur, h->X0, h->X0T, nullptr, h->Dp, h->Bp, h->stream);
-  launch_prep_lambda(sg, h->lam, h->Lm, h->LmT, h->Dp, h->Bp, h->stream);
+  const int64_t px = (int64_t)h->Bp * h->Dp;
+  launch_prep_plain(sg, h->ucur, h->X0, h->X0T, h->ucur, h->Dp, h->Bp, px, px, h->stream);
+  launch_prep_lambda(sg, h->lam, h->Lm, h->LmT, h->Dp, h->Bp, px, px, h->stream);
   if (!h->dp32 && dalloc_t(h, &h->dp32, h->P * sizeof(float))) return NODE_ERR_HIP;
   vjp_eval(h, 0, SM_PLAIN, GATE_ALWAYS, WG_STORE_PTR, 0, 0, 0, 0, nullptr, h->scratch, h->dp32);
   HIPCHK(h, hipGetLastError());

@@ -49,14 +49,17 @@ struct Ctl {
 struct GemmArgs {
   const void* A; int64_t lda;  // rows x K, K contiguous
   const void* B; int64_t ldb;  // cols x K, K contiguous
-  int K;                       // multiple of the K tile
+  int64_t pa, pb;              // plane strides (elements) of the hi/lo planes (split BF16)
+  int K;                       // multiple of the K tile (times ksplit)
   int row_tiles, col_tiles;
+  int ksplit;                  // split-K factor (weight gradients of narrow layers)
 };
 
 // hidden layer forward: Y = act(acc + bias) (+ transposed copy, + act')
 struct EpiHidden {
   const float* bias; int act; int C; int ldo; int ldt;
   void* Y; void* YT; void* S;
+  int64_t py, pyt;             // plane strides of Y / YT
 };
 
 // last forward layer: k_s = acc + bias, then the next Tsit5 stage input
@@ -66,11 +69,13 @@ struct EpiFwdStage {
   void* X0; int ldx; int map;
   float* scratch;                          // initdt f1 / plain rhs output
   double* part;
+  int64_t px;                              // plane stride of X0
 };
 
 // hidden data gradient: delta = acc * act'(z)  (row-major + transposed)
 struct EpiBwdData {
   int C; int ldo; int ldt; const void* S; void* Dl; void* DlT;
+  int64_t pd, pdt;             // plane strides of Dl / DlT
 };
 
 // layer-1 data gradient: k^lambda_s = -(acc * map'(u)), next stage inputs
@@ -83,6 +88,7 @@ struct EpiBwdLast {
   const float* tape; int64_t ent;
   double* part;
   float* plain_out;                        // node_vjp: write -k (= lambda^T df/du)
+  int64_t pl, plt, px, pxt;                // plane strides of L / LT / X0 / X0T
 };
 
 // weight gradient (K = batch): fold into the mu accumulators
@@ -92,6 +98,23 @@ struct EpiWgrad {
   int in, out; int64_t w0;
   float* accb; float* acce; float* gf0; float* gf1; float* gptr;
   float cb, ce, cb1, ce1; int flags;
+  float* wpart;                // split-K partials [ksplit][in*out] (fold kernel reduces)
+};
+
+// fold kernel work: bias rows of every layer + split-K partial sums of the
+// narrow layers' weight gradients
+struct FoldArgs {
+  int nl;                      // layers
+  const void* dT[MAXL];        // transposed delta of each layer's output (K = batch)
+  int64_t dplane[MAXL];
+  int outs[MAXL];
+  int64_t b0s[MAXL];
+  int bias_rows;
+  int ldt, Bp;
+  int nsk;                     // narrow layers with split-K partials
+  int64_t sk_w0[MAXL], sk_n[MAXL];  // flat offset / entry count
+  float* sk_part[MAXL];
+  int sk_split[MAXL];
 };
 
 // tableau in constant memory
@@ -117,21 +140,21 @@ int gemm_cfg_count();
 // elementwise / control kernels
 struct StateGeom {
   int B, Bp, D, Dp;          // real / padded batch, state dim, padded state dim
-  int map, prec;
+  int map, prec;             // prec: 0 fp32, 1 bf16, 2 split bf16 (hi/lo planes)
   int64_t sv;                // floats per state vector (Bp * D)
   int64_t ent;               // floats per tape entry (8 * sv)
 };
 
 void launch_fwd_init(const StateGeom& sg, const float* u0, float* tape, void* X0,
-                     const Ctl* ctl, double* part, int nblk, hipStream_t s);
+                     int64_t px, const Ctl* ctl, double* part, int nblk, hipStream_t s);
 void launch_initdt_a(Ctl* ctl, const double* p0, int n0, const double* px, int nx,
                      const double* pm, int nm, double n_elems, hipStream_t s);
 void launch_initdt_b(Ctl* ctl, const double* px, int nx, const double* pm, int nm,
                      double n_elems, const double* save_times, int is_adjoint,
                      const double* fwd_times, hipStream_t s);
-void launch_fwd_initdt_prep(const StateGeom& sg, const float* tape, void* X0,
+void launch_fwd_initdt_prep(const StateGeom& sg, const float* tape, void* X0, int64_t px,
                             const Ctl* ctl, hipStream_t s);
-void launch_fwd_step_begin(const StateGeom& sg, const float* tape, void* X0,
+void launch_fwd_step_begin(const StateGeom& sg, const float* tape, void* X0, int64_t px,
                            const double* save_times, float* out, const Ctl* ctl,
                            hipStream_t s);
 void launch_fwd_controller(Ctl* ctl, const double* part, int n_part,
@@ -144,6 +167,7 @@ struct AdjBufs {
   float* lam; float* lam_new; float* kl; float* U;
   void* L; void* LT; void* X0; void* X0T;
   int ldl, ldlt, ldx, ldxt;
+  int64_t pl, plt, px, pxt;  // plane strides
   const float* tape; const double* fwd_times;
   const double* cb_times; const float* dLdout;
   double* mu; float* accb; float* acce; float* gf0; float* gf1;
@@ -162,15 +186,14 @@ void launch_adj_controller(Ctl* ctl, const double* part, int n_lam,
                            const double* part_mu, int n_mu, double n_elems,
                            const double* cb_times, const double* fwd_times,
                            hipStream_t s);
-void launch_bias_fold(int prec, int nl, const void* const* dT, const int* outs,
-                      const int64_t* b0s, int ldt, int Bp, int rows_total,
-                      const EpiWgrad& tmpl, const Ctl* ctl, int gate, hipStream_t s);
+void launch_fold(int prec, const FoldArgs& fa, const EpiWgrad& tmpl, const Ctl* ctl, int gate,
+                 hipStream_t s);
 void launch_set_stage_interp(Ctl* ctl, int slot, double tau, const double* fwd_times,
                              hipStream_t s);
 void launch_prep_plain(const StateGeom& sg, const float* u, void* X0, void* X0T,
-                       float* Ucur, int ldx, int ldxt, hipStream_t s);
+                       float* Ucur, int ldx, int ldxt, int64_t px, int64_t pxt, hipStream_t s);
 void launch_prep_lambda(const StateGeom& sg, const float* lam, void* L, void* LT,
-                        int ldl, int ldlt, hipStream_t s);
+                        int ldl, int ldlt, int64_t pl, int64_t plt, hipStream_t s);
 void launch_gather(const StateGeom& sg, const float* src, int64_t src_stride_floats,
                    float* dst, int64_t n, hipStream_t s);
 

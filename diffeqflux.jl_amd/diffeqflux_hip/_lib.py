@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnode_hip.so")
 MAX_LAYERS = 8
 ACT = {"identity": 0, "tanh": 1, "softplus": 2, "relu": 3, "sigmoid": 4}
 MAP_IDENTITY, MAP_CUBE = 0, 1
-PREC = {"fp32": 0, "bf16": 1}
+PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 RET_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN",
              4: "MaxIters", 5: "DtLessThanMin", 6: "Unstable"}
 ERR_NAMES = {-1: "NODE_ERR_INVALID", -2: "NODE_ERR_HIP", -3: "NODE_ERR_STATE",

@@ -47,9 +47,16 @@ enum { NODE_ACT_IDENTITY = 0, NODE_ACT_TANH = 1, NODE_ACT_SOFTPLUS = 2,
  * docs/src/examples/neural_ode.md:32) */
 enum { NODE_MAP_IDENTITY = 0, NODE_MAP_CUBE = 1 };
 /* GEMM precision of the right-hand side and its VJPs.  State, stage sums and
- * adjoint state stay FP32 in both; BF16 runs the GEMMs on BF16 MFMA with FP32
- * accumulation. */
-enum { NODE_PREC_FP32 = 0, NODE_PREC_BF16 = 1 };
+ * adjoint state stay FP32 in all modes.
+ *  FP32   : FP32 operands on FP32 MFMA.
+ *  BF16   : BF16 operands on BF16 MFMA, FP32 accumulation.  Activation
+ *           rounding makes the RHS jitter at the 2^-9 level, which Tsit5's error
+ *           estimate sees at tight tolerances (more steps), and weight rounding
+ *           bounds the gradient accuracy to ~1e-3..1e-2.
+ *  BF16X3 : every operand split into two BF16 planes (hi + lo); each GEMM runs
+ *           three BF16 MFMA passes (hi*hi + lo*hi + hi*lo), keeping ~16
+ *           mantissa bits: FP32-like step counts and gradients. */
+enum { NODE_PREC_FP32 = 0, NODE_PREC_BF16 = 1, NODE_PREC_BF16X3 = 2 };
 
 /* SciMLBase.ReturnCode numbering */
 enum { NODE_RET_DEFAULT = 0, NODE_RET_SUCCESS = 1, NODE_RET_TERMINATED = 2,
