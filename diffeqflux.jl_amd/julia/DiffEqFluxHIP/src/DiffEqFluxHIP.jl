@@ -1,0 +1,232 @@
+"""
+    DiffEqFluxHIP
+
+Routes DiffEqFlux's `NeuralODE` (src/neural_de.jl:35-61 of the reference) to
+the MI355X HIP library `libnode_hip.so` (C ABI: include/node_hip.h) for the
+supported case, and falls back to the reference method otherwise:
+
+  * model: a Lux `Chain` of `Dense` layers (identity / tanh / softplus / relu /
+    sigmoid), optionally preceded by `x -> x.^3`, with equal input/output width;
+  * alg: `Tsit5()`;
+  * sensealg: the default `InterpolatingAdjoint(autojacvec = ZygoteVJP())`;
+  * kwargs: saveat (range, vector or number), save_everystep, save_start,
+    save_end, abstol, reltol, dt, adaptive, dtmax, dtmin, maxiters;
+  * u0: `Vector{Float32}` (D) or `Matrix{Float32}` (D x B), t1 > t0.
+
+`ChainRulesCore.rrule(::typeof(hip_neuralode), ...)` is the pullback: it
+calls `node_backward` (the HIP InterpolatingAdjoint) on the forward tape kept
+in the handle.  Julia was not available where this package was written; the
+call contract it binds is exercised from Python through the same C ABI.
+"""
+module DiffEqFluxHIP
+
+using ChainRulesCore: ChainRulesCore, NoTangent, ZeroTangent, unthunk
+using DiffEqFlux: DiffEqFlux, NeuralODE
+using Lux: Lux, Chain, Dense, WrappedFunction
+using OrdinaryDiffEq: Tsit5
+using SciMLSensitivity: InterpolatingAdjoint, ZygoteVJP
+
+const CRC = ChainRulesCore
+
+libnode() = get(ENV, "DIFFEQFLUX_HIP_LIB",
+                joinpath(@__DIR__, "..", "..", "..", "diffeqflux_hip", "libnode_hip.so"))
+
+const MAX_LAYERS = 8
+
+# ---- C structs (layout of include/node_hip.h)
+struct NodeDesc
+    n_layers::Int32
+    widths::NTuple{MAX_LAYERS + 1, Int32}
+    acts::NTuple{MAX_LAYERS, Int32}
+    input_map::Int32
+    precision::Int32
+    batch::Int64
+end
+
+struct SolveOpts
+    t0::Float64; t1::Float64; abstol::Float64; reltol::Float64
+    adaptive::Int32; save_start::Int32; save_end::Int32; save_everystep::Int32
+    dt::Float64; dtmax::Float64; dtmin::Float64; maxiters::Int64
+end
+
+mutable struct Stats
+    nf::Int64; naccept::Int64; nreject::Int64; nsaved::Int64
+    retcode::Int32; reserved::Int32; t_final::Float64
+end
+Stats() = Stats(0, 0, 0, 0, 0, 0, 0.0)
+
+const ACT = Dict(identity => Int32(0), tanh => Int32(1), Lux.NNlib.softplus => Int32(2),
+                 Lux.NNlib.relu => Int32(3), Lux.NNlib.sigmoid => Int32(4))
+const RETCODES = (:Default, :Success, :Terminated, :DtNaN, :MaxIters, :DtLessThanMin,
+                  :Unstable)
+
+# ---- handle
+mutable struct Handle
+    ptr::Ptr{Cvoid}
+    D::Int
+    B::Int
+    P::Int
+    saved_t::Vector{Float64}
+    function Handle(desc::NodeDesc)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:node_create, libnode()), Int32, (Ref{NodeDesc}, Ref{Ptr{Cvoid}}), desc, r)
+        rc == 0 || error("node_create failed ($rc)")
+        P = ccall((:node_num_params, libnode()), Int64, (Ref{NodeDesc},), desc)
+        h = new(r[], Int(desc.widths[1]), Int(desc.batch), Int(P), Float64[])
+        finalizer(x -> ccall((:node_destroy, libnode()), Int32, (Ptr{Cvoid},), x.ptr), h)
+        return h
+    end
+end
+
+function check(h::Handle, rc, what)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:node_last_error, libnode()), Cstring, (Ptr{Cvoid},), h.ptr))
+    error("$what failed ($rc): $msg")
+end
+
+# ---- support check (anything else falls back to the reference path)
+function chain_spec(model)
+    model isa Chain || return nothing
+    layers = collect(values(model.layers))
+    imap = Int32(0)
+    if !isempty(layers) && layers[1] isa WrappedFunction
+        imap = Int32(1)          # x -> x.^3 is the only wrapped function on the path
+        layers = layers[2:end]
+    end
+    (isempty(layers) || length(layers) > MAX_LAYERS) && return nothing
+    all(l -> l isa Dense && haskey(ACT, l.activation) && Lux.has_bias(l), layers) ||
+        return nothing
+    widths = Int32[layers[1].in_dims; [l.out_dims for l in layers]...]
+    widths[1] == widths[end] || return nothing
+    acts = Int32[ACT[l.activation] for l in layers]
+    return widths, acts, imap
+end
+
+supported(n::NeuralODE, x) = chain_spec(n.model) !== nothing && length(n.args) >= 1 &&
+    n.args[1] isa Tsit5 && eltype(x) === Float32 && ndims(x) in (1, 2) &&
+    n.tspan[2] > n.tspan[1] &&
+    (!haskey(n.kwargs, :sensealg) || (n.kwargs[:sensealg] isa InterpolatingAdjoint &&
+                                      n.kwargs[:sensealg].autojacvec isa ZygoteVJP))
+
+# OrdinaryDiffEq's saving keywords -> interior save times + flags
+function save_plan(t0, t1, kw)
+    saveat = get(kw, :saveat, Float64[])
+    isnum = saveat isa Number
+    seq = isnum ? Float64[] : Float64.(collect(saveat))
+    empty = !isnum && isempty(seq)
+    sev = get(kw, :save_everystep, empty)
+    interior = isnum ? [t for t in collect(t0:saveat:t1)[2:end] if t0 < t < t1] :
+                       sort(unique([t for t in seq if t0 < t < t1]))
+    ss = get(kw, :save_start, sev || empty || isnum || t0 in seq)
+    se = get(kw, :save_end, sev || empty || isnum || t1 in seq)
+    return Float64.(interior), ss, se, sev
+end
+
+function make_opts(n::NeuralODE)
+    kw = n.kwargs
+    t0, t1 = Float64.(n.tspan)
+    interior, ss, se, sev = save_plan(t0, t1, kw)
+    o = SolveOpts(t0, t1, get(kw, :abstol, 1e-6), get(kw, :reltol, 1e-3),
+                  Int32(get(kw, :adaptive, true)), Int32(ss), Int32(se), Int32(sev),
+                  Float64(get(kw, :dt, 0.0)), Float64(get(kw, :dtmax, 0.0)),
+                  Float64(get(kw, :dtmin, 0.0)), Int64(get(kw, :maxiters, 0)))
+    return o, interior
+end
+
+const HANDLES = Dict{Tuple{UInt, Int}, Handle}()
+
+function handle_for(n::NeuralODE, B::Int; precision::Int32 = Int32(0))
+    key = (objectid(n), B)
+    get!(HANDLES, key) do
+        widths, acts, imap = chain_spec(n.model)
+        w = ntuple(i -> i <= length(widths) ? widths[i] : Int32(0), MAX_LAYERS + 1)
+        a = ntuple(i -> i <= length(acts) ? acts[i] : Int32(0), MAX_LAYERS)
+        Handle(NodeDesc(Int32(length(acts)), w, a, imap, precision, Int64(B)))
+    end
+end
+
+"""
+    hip_neuralode(x, p_flat, h, opts, interior) -> Array (D x B x Nt) and times
+
+Forward Tsit5 solve on the GPU (node_forward).  Differentiable through the
+`rrule` below (node_backward = InterpolatingAdjoint).
+"""
+function hip_neuralode(x::AbstractArray{Float32}, p::AbstractVector{Float32}, h::Handle,
+                       o::SolveOpts, interior::Vector{Float64})
+    check(h, ccall((:node_set_params, libnode()), Int32, (Ptr{Cvoid}, Ptr{Float32}, Int64),
+                   h.ptr, p, length(p)), "node_set_params")
+    st = Stats()
+    cap = o.save_everystep != 0 ? 0 : length(interior) + o.save_start + o.save_end
+    out = Array{Float32}(undef, h.D, h.B, max(cap, 1))
+    u0 = Matrix{Float32}(reshape(x, h.D, h.B))
+    rc = GC.@preserve u0 interior out ccall((:node_forward, libnode()), Int32,
+        (Ptr{Cvoid}, Ref{SolveOpts}, Ptr{Float32}, Ptr{Float64}, Int64, Ptr{Float32}, Int64,
+         Ref{Stats}), h.ptr, o, u0, interior, length(interior),
+        cap > 0 ? pointer(out) : C_NULL, cap, st)
+    check(h, rc, "node_forward")
+    if o.save_everystep != 0
+        out = Array{Float32}(undef, h.D, h.B, st.nsaved)
+        check(h, ccall((:node_copy_saved, libnode()), Int32, (Ptr{Cvoid}, Ptr{Float32}, Int64),
+                       h.ptr, out, st.nsaved), "node_copy_saved")
+    end
+    ts = Vector{Float64}(undef, st.nsaved)
+    check(h, ccall((:node_saved_times, libnode()), Int32, (Ptr{Cvoid}, Ptr{Float64}, Int64),
+                   h.ptr, ts, length(ts)), "node_saved_times")
+    return out[:, :, 1:st.nsaved], ts, RETCODES[st.retcode + 1]
+end
+
+function CRC.rrule(::typeof(hip_neuralode), x, p, h::Handle, o::SolveOpts, interior)
+    out, ts, ret = hip_neuralode(x, p, h, o, interior)
+    function hip_neuralode_pullback(Δ)
+        Δout = unthunk(Δ[1])
+        Δout isa Union{NoTangent, ZeroTangent} && (Δout = zero(out))
+        g = Array{Float32}(Δout)
+        du0 = Matrix{Float32}(undef, h.D, h.B)
+        dp = Vector{Float64}(undef, h.P)
+        st = Stats()
+        rc = ccall((:node_backward, libnode()), Int32,
+                   (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Ptr{Float64}, Ref{Stats}),
+                   h.ptr, g, du0, dp, st)
+        check(h, rc, "node_backward")
+        return NoTangent(), reshape(du0, size(x)), Float32.(dp), NoTangent(), NoTangent(),
+               NoTangent()
+    end
+    return (out, ts, ret), hip_neuralode_pullback
+end
+
+"""
+Solution returned on the HIP path: `.u`, `.t`, `.retcode`, `Array(sol)`,
+`last`, `sum`, indexing -- what downstream code uses of an `ODESolution`
+(docs/src/examples/neural_ode.md:37, GPUs.md:107, cuda_tests.jl:52).
+"""
+struct HIPSolution{A, T}
+    array::A
+    t::T
+    retcode::Symbol
+end
+Base.Array(s::HIPSolution) = s.array
+Base.getproperty(s::HIPSolution, f::Symbol) =
+    f === :u ? [selectdim(getfield(s, :array), ndims(getfield(s, :array)), i)
+                for i in 1:size(getfield(s, :array), ndims(getfield(s, :array)))] :
+    getfield(s, f)
+Base.last(s::HIPSolution) = selectdim(s.array, ndims(s.array), size(s.array, ndims(s.array)))
+Base.sum(s::HIPSolution) = sum(s.array)
+Base.length(s::HIPSolution) = length(s.t)
+
+# the drop-in: a more specific call method for supported NeuralODEs
+function (n::NeuralODE{<:Chain})(x::AbstractArray{Float32}, p, st)
+    supported(n, x) || return invoke(n, Tuple{Any, Any, Any}, x, p, st)  # reference path
+    B = ndims(x) == 1 ? 1 : size(x, 2)
+    h = handle_for(n, B)
+    o, interior = make_opts(n)
+    # flat Lux/ComponentArray order (test/BasicNeuralDE/neural_de_tests.jl:45); for a
+    # ComponentArray `Vector(p)` is already that vector and Zygote restructures dp
+    pflat = p isa AbstractVector ? Vector{Float32}(p) :
+        reduce(vcat, [vcat(vec(l.weight), l.bias) for l in values(p)
+                      if l isa NamedTuple && haskey(l, :weight)])
+    out, ts, ret = hip_neuralode(x, pflat, h, o, interior)
+    arr = ndims(x) == 1 ? reshape(out, size(out, 1), size(out, 3)) : out
+    return HIPSolution(arr, ts, ret), st
+end
+
+end # module
