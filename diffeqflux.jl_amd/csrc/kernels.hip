@@ -94,19 +94,33 @@ __device__ __forceinline__ bool gate_ok(const Ctl* c, int gate) {
   return true;
 }
 
+// Activations from the hardware exp2 / log2 / reciprocal (v_exp_f32,
+// v_log_f32, v_rcp_f32, ~1 ulp each): the epilogues run them once per output
+// element at low occupancy, where the full-range library forms dominated the
+// kernel time.  Absolute error stays at the FP32 rounding level of the result.
+#define LOG2E_F 1.4426950408889634f
+#define LN2_F 0.6931471805599453f
+__device__ __forceinline__ float fast_sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z * LOG2E_F));
+}
 __device__ __forceinline__ float act_f(int a, float z) {
   switch (a) {
-    case 1: return tanhf(z);
-    case 2: return log1pf(expf(-fabsf(z))) + fmaxf(z, 0.f);
+    case 1: {  // tanh(z) = sign(z) (1 - t) / (1 + t), t = exp(-2|z|)
+      const float t = __builtin_amdgcn_exp2f(-2.f * LOG2E_F * fabsf(z));
+      return copysignf((1.f - t) * __builtin_amdgcn_rcpf(1.f + t), z);
+    }
+    case 2:  // softplus: log1p(exp(-|z|)) + max(z, 0)
+      return fmaxf(z, 0.f) +
+             LN2_F * __builtin_amdgcn_logf(1.f + __builtin_amdgcn_exp2f(-fabsf(z) * LOG2E_F));
     case 3: return fmaxf(z, 0.f);
-    case 4: return 1.f / (1.f + expf(-z));
+    case 4: return fast_sigmoid(z);
     default: return z;
   }
 }
 __device__ __forceinline__ float act_d(int a, float z, float y) {
   switch (a) {
     case 1: return 1.f - y * y;
-    case 2: return 1.f / (1.f + expf(-z));
+    case 2: return fast_sigmoid(z);
     case 3: return z > 0.f ? 1.f : 0.f;
     case 4: return y * (1.f - y);
     default: return 1.f;
@@ -235,56 +249,155 @@ template <> struct Mma<float> {
       const int r0 = rb + mi * 32 + q * 8 + (lane >> 5) * 4;
 #define EPI_LOOP_END }}
 
-// ---------------------------------------------------------------- epilogues
-template <class PP, int MR, int MC>
-__device__ __forceinline__ void epi_apply(const EpiHidden& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl*, int) {
-  using T = typename PP::T;
-  using ST = typename PP::ST;
-  T* Y = (T*)e.Y;
-  T* YT = (T*)e.YT;
-  ST* S = (ST*)e.S;
-  EPI_LOOP_BEGIN
-    const bool cv = c < e.C;
-    const float bias = cv ? e.bias[c] : 0.f;
-    float yv[4];
+// ---------------------------------------------------------------- tile staging
+// Where a workgroup's tile sits and which wave block this thread owns.
+struct TileGeo {
+  int row0, col0;   // tile origin
+  int rb, cb;       // this wave's block origin (absolute)
+  int lr0, lc0;     // this wave's block origin inside the tile
+};
+
+typedef bf16 bf16x8v __attribute__((ext_vector_type(8)));
+
+// Copy a rows x cols FP32 tile staged in LDS (row pitch `pitch` floats) to
+// global memory as OT (16-byte stores along cols); SPLIT writes the bf16
+// rounding residual to the second plane (dst + plane).
+template <typename OT, bool SPLIT, int NT>
+__device__ __forceinline__ void tile_copy_out(const float* lt, int pitch, int rows, int cols,
+                                              OT* dst, int64_t ld, int64_t plane) {
+  constexpr int E = 16 / sizeof(OT);
+  const int cpr = cols / E;
+  for (int i = threadIdx.x; i < rows * cpr; i += NT) {
+    const int r = i / cpr, cc = (i - r * cpr) * E;
+    const float* src = lt + r * pitch + cc;
+    OT* d = dst + (int64_t)r * ld + cc;
+    if constexpr (sizeof(OT) == 4) {
+      *(f32x4v*)d = *(const f32x4v*)src;
+    } else {
+      const f32x4v a = *(const f32x4v*)src, b = *(const f32x4v*)(src + 4);
+      const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      bf16x8v hi, lo;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const float z = acc[mi][mj][q * 4 + p] + bias;
-      const float y = act_f(e.act, z);
-      yv[p] = cv ? y : 0.f;
-      const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
-      put<PP>(Y + o, e.py, yv[p]);
-      if (S) S[o] = cvt<ST>(cv ? act_d(e.act, z, y) : 0.f);
+      for (int k = 0; k < 8; ++k) {
+        hi[k] = (bf16)v[k];
+        if constexpr (SPLIT) lo[k] = (bf16)(v[k] - (float)hi[k]);
+      }
+      *(bf16x8v*)d = hi;
+      if constexpr (SPLIT) *(bf16x8v*)(d + plane) = lo;
     }
-    if (YT) put4<PP>(YT + (int64_t)c * e.ldt + r0, e.pyt, yv);
-  EPI_LOOP_END
+  }
 }
 
-template <class PP, int MR, int MC>
-__device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl*, int) {
+// Stage this thread's accumulator-layout values (vals[mi][mj][v]) into LDS as
+// a float tile, row-major [r][c] (TRANS = false) or transposed [c][r].
+template <int MR, int MC, bool TRANS>
+__device__ __forceinline__ void tile_stage(float* lt, int pitch, const f32x16 (&vals)[MR][MC],
+                                           const TileGeo& g, int lane) {
+#pragma unroll
+  for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+    for (int mj = 0; mj < MC; ++mj) {
+      const int c = g.lc0 + mj * 32 + (lane & 31);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r0 = g.lr0 + mi * 32 + q * 8 + (lane >> 5) * 4;
+        if constexpr (TRANS) {
+          *(f32x4v*)(lt + c * pitch + r0) =
+              f32x4v{vals[mi][mj][q * 4], vals[mi][mj][q * 4 + 1], vals[mi][mj][q * 4 + 2],
+                     vals[mi][mj][q * 4 + 3]};
+        } else {
+#pragma unroll
+          for (int p = 0; p < 4; ++p) lt[(r0 + p) * pitch + c] = vals[mi][mj][q * 4 + p];
+        }
+      }
+    }
+}
+
+// Write a BR x BC tile of values to a row-major [r][c] buffer (ld) and/or its
+// transposed [c][r] buffer (ldt) through LDS.  All threads must call it.
+template <typename OT, bool SPLIT, int BR, int BC, int NT, int MR, int MC>
+__device__ void tile_write(unsigned char* lds, const f32x16 (&vals)[MR][MC], const TileGeo& g,
+                           int lane, OT* rowmajor, int64_t ld, int64_t plane, OT* trans,
+                           int64_t ldt, int64_t tplane) {
+  float* lt = (float*)lds;
+  if (rowmajor) {
+    __syncthreads();
+    tile_stage<MR, MC, false>(lt, BC + 4, vals, g, lane);
+    __syncthreads();
+    tile_copy_out<OT, SPLIT, NT>(lt, BC + 4, BR, BC, rowmajor + (int64_t)g.row0 * ld + g.col0, ld,
+                                 plane);
+  }
+  if (trans) {
+    __syncthreads();
+    tile_stage<MR, MC, true>(lt, BR + 4, vals, g, lane);
+    __syncthreads();
+    tile_copy_out<OT, SPLIT, NT>(lt, BR + 4, BC, BR, trans + (int64_t)g.col0 * ldt + g.row0, ldt,
+                                 tplane);
+  }
+}
+
+// ---------------------------------------------------------------- epilogues
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiHidden& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& g, int lane, const Ctl*, int,
+                                          unsigned char* lds) {
+  using T = typename PP::T;
+  using ST = typename PP::ST;
+  const bool want_d = e.S != nullptr;
+  f32x16 sv[MR][MC];
+#pragma unroll
+  for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+    for (int mj = 0; mj < MC; ++mj) {
+      const int c = g.cb + mj * 32 + (lane & 31);
+      const bool cv = c < e.C;
+      const float bias = cv ? e.bias[c] : 0.f;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float z = acc[mi][mj][v] + bias;
+        const float y = act_f(e.act, z);
+        sv[mi][mj][v] = (cv && want_d) ? act_d(e.act, z, y) : 0.f;
+        acc[mi][mj][v] = cv ? y : 0.f;
+      }
+    }
+  tile_write<T, PP::SPLIT, BR, BC, NT>(lds, acc, g, lane, (T*)e.Y, e.ldo, e.py, (T*)e.YT, e.ldt,
+                                       e.pyt);
+  if (want_d)
+    tile_write<ST, false, BR, BC, NT>(lds, sv, g, lane, (ST*)e.S, e.ldo, 0, (ST*)nullptr, 0, 0);
+}
+
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& g, int lane, const Ctl*, int,
+                                          unsigned char* lds) {
   using T = typename PP::T;
   using ST = typename PP::ST;
   const ST* S = (const ST*)e.S;
-  T* Dl = (T*)e.Dl;
-  T* DlT = (T*)e.DlT;
-  EPI_LOOP_BEGIN
-    const bool cv = c < e.C;
-    float dv[4];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
-      dv[p] = cv ? acc[mi][mj][q * 4 + p] * tof<ST>(S[o]) : 0.f;
-      put<PP>(Dl + o, e.pd, dv[p]);
+  for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+    for (int mj = 0; mj < MC; ++mj) {
+      const int c = g.cb + mj * 32 + (lane & 31);
+      const bool cv = c < e.C;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r0 = g.rb + mi * 32 + q * 8 + (lane >> 5) * 4;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
+          acc[mi][mj][q * 4 + p] = cv ? acc[mi][mj][q * 4 + p] * tof<ST>(S[o]) : 0.f;
+        }
+      }
     }
-    put4<PP>(DlT + (int64_t)c * e.ldt + r0, e.pdt, dv);
-  EPI_LOOP_END
+  tile_write<T, PP::SPLIT, BR, BC, NT>(lds, acc, g, lane, (T*)e.Dl, e.ldo, e.pd, (T*)e.DlT,
+                                       e.ldt, e.pdt);
 }
 
-template <class PP, int MR, int MC>
-__device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl* ctl, int) {
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& geo, int lane, const Ctl* ctl,
+                                          int , unsigned char*) {
+  const int rb = geo.rb, cb = geo.cb;
   using T = typename PP::T;
   const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
   float* ent = e.tape + n * e.ent;
@@ -349,9 +462,11 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
   if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
 }
 
-template <class PP, int MR, int MC>
-__device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl* ctl, int) {
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& geo, int lane, const Ctl* ctl,
+                                          int , unsigned char*) {
+  const int rb = geo.rb, cb = geo.cb;
   using T = typename PP::T;
   const float dt = (float)ctl->dt;
   const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
@@ -455,9 +570,11 @@ __device__ __forceinline__ void fold_one(const EpiWgrad& e, float* gcur, float* 
   if (e.flags & WG_STORE_NEXT) gnext[idx] = g;
 }
 
-template <class PP, int MR, int MC>
-__device__ __forceinline__ void epi_apply(const EpiWgrad& e, f32x16 (&acc)[MR][MC], int rb,
-                                          int cb, int lane, const Ctl* ctl, int split) {
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiWgrad& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& geo, int lane, const Ctl* ctl,
+                                          int split, unsigned char*) {
+  const int rb = geo.rb, cb = geo.cb;
   const int gsel = ctl->gsel;
   float* gcur = gsel ? e.gf1 : e.gf0;
   float* gnext = gsel ? e.gf0 : e.gf1;
@@ -480,13 +597,16 @@ __device__ __forceinline__ void epi_apply(const EpiWgrad& e, f32x16 (&acc)[MR][M
 // ---------------------------------------------------------------- GEMM
 // C[r][c] = sum_k A[r][k] * B[c][k]; rows = batch (or input feature for
 // weight gradients), cols = output feature.  WRxWC waves per workgroup, each
-// wave a (BR/WR)x(BC/WC) block of 32x32 MFMA tiles; K is staged through LDS in
-// 128-byte rows (64 bf16 / 32 fp32), double-buffered, one barrier per K tile.
-// Split BF16 runs three passes over K: (A hi, B hi), (A lo, B hi), (A hi, B lo).
+// wave a (BR/WR)x(BC/WC) block of 32x32 MFMA tiles.  K is staged through LDS
+// in 128-byte rows (64 bf16 / 32 fp32 per row), double-buffered, one barrier
+// per K tile, with an S-deep register prefetch ring: while tile t is computed
+// from LDS, the loads of tiles t+1..t+S are in flight (the loop is unrolled by
+// S so every ring index is static).  Split BF16 loads the hi and lo planes of
+// both operands per K tile and issues hi*hi + lo*hi + hi*lo MFMAs.
 // Split-K (ksplit > 1): workgroup group s covers K tiles [s*nk, (s+1)*nk).
 // Tile order is XCD-aware: consecutive tiles of the row-major tile grid go to
 // the same XCD so that its L2 holds their operands.
-template <class PP, int BR, int BC, int WR, int WC, typename Epi>
+template <class PP, int BR, int BC, int WR, int WC, int S, typename Epi>
 __global__ void __launch_bounds__(WR * WC * 64)
     gemm_tn(GemmArgs g, Epi epi, const Ctl* __restrict__ ctl, int gate) {
   using T = typename PP::T;
@@ -496,10 +616,11 @@ __global__ void __launch_bounds__(WR * WC * 64)
   constexpr int BKB = 128;  // bytes of K per tile row
   constexpr int MR = BR / WR / 32, MC = BC / WC / 32;
   constexpr int RP = NT / 8;  // rows per load pass
+  constexpr int NPL = PP::SPLIT ? 2 : 1;        // operand planes per K tile
   constexpr int A_LOADS = BR / RP, B_LOADS = BC / RP;
-  constexpr int NPASS = PP::SPLIT ? 3 : 1;
+  constexpr int ROWS = (BR + BC) * NPL;          // LDS rows per buffer
   static_assert(BR % RP == 0 && BC % RP == 0, "tile/thread mismatch");
-  __shared__ __attribute__((aligned(16))) unsigned char lds[2 * (BR + BC) * PITCH];
+  __shared__ __attribute__((aligned(16))) unsigned char lds[2 * ROWS * PITCH];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave / WC, wc = wave % WC;
@@ -512,11 +633,11 @@ __global__ void __launch_bounds__(WR * WC * 64)
 
   const int64_t ldab = g.lda * (int64_t)sizeof(T), ldbb = g.ldb * (int64_t)sizeof(T);
   const int64_t pab = g.pa * (int64_t)sizeof(T), pbb = g.pb * (int64_t)sizeof(T);
-  const unsigned char* Ag = (const unsigned char*)g.A + (int64_t)row0 * ldab;
-  const unsigned char* Bg = (const unsigned char*)g.B + (int64_t)col0 * ldbb;
   const int lr = tid >> 3, lc = tid & 7;
+  const unsigned char* Ag = (const unsigned char*)g.A + (int64_t)(row0 + lr) * ldab + lc * 16;
+  const unsigned char* Bg = (const unsigned char*)g.B + (int64_t)(col0 + lr) * ldbb + lc * 16;
 
-  u32x4 ra[A_LOADS], rbv[B_LOADS];
+  u32x4 ra[S][NPL][A_LOADS], rbv[S][NPL][B_LOADS];
   f32x16 acc[MR][MC];
 #pragma unroll
   for (int i = 0; i < MR; ++i)
@@ -527,59 +648,83 @@ __global__ void __launch_bounds__(WR * WC * 64)
 
   const int nk = (int)((int64_t)g.K * sizeof(T) / BKB) / g.ksplit;
   const int k0 = split * nk;
-  const int ntot = NPASS * nk;
-  // global -> registers (tile t of the pass-major K sequence)
-#define GEMM_GLOAD(t)                                                              \
+  // global -> register ring slot (static slot index)
+#define GEMM_GLOAD(t, slot)                                                        \
   do {                                                                             \
-    const int pass_ = (t) / nk, kt_ = k0 + (t) - pass_ * nk;                       \
-    const int64_t kb_ = (int64_t)kt_ * BKB + lc * 16;                              \
-    const unsigned char* Ap_ = Ag + (pass_ == 1 ? pab : 0) + kb_;                  \
-    const unsigned char* Bp_ = Bg + (pass_ == 2 ? pbb : 0) + kb_;                  \
-    _Pragma("unroll") for (int i_ = 0; i_ < A_LOADS; ++i_)                         \
-      ra[i_] = *(const u32x4*)(Ap_ + (int64_t)(lr + i_ * RP) * ldab);              \
-    _Pragma("unroll") for (int i_ = 0; i_ < B_LOADS; ++i_)                         \
-      rbv[i_] = *(const u32x4*)(Bp_ + (int64_t)(lr + i_ * RP) * ldbb);             \
+    const int64_t kb_ = (int64_t)(k0 + (t)) * BKB;                                 \
+    _Pragma("unroll") for (int p_ = 0; p_ < NPL; ++p_) {                           \
+      _Pragma("unroll") for (int i_ = 0; i_ < A_LOADS; ++i_)                       \
+        ra[slot][p_][i_] = *(const u32x4*)(Ag + p_ * pab + (int64_t)(i_ * RP) * ldab + kb_); \
+      _Pragma("unroll") for (int i_ = 0; i_ < B_LOADS; ++i_)                       \
+        rbv[slot][p_][i_] = *(const u32x4*)(Bg + p_ * pbb + (int64_t)(i_ * RP) * ldbb + kb_); \
+    }                                                                              \
   } while (0)
-  // registers -> LDS buffer
-#define GEMM_SSTORE(buf)                                                           \
+  // register ring slot -> LDS buffer; rows: [A plane 0 | A plane 1 | B plane 0 | B plane 1]
+#define GEMM_SSTORE(buf, slot)                                                     \
   do {                                                                             \
-    unsigned char* base_ = lds + (buf) * (BR + BC) * PITCH;                        \
-    _Pragma("unroll") for (int i_ = 0; i_ < A_LOADS; ++i_)                         \
-      *(u32x4*)(base_ + (lr + i_ * RP) * PITCH + lc * 16) = ra[i_];               \
-    _Pragma("unroll") for (int i_ = 0; i_ < B_LOADS; ++i_)                         \
-      *(u32x4*)(base_ + (BR + lr + i_ * RP) * PITCH + lc * 16) = rbv[i_];         \
+    unsigned char* base_ = lds + (buf) * ROWS * PITCH + lr * PITCH + lc * 16;      \
+    _Pragma("unroll") for (int p_ = 0; p_ < NPL; ++p_) {                           \
+      _Pragma("unroll") for (int i_ = 0; i_ < A_LOADS; ++i_)                       \
+        *(u32x4*)(base_ + (p_ * BR + i_ * RP) * PITCH) = ra[slot][p_][i_];         \
+      _Pragma("unroll") for (int i_ = 0; i_ < B_LOADS; ++i_)                       \
+        *(u32x4*)(base_ + (NPL * BR + p_ * BC + i_ * RP) * PITCH) = rbv[slot][p_][i_]; \
+    }                                                                              \
   } while (0)
 
-  GEMM_GLOAD(0);
-  GEMM_SSTORE(0);
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+    if (j < nk) GEMM_GLOAD(j, j);
+  GEMM_SSTORE(0, 0);
   __syncthreads();
   const int arow = wr * (BR / WR) + (lane & 31);
-  const int brow = BR + wc * (BC / WC) + (lane & 31);
+  const int brow = NPL * BR + wc * (BC / WC) + (lane & 31);
   const int koff = (lane >> 5) * 16;
-  for (int t = 0; t < ntot; ++t) {
-    const int cur = t & 1;
-    if (t + 1 < ntot) GEMM_GLOAD(t + 1);
-    const unsigned char* base = lds + cur * (BR + BC) * PITCH;
+  for (int t0 = 0; t0 < nk; t0 += S) {
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      u32x4 fa[MR], fb[MC];
+    for (int j = 0; j < S; ++j) {
+      const int t = t0 + j;
+      if (t < nk) {
+        // slot j held tile t, already in LDS: refill it with tile t+S
+        if (t + S < nk) GEMM_GLOAD(t + S, j);
+        const unsigned char* base = lds + (t & 1) * ROWS * PITCH;
 #pragma unroll
-      for (int mi = 0; mi < MR; ++mi)
-        fa[mi] = *(const u32x4*)(base + (arow + mi * 32) * PITCH + kk * 32 + koff);
+        for (int kk = 0; kk < 4; ++kk) {
+          u32x4 fa[NPL][MR], fb[NPL][MC];
 #pragma unroll
-      for (int mj = 0; mj < MC; ++mj)
-        fb[mj] = *(const u32x4*)(base + (brow + mj * 32) * PITCH + kk * 32 + koff);
+          for (int p = 0; p < NPL; ++p) {
 #pragma unroll
-      for (int mi = 0; mi < MR; ++mi)
+            for (int mi = 0; mi < MR; ++mi)
+              fa[p][mi] = *(const u32x4*)(base + (arow + p * BR + mi * 32) * PITCH + kk * 32 + koff);
 #pragma unroll
-        for (int mj = 0; mj < MC; ++mj) Mma<T>::run(acc[mi][mj], fa[mi], fb[mj]);
+            for (int mj = 0; mj < MC; ++mj)
+              fb[p][mj] = *(const u32x4*)(base + (brow + p * BC + mj * 32) * PITCH + kk * 32 + koff);
+          }
+#pragma unroll
+          for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+            for (int mj = 0; mj < MC; ++mj) {
+              Mma<T>::run(acc[mi][mj], fa[0][mi], fb[0][mj]);
+              if constexpr (PP::SPLIT) {
+                Mma<T>::run(acc[mi][mj], fa[NPL - 1][mi], fb[0][mj]);
+                Mma<T>::run(acc[mi][mj], fa[0][mi], fb[NPL - 1][mj]);
+              }
+            }
+        }
+        if (t + 1 < nk) GEMM_SSTORE((t + 1) & 1, (j + 1) % S);
+        __syncthreads();
+      }
     }
-    if (t + 1 < ntot) GEMM_SSTORE(cur ^ 1);
-    __syncthreads();
   }
 #undef GEMM_GLOAD
 #undef GEMM_SSTORE
-  epi_apply<PP, MR, MC>(epi, acc, row0 + wr * (BR / WR), col0 + wc * (BC / WC), lane, ctl, split);
+  TileGeo geo;
+  geo.row0 = row0;
+  geo.col0 = col0;
+  geo.lr0 = wr * (BR / WR);
+  geo.lc0 = wc * (BC / WC);
+  geo.rb = row0 + geo.lr0;
+  geo.cb = col0 + geo.lc0;
+  epi_apply<PP, BR, BC, NT>(epi, acc, geo, lane, ctl, split, lds);
 }
 
 // tile configurations: {BR, BC, WR, WC}
@@ -587,6 +732,12 @@ static const int kCfg[4][4] = {{128, 128, 2, 2}, {128, 64, 2, 2}, {64, 64, 2, 2}
 int gemm_cfg_count() { return 4; }
 int gemm_cfg_tile_rows(int cfg) { return kCfg[cfg][0]; }
 int gemm_cfg_tile_cols(int cfg) { return kCfg[cfg][1]; }
+
+// register prefetch depth per tile: deeper for small tiles (less MFMA work per
+// K tile to cover the load latency), shallower where registers are scarce
+template <class PP, int BR, int BC> struct Depth {
+  static constexpr int value = (BR * BC >= 128 * 128) ? (PP::SPLIT ? 2 : 3) : (PP::SPLIT ? 3 : 4);
+};
 
 template <class PP, typename Epi>
 static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, int gate,
@@ -601,11 +752,11 @@ static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, i
   switch (cfg) {
     case 0:
       if constexpr (!kStage)
-        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 2, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate);
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 2, Depth<PP, 128, 128>::value, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate);
       break;
-    case 1: hipLaunchKernelGGL((gemm_tn<PP, 128, 64, 2, 2, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
-    case 2: hipLaunchKernelGGL((gemm_tn<PP, 64, 64, 2, 2, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
-    default: hipLaunchKernelGGL((gemm_tn<PP, 32, 64, 1, 2, Epi>), dim3(nb), dim3(128), 0, s, ga, e, ctl, gate); break;
+    case 1: hipLaunchKernelGGL((gemm_tn<PP, 128, 64, 2, 2, Depth<PP, 128, 64>::value, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
+    case 2: hipLaunchKernelGGL((gemm_tn<PP, 64, 64, 2, 2, Depth<PP, 64, 64>::value, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
+    default: hipLaunchKernelGGL((gemm_tn<PP, 32, 64, 1, 2, Depth<PP, 32, 64>::value, Epi>), dim3(nb), dim3(128), 0, s, ga, e, ctl, gate); break;
   }
 }
 void launch_gemm_hidden(int prec, int cfg, const GemmArgs& g, const EpiHidden& e,

@@ -42,6 +42,8 @@ struct Layer {
   // weight-gradient GEMM plan (rows = inp, cols = outp, K = batch)
   int wg_cfg = 0, wg_rt = 0, wg_ct = 0, wg_ks = 1;
   float* wpart = nullptr; // split-K partials [wg_ks][in*out] when wg_ks > 1
+  int64_t wpart_cap = 0;  // floats allocated for wpart
+  int wg_ks0 = 1;         // heuristic split-K (autotuning off)
 };
 
 struct GemmPlan {
@@ -100,6 +102,19 @@ struct node_handle {
   bool everystep = false;
 
   std::vector<void*> allocs;
+
+  // autotuned tile configuration per (role, layer); -1 = heuristic
+  int tcfg[6][MAXL];
+  bool tuned = false, autotune = true;
+  int tune_role = -1, tune_layer = -1, tune_cfg = -1, tune_ks = 1;  // override while timing
+  float* tune_wpart = nullptr;
+
+  // captured step-attempt graphs (re-captured when a captured pointer changes)
+  bool use_graphs = true;
+  hipGraphExec_t fwd_graph = nullptr, adj_graph = nullptr;
+  const void* fwd_graph_sig[3] = {nullptr, nullptr, nullptr};  // tape, save times, outputs
+  const void* adj_graph_sig[3] = {nullptr, nullptr, nullptr};  // tape, save times, cotangents
+  int graph_attempts = 4;
 
   // profiling (node_prof_enable)
   uint64_t prof_mask = 0;
@@ -220,133 +235,121 @@ int validate_desc(const node_desc* d, std::string& msg) {
 }
 
 // ---------------------------------------------------------------- RHS / VJP
+// Tile plan of one GEMM role: the autotuned choice when there is one, else the
+// occupancy heuristic.  min_cfg 1 for the stage epilogues (<= 64 columns).
+GemmPlan plan_role(node_handle* h, int role, int layer, int rows, int cols, int min_cfg) {
+  if (h->tune_role == role && h->tune_layer == layer && h->tune_cfg >= 0) {
+    const int c = h->tune_cfg;
+    return {c, rows / gemm_cfg_tile_rows(c), cols / gemm_cfg_tile_cols(c)};
+  }
+  const int c = h->tcfg[role][layer];
+  if (c >= 0) return {c, rows / gemm_cfg_tile_rows(c), cols / gemm_cfg_tile_cols(c)};
+  return plan(rows, cols, min_cfg);
+}
+
+// hidden layer l: forward (Y) or adjoint recompute (Y, Y^T, act')
+void launch_hidden(node_handle* h, int l, int gate, bool adjoint) {
+  const Layer& ly = h->layers[l];
+  GemmArgs g{};
+  g.A = l == 0 ? h->X0 : h->H[l - 1];
+  g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
+  g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
+  const int role = adjoint ? PR_ADJ_HIDDEN : PR_FWD_HIDDEN;
+  GemmPlan p = plan_role(h, role, l, h->Bp, ly.outp, 0);
+  g.row_tiles = p.rt; g.col_tiles = p.ct;
+  const int64_t ph = (int64_t)h->Bp * ly.outp;
+  EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], adjoint ? h->HT[l] : nullptr,
+              adjoint ? h->S[l] : nullptr, ph, ph};
+  ProfScope ps(h, role, l);
+  launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+}
+
+// last layer with the Tsit5 stage epilogue
+void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* part) {
+  const int l = h->L - 1;
+  const Layer& ly = h->layers[l];
+  GemmArgs g{};
+  g.A = l == 0 ? h->X0 : h->H[l - 1];
+  g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
+  g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
+  GemmPlan p = plan_role(h, PR_FWD_STAGE, l, h->Bp, ly.outp, 1);
+  g.row_tiles = p.rt; g.col_tiles = p.ct;
+  EpiFwdStage e{};
+  e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
+  e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D; e.sv = (int64_t)h->Bp * h->D;
+  e.X0 = h->X0; e.ldx = h->Dp; e.map = h->desc.input_map;
+  e.scratch = h->scratch; e.part = part;
+  e.px = (int64_t)h->Bp * h->Dp;
+  ProfScope ps(h, PR_FWD_STAGE, l);
+  launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+}
+
 // One batched right-hand side: X0 -> layers -> last-layer epilogue.
 void rhs_eval(node_handle* h, int stage, int mode, int gate, double* part) {
-  const int L = h->L;
-  for (int l = 0; l < L; ++l) {
-    const Layer& ly = h->layers[l];
-    GemmArgs g{};
-    g.A = l == 0 ? h->X0 : h->H[l - 1];
-    g.lda = ly.inp;
-    g.B = ly.Wr;
-    g.ldb = ly.inp;
-    g.K = ly.inp;
-    g.pa = (int64_t)h->Bp * ly.inp;
-    g.pb = (int64_t)ly.outp * ly.inp;
-    g.ksplit = 1;
-    if (l < L - 1) {
-      GemmPlan p = plan(h->Bp, ly.outp);
-      g.row_tiles = p.rt; g.col_tiles = p.ct;
-      EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], nullptr, nullptr,
-                  (int64_t)h->Bp * ly.outp, 0};
-      ProfScope ps(h, PR_FWD_HIDDEN, l);
-      launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
-    } else {
-      GemmPlan p = plan(h->Bp, ly.outp, 1);
-      g.row_tiles = p.rt; g.col_tiles = p.ct;
-      EpiFwdStage e{};
-      e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
-      e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D; e.sv = (int64_t)h->Bp * h->D;
-      e.X0 = h->X0; e.ldx = h->Dp; e.map = h->desc.input_map;
-      e.scratch = h->scratch; e.part = part;
-      e.px = (int64_t)h->Bp * h->Dp;
-      ProfScope ps(h, PR_FWD_STAGE, l);
-      launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
-    }
-  }
+  for (int l = 0; l < h->L - 1; ++l) launch_hidden(h, l, gate, false);
+  launch_fwd_stage(h, stage, mode, gate, part);
 }
 
 int rhs_last_tiles(node_handle* h) {
-  GemmPlan p = plan(h->Bp, h->layers[h->L - 1].outp, 1);
+  const Layer& ly = h->layers[h->L - 1];
+  GemmPlan p = plan_role(h, PR_FWD_STAGE, h->L - 1, h->Bp, ly.outp, 1);
   return p.rt * p.ct;
 }
 
-// One adjoint stage: forward recompute at the interpolated u, data and weight
-// gradients, bias gradients, and the layer-1 epilogue (k^lambda + next stage).
-void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float cb, float ce,
-              float cb1, float ce1, double* part, float* plain_out, float* gptr) {
+// data gradient of layer l >= 1: delta_{l-1} = (delta_l W_l) * act'
+void launch_bwd_data(node_handle* h, int l, int gate) {
+  const Layer& ly = h->layers[l];
+  GemmArgs g{};
+  g.A = l == h->L - 1 ? h->Lm : h->Dl[l];
+  g.lda = ly.outp; g.B = ly.Wc; g.ldb = ly.outp; g.K = ly.outp;
+  g.pa = (int64_t)h->Bp * ly.outp; g.pb = (int64_t)ly.inp * ly.outp; g.ksplit = 1;
+  GemmPlan p = plan_role(h, PR_BWD_DATA, l, h->Bp, ly.inp, 0);
+  g.row_tiles = p.rt; g.col_tiles = p.ct;
+  const int64_t pd = (int64_t)h->Bp * ly.inp;
+  EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd};
+  ProfScope ps(h, PR_BWD_DATA, l);
+  launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+}
+
+// weight gradient of layer l (K = batch), folded into the mu accumulators or
+// written as split-K partials
+void launch_wgrad(node_handle* h, int l, int gate, const EpiWgrad& wt) {
+  const Layer& ly = h->layers[l];
+  GemmArgs g{};
+  g.A = l == 0 ? h->X0T : h->HT[l - 1];
+  g.lda = h->Bp;
+  g.B = l == h->L - 1 ? h->LmT : h->DlT[l];
+  g.ldb = h->Bp;
+  g.K = h->Bp;
+  g.pa = (int64_t)ly.inp * h->Bp;
+  g.pb = (int64_t)ly.outp * h->Bp;
+  int cfg = ly.wg_cfg, ks = ly.wg_ks;
+  float* wpart = ly.wpart;
+  if (h->tune_role == PR_WGRAD && h->tune_layer == l && h->tune_cfg >= 0) {
+    cfg = h->tune_cfg;
+    ks = h->tune_ks;
+    wpart = h->tune_wpart;
+  }
+  g.row_tiles = ly.inp / gemm_cfg_tile_rows(cfg);
+  g.col_tiles = ly.outp / gemm_cfg_tile_cols(cfg);
+  g.ksplit = ks;
+  EpiWgrad e = wt;
+  e.in = ly.in; e.out = ly.out; e.w0 = ly.w0;
+  e.wpart = ks > 1 ? wpart : nullptr;
+  ProfScope ps(h, PR_WGRAD, l);
+  launch_gemm_wgrad(h->prec, cfg, g, e, h->ctl, gate, h->stream);
+}
+
+// layer-1 data gradient with the adjoint stage epilogue
+void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part,
+                     float* plain_out) {
   const int L = h->L;
-  // forward recompute of the hidden layers (activations, transposes, act')
-  for (int l = 0; l < L - 1; ++l) {
-    const Layer& ly = h->layers[l];
-    GemmArgs g{};
-    g.A = l == 0 ? h->X0 : h->H[l - 1];
-    g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
-    g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
-    GemmPlan p = plan(h->Bp, ly.outp);
-    g.row_tiles = p.rt; g.col_tiles = p.ct;
-    const int64_t ph = (int64_t)h->Bp * ly.outp;
-    EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], h->HT[l], h->S[l], ph, ph};
-    ProfScope ps(h, PR_ADJ_HIDDEN, l);
-    launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
-  }
-  EpiWgrad wt{};
-  wt.accb = h->accb; wt.acce = h->acce; wt.gf0 = h->gf0; wt.gf1 = h->gf1; wt.gptr = gptr;
-  wt.cb = cb; wt.ce = ce; wt.cb1 = cb1; wt.ce1 = ce1; wt.flags = wflags;
-  auto wgrad = [&](int l) {
-    const Layer& ly = h->layers[l];
-    GemmArgs g{};
-    g.A = l == 0 ? h->X0T : h->HT[l - 1];
-    g.lda = h->Bp;
-    g.B = l == L - 1 ? h->LmT : h->DlT[l];
-    g.ldb = h->Bp;
-    g.K = h->Bp;
-    g.pa = (int64_t)ly.inp * h->Bp;
-    g.pb = (int64_t)ly.outp * h->Bp;
-    g.row_tiles = ly.wg_rt; g.col_tiles = ly.wg_ct; g.ksplit = ly.wg_ks;
-    EpiWgrad e = wt;
-    e.in = ly.in; e.out = ly.out; e.w0 = ly.w0;
-    e.wpart = ly.wg_ks > 1 ? ly.wpart : nullptr;
-    ProfScope ps(h, PR_WGRAD, l);
-    launch_gemm_wgrad(h->prec, ly.wg_cfg, g, e, h->ctl, gate, h->stream);
-  };
-  // data gradients of layers L..2 (each followed by that layer's weight gradient)
-  for (int l = L - 1; l >= 1; --l) {
-    const Layer& ly = h->layers[l];
-    GemmArgs g{};
-    g.A = l == L - 1 ? h->Lm : h->Dl[l];
-    g.lda = ly.outp; g.B = ly.Wc; g.ldb = ly.outp; g.K = ly.outp;
-    g.pa = (int64_t)h->Bp * ly.outp; g.pb = (int64_t)ly.inp * ly.outp; g.ksplit = 1;
-    GemmPlan p = plan(h->Bp, ly.inp);
-    g.row_tiles = p.rt; g.col_tiles = p.ct;
-    const int64_t pd = (int64_t)h->Bp * ly.inp;
-    EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd};
-    {
-      ProfScope ps(h, PR_BWD_DATA, l);
-      launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
-    }
-    wgrad(l);
-  }
-  wgrad(0);
-  {
-    FoldArgs fa{};
-    fa.nl = L;
-    for (int l = 0; l < L; ++l) {
-      const Layer& ly = h->layers[l];
-      fa.dT[l] = l == L - 1 ? h->LmT : h->DlT[l];
-      fa.dplane[l] = (int64_t)ly.outp * h->Bp;
-      fa.outs[l] = ly.out;
-      fa.b0s[l] = ly.b0;
-      fa.bias_rows += ly.out;
-      if (ly.wg_ks > 1) {
-        fa.sk_w0[fa.nsk] = ly.w0;
-        fa.sk_n[fa.nsk] = (int64_t)ly.in * ly.out;
-        fa.sk_part[fa.nsk] = ly.wpart;
-        fa.sk_split[fa.nsk] = ly.wg_ks;
-        fa.nsk++;
-      }
-    }
-    fa.ldt = h->Bp;
-    fa.Bp = h->Bp;
-    launch_fold(h->prec, fa, wt, h->ctl, gate, h->stream);
-  }
-  // layer-1 data gradient with the stage epilogue
   const Layer& l0 = h->layers[0];
   GemmArgs g{};
   g.A = L == 1 ? h->Lm : h->Dl[0];
   g.lda = l0.outp; g.B = l0.Wc; g.ldb = l0.outp; g.K = l0.outp;
   g.pa = (int64_t)h->Bp * l0.outp; g.pb = (int64_t)l0.inp * l0.outp; g.ksplit = 1;
-  GemmPlan p = plan(h->Bp, h->Dp, 1);
+  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 1);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   EpiBwdLast e{};
   e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
@@ -362,8 +365,49 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
 }
 
 int vjp_last_tiles(node_handle* h) {
-  GemmPlan p = plan(h->Bp, h->Dp, 1);
+  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 1);
   return p.rt * p.ct;
+}
+
+void launch_fold_all(node_handle* h, int gate, const EpiWgrad& wt) {
+  FoldArgs fa{};
+  fa.nl = h->L;
+  for (int l = 0; l < h->L; ++l) {
+    const Layer& ly = h->layers[l];
+    fa.dT[l] = l == h->L - 1 ? h->LmT : h->DlT[l];
+    fa.dplane[l] = (int64_t)ly.outp * h->Bp;
+    fa.outs[l] = ly.out;
+    fa.b0s[l] = ly.b0;
+    fa.bias_rows += ly.out;
+    if (ly.wg_ks > 1) {
+      fa.sk_w0[fa.nsk] = ly.w0;
+      fa.sk_n[fa.nsk] = (int64_t)ly.in * ly.out;
+      fa.sk_part[fa.nsk] = ly.wpart;
+      fa.sk_split[fa.nsk] = ly.wg_ks;
+      fa.nsk++;
+    }
+  }
+  fa.ldt = h->Bp;
+  fa.Bp = h->Bp;
+  launch_fold(h->prec, fa, wt, h->ctl, gate, h->stream);
+}
+
+// One adjoint stage: forward recompute at the interpolated u, data and weight
+// gradients, bias gradients, and the layer-1 epilogue (k^lambda + next stage).
+void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float cb, float ce,
+              float cb1, float ce1, double* part, float* plain_out, float* gptr) {
+  const int L = h->L;
+  for (int l = 0; l < L - 1; ++l) launch_hidden(h, l, gate, true);
+  EpiWgrad wt{};
+  wt.accb = h->accb; wt.acce = h->acce; wt.gf0 = h->gf0; wt.gf1 = h->gf1; wt.gptr = gptr;
+  wt.cb = cb; wt.ce = ce; wt.cb1 = cb1; wt.ce1 = ce1; wt.flags = wflags;
+  for (int l = L - 1; l >= 1; --l) {
+    launch_bwd_data(h, l, gate);
+    launch_wgrad(h, l, gate, wt);
+  }
+  launch_wgrad(h, 0, gate, wt);
+  launch_fold_all(h, gate, wt);
+  launch_bwd_last(h, stage, mode, gate, part, plain_out);
 }
 
 int ensure_tape(node_handle* h, int64_t entries) {
@@ -387,6 +431,117 @@ int ensure_tape(node_handle* h, int64_t entries) {
   return 0;
 }
 
+int ensure_wpart(node_handle* h) {
+  for (Layer& ly : h->layers) {
+    const int64_t need = ly.wg_ks > 1 ? (int64_t)ly.wg_ks * ly.in * ly.out : 0;
+    if (need > ly.wpart_cap) {
+      dfree(h, ly.wpart);
+      ly.wpart = nullptr;
+      if (int rc = dalloc_t(h, &ly.wpart, (size_t)need * sizeof(float))) return rc;
+      ly.wpart_cap = need;
+    }
+  }
+  return 0;
+}
+
+int ensure_adjoint(node_handle* h);
+
+// Average duration (ms) of `reps` back-to-back launches after one warm-up.
+template <typename F>
+float time_launches(node_handle* h, F&& f, int reps) {
+  hipEvent_t a = prof_event(h), b = prof_event(h);
+  f();
+  (void)hipEventRecord(a, h->stream);
+  for (int r = 0; r < reps; ++r) f();
+  (void)hipEventRecord(b, h->stream);
+  (void)hipEventSynchronize(b);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  h->ev_pool.push_back(a);
+  h->ev_pool.push_back(b);
+  return ms / reps;
+}
+
+// Autotuner: times every valid tile configuration (and split-K factor for the
+// weight gradients) of every GEMM role on this handle's own buffers and shapes,
+// and keeps the fastest.  Runs once per handle, before its first solve.
+int autotune_impl(node_handle* h) {
+  if (int rc = ensure_adjoint(h)) return rc;
+  const int L = h->L;
+  const int reps = 5;
+  int64_t maxio = 0;
+  for (const Layer& ly : h->layers) maxio = std::max<int64_t>(maxio, (int64_t)ly.in * ly.out);
+  float* tmp = nullptr;
+  if (int rc = dalloc_t(h, &tmp, (size_t)8 * maxio * sizeof(float))) return rc;
+  const uint64_t saved_mask = h->prof_mask;
+  h->prof_mask = 0;
+  auto tune_slot = [&](int role, int layer, int rows, int cols, int min_cfg, auto&& launch) {
+    float best = 1e30f;
+    int bc = -1;
+    for (int c = min_cfg; c < gemm_cfg_count(); ++c) {
+      if (rows % gemm_cfg_tile_rows(c) || cols % gemm_cfg_tile_cols(c)) continue;
+      h->tune_role = role; h->tune_layer = layer; h->tune_cfg = c; h->tune_ks = 1;
+      const float t = time_launches(h, launch, reps);
+      if (t < best) { best = t; bc = c; }
+    }
+    h->tune_role = h->tune_layer = h->tune_cfg = -1;
+    h->tcfg[role][layer] = bc;
+  };
+  for (int l = 0; l < L - 1; ++l) {
+    const Layer& ly = h->layers[l];
+    tune_slot(PR_FWD_HIDDEN, l, h->Bp, ly.outp, 0, [&] { launch_hidden(h, l, GATE_ALWAYS, false); });
+    tune_slot(PR_ADJ_HIDDEN, l, h->Bp, ly.outp, 0, [&] { launch_hidden(h, l, GATE_ALWAYS, true); });
+  }
+  tune_slot(PR_FWD_STAGE, L - 1, h->Bp, h->layers[L - 1].outp, 1,
+            [&] { launch_fwd_stage(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr); });
+  for (int l = 1; l < L; ++l)
+    tune_slot(PR_BWD_DATA, l, h->Bp, h->layers[l].inp, 0, [&] { launch_bwd_data(h, l, GATE_ALWAYS); });
+  tune_slot(PR_BWD_LAST, 0, h->Bp, h->Dp, 1,
+            [&] { launch_bwd_last(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr, h->scratch); });
+  // weight gradients: tile x split-K; a split pays for its partial sums in the fold
+  EpiWgrad wt{};
+  wt.accb = h->accb; wt.acce = h->acce; wt.gf0 = h->gf0; wt.gf1 = h->gf1; wt.gptr = h->accb;
+  wt.flags = WG_STORE_PTR;
+  const int ktiles = (int)((int64_t)h->Bp * h->esz / 128);
+  for (int l = 0; l < L; ++l) {
+    Layer& ly = h->layers[l];
+    float best = 1e30f;
+    int bc = ly.wg_cfg, bks = ly.wg_ks;
+    for (int c = 0; c < gemm_cfg_count(); ++c) {
+      if (ly.inp % gemm_cfg_tile_rows(c) || ly.outp % gemm_cfg_tile_cols(c)) continue;
+      const int tiles = (ly.inp / gemm_cfg_tile_rows(c)) * (ly.outp / gemm_cfg_tile_cols(c));
+      for (int ks = 1; ks <= 8; ks *= 2) {
+        if (ktiles % ks || tiles * ks > 4096) continue;
+        h->tune_role = PR_WGRAD; h->tune_layer = l; h->tune_cfg = c; h->tune_ks = ks;
+        h->tune_wpart = tmp;
+        float t = time_launches(h, [&] { launch_wgrad(h, l, GATE_ALWAYS, wt); }, reps);
+        if (ks > 1) t += (float)((double)ks * ly.in * ly.out * 4.0 / 3.0e12 * 1e3);  // fold reads
+        if (t < best) { best = t; bc = c; bks = ks; }
+      }
+    }
+    ly.wg_cfg = bc;
+    ly.wg_ks = bks;
+    ly.wg_rt = ly.inp / gemm_cfg_tile_rows(bc);
+    ly.wg_ct = ly.outp / gemm_cfg_tile_cols(bc);
+  }
+  h->tune_role = h->tune_layer = h->tune_cfg = -1;
+  h->tune_wpart = nullptr;
+  h->prof_mask = saved_mask;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  dfree(h, tmp);
+  if (int rc = ensure_wpart(h)) return rc;
+  h->tuned = true;
+  if (h->fwd_graph) { (void)hipGraphExecDestroy(h->fwd_graph); h->fwd_graph = nullptr; }
+  if (h->adj_graph) { (void)hipGraphExecDestroy(h->adj_graph); h->adj_graph = nullptr; }
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int maybe_autotune(node_handle* h) {
+  if (!h->autotune || h->tuned || !h->have_params) return 0;
+  return autotune_impl(h);
+}
+
 int ensure_adjoint(node_handle* h) {
   if (h->mu) return 0;
   const int64_t sv = (int64_t)h->Bp * h->D;
@@ -401,10 +556,7 @@ int ensure_adjoint(node_handle* h) {
   if (int rc = dalloc_t(h, &h->acce, h->P * 4)) return rc;
   if (int rc = dalloc_t(h, &h->gf0, h->P * 4)) return rc;
   if (int rc = dalloc_t(h, &h->gf1, h->P * 4)) return rc;
-  for (Layer& ly : h->layers)
-    if (ly.wg_ks > 1 && !ly.wpart)
-      if (int rc = dalloc_t(h, &ly.wpart, (size_t)ly.wg_ks * ly.in * ly.out * sizeof(float)))
-        return rc;
+  if (int rc = ensure_wpart(h)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -479,6 +631,7 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
                  const double* saveat, int64_t n_saveat, node_stats* st) {
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
   if (!o) return fail(h, NODE_ERR_INVALID, "opts is NULL");
+  if (int rc = maybe_autotune(h)) return rc;
   if (!(o->t1 != o->t0) || !std::isfinite(o->t0) || !std::isfinite(o->t1))
     return fail(h, NODE_ERR_INVALID, "tspan must be finite with t1 != t0");
   if (!o->adaptive && !(o->dt != 0.0)) return fail(h, NODE_ERR_INVALID, "fixed-step solve needs dt");
@@ -523,7 +676,14 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
   launch_initdt_b(h->ctl, px, nx, nullptr, 0, nel, h->save_times, 0, h->fwd_times, h->stream);
   HIPCHK(h, hipGetLastError());
 
-  int64_t chunk = 2, enqueued = 0;
+  auto attempt = [&]() {
+    launch_fwd_step_begin(sg, h->tape, h->X0, xpl, h->save_times, h->out_dev, h->ctl, h->stream);
+    for (int s = 1; s < 7; ++s) rhs_eval(h, s, SM_STEP, GATE_ACTIVE, px);
+    launch_fwd_controller(h->ctl, px, nx, h->save_times, h->fwd_times, nel, h->stream);
+  };
+  const bool graphs = h->use_graphs && h->prof_mask == 0;
+  const int ga = h->graph_attempts;
+  int64_t chunk = graphs ? ga : 2, enqueued = 0;
   for (;;) {
     if (int rc = poll_ctl(h)) return rc;
     if (h->hctl->done) break;
@@ -531,10 +691,24 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
       return fail(h, NODE_ERR_STATE, "step controller did not terminate (internal error)");
     enqueued += chunk;
     if (int rc = ensure_tape(h, h->hctl->n + chunk + 2)) return rc;
-    for (int64_t k = 0; k < chunk; ++k) {
-      launch_fwd_step_begin(sg, h->tape, h->X0, xpl, h->save_times, h->out_dev, h->ctl, h->stream);
-      for (int s = 1; s < 7; ++s) rhs_eval(h, s, SM_STEP, GATE_ACTIVE, px);
-      launch_fwd_controller(h->ctl, px, nx, h->save_times, h->fwd_times, nel, h->stream);
+    if (graphs) {
+      const void* sig[3] = {h->tape, h->save_times, h->out_dev};
+      if (h->fwd_graph && std::memcmp(sig, h->fwd_graph_sig, sizeof(sig)) != 0) {
+        (void)hipGraphExecDestroy(h->fwd_graph);
+        h->fwd_graph = nullptr;
+      }
+      if (!h->fwd_graph) {
+        hipGraph_t gr = nullptr;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        for (int k = 0; k < ga; ++k) attempt();
+        HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
+        HIPCHK(h, hipGraphInstantiate(&h->fwd_graph, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        std::memcpy(h->fwd_graph_sig, sig, sizeof(sig));
+      }
+      for (int64_t k = 0; k < chunk; k += ga) HIPCHK(h, hipGraphLaunch(h->fwd_graph, h->stream));
+    } else {
+      for (int64_t k = 0; k < chunk; ++k) attempt();
     }
     HIPCHK(h, hipGetLastError());
     chunk = std::min<int64_t>(chunk * 2, 16);
@@ -580,6 +754,7 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   if (!(h->fwd_opts.t1 > h->fwd_opts.t0))
     return fail(h, NODE_ERR_UNSUPPORTED, "adjoint needs a forward solve with t1 > t0");
   if (int rc = ensure_adjoint(h)) return rc;
+  if (int rc = maybe_autotune(h)) return rc;
   const node_solve_opts& o = h->fwd_opts;
   const StateGeom sg = geom(h);
   const int64_t ncb = (int64_t)h->h_save.size();
@@ -616,26 +791,47 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   launch_initdt_b(h->ctl, px, nx, pm, nmb, nel, h->save_times, 1, h->fwd_times, h->stream);
   HIPCHK(h, hipGetLastError());
 
-  int64_t chunk = 2, enqueued = 0;
+  auto attempt = [&]() {
+    launch_adj_step_begin(sg, ab, h->ctl, GATE_ALWAYS, h->stream);
+    vjp_eval(h, 0, SM_STEP, GATE_FSAL, WG_STORE_CUR, 0, 0, 0, 0, px, nullptr, nullptr);
+    for (int s = 1; s < 7; ++s) {
+      int fl;
+      float cb = (float)kB[s], ce = (float)kBT[s], cb1 = 0, ce1 = 0;
+      if (s == 1) { fl = WG_INIT; cb1 = (float)kB[0]; ce1 = (float)kBT[0]; }
+      else if (s < 6) fl = WG_B | WG_E;
+      else fl = WG_E | WG_STORE_NEXT;
+      vjp_eval(h, s, SM_STEP, GATE_ACTIVE, fl, cb, ce, cb1, ce1, px, nullptr, nullptr);
+    }
+    launch_mu_norm(0, ab, h->ctl, pm, nmb, GATE_ACTIVE, h->stream);
+    launch_adj_controller(h->ctl, px, nx, pm, nmb, nel, h->save_times, h->fwd_times, h->stream);
+  };
+  const bool graphs = h->use_graphs && h->prof_mask == 0;
+  const int ga = h->graph_attempts;
+  const void* asig[3] = {h->tape, h->save_times, d_dLdout};
+  if (h->adj_graph && std::memcmp(asig, h->adj_graph_sig, sizeof(asig)) != 0) {
+    (void)hipGraphExecDestroy(h->adj_graph);
+    h->adj_graph = nullptr;
+  }
+  int64_t chunk = graphs ? ga : 2, enqueued = 0;
   for (;;) {
     if (int rc = poll_ctl(h)) return rc;
     if (h->hctl->done) break;
     if (enqueued > c.maxiters + 64)
       return fail(h, NODE_ERR_STATE, "adjoint step controller did not terminate (internal error)");
     enqueued += chunk;
-    for (int64_t k = 0; k < chunk; ++k) {
-      launch_adj_step_begin(sg, ab, h->ctl, GATE_ALWAYS, h->stream);
-      vjp_eval(h, 0, SM_STEP, GATE_FSAL, WG_STORE_CUR, 0, 0, 0, 0, px, nullptr, nullptr);
-      for (int s = 1; s < 7; ++s) {
-        int fl;
-        float cb = (float)kB[s], ce = (float)kBT[s], cb1 = 0, ce1 = 0;
-        if (s == 1) { fl = WG_INIT; cb1 = (float)kB[0]; ce1 = (float)kBT[0]; }
-        else if (s < 6) fl = WG_B | WG_E;
-        else fl = WG_E | WG_STORE_NEXT;
-        vjp_eval(h, s, SM_STEP, GATE_ACTIVE, fl, cb, ce, cb1, ce1, px, nullptr, nullptr);
+    if (graphs) {
+      if (!h->adj_graph) {
+        hipGraph_t gr = nullptr;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        for (int k = 0; k < ga; ++k) attempt();
+        HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
+        HIPCHK(h, hipGraphInstantiate(&h->adj_graph, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        std::memcpy(h->adj_graph_sig, asig, sizeof(asig));
       }
-      launch_mu_norm(0, ab, h->ctl, pm, nmb, GATE_ACTIVE, h->stream);
-      launch_adj_controller(h->ctl, px, nx, pm, nmb, nel, h->save_times, h->fwd_times, h->stream);
+      for (int64_t k = 0; k < chunk; k += ga) HIPCHK(h, hipGraphLaunch(h->adj_graph, h->stream));
+    } else {
+      for (int64_t k = 0; k < chunk; ++k) attempt();
     }
     HIPCHK(h, hipGetLastError());
     chunk = std::min<int64_t>(chunk * 2, 16);
@@ -694,6 +890,8 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
     return NODE_ERR_HIP;
   }
   node_handle* h = new node_handle();
+  for (auto& row : h->tcfg)
+    for (int& c : row) c = -1;
   h->desc = *desc;
   h->L = desc->n_layers;
   h->D = desc->widths[0];
@@ -730,6 +928,7 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
       const int ktiles = (int)((int64_t)h->Bp * h->esz / 128);
       while (p.rt * p.ct * ly.wg_ks < 256 && ly.wg_ks < 16 && ktiles % (2 * ly.wg_ks) == 0)
         ly.wg_ks *= 2;
+      ly.wg_ks0 = ly.wg_ks;
     }
     if (!rc) rc = dalloc_t(h, &ly.bias, (size_t)ly.outp * sizeof(float));
     h->layers.push_back(ly);
@@ -765,6 +964,8 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
 int32_t node_destroy(node_handle_t* h) {
   if (!h) return 0;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->fwd_graph) (void)hipGraphExecDestroy(h->fwd_graph);
+  if (h->adj_graph) (void)hipGraphExecDestroy(h->adj_graph);
   for (void* p : h->allocs) (void)hipFree(p);
   for (auto& r : h->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -864,6 +1065,7 @@ int32_t node_forward(node_handle_t* h, const node_solve_opts* opts, const float*
   std::lock_guard<std::mutex> g(h->mtx);
   if (!u0) return fail(h, NODE_ERR_INVALID, "u0 is NULL");
   if (n_saveat < 0 || (n_saveat > 0 && !saveat)) return fail(h, NODE_ERR_INVALID, "bad saveat");
+  if (int rc = maybe_autotune(h)) return rc;  // before u0 is staged in the scratch buffer
   float* du0 = h->scratch;  // u0 staged through the scratch state buffer
   HIPCHK(h, hipMemcpyAsync(du0, u0, (size_t)h->B * h->D * sizeof(float), hipMemcpyHostToDevice, h->stream));
   int rc = forward_impl(h, opts, du0, saveat, n_saveat, stats);
@@ -927,6 +1129,7 @@ int32_t node_rhs(node_handle_t* h, const float* u, float* du) {
   if (!h || !u || !du) return NODE_ERR_INVALID;
   std::lock_guard<std::mutex> g(h->mtx);
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  if (int rc = maybe_autotune(h)) return rc;
   const StateGeom sg = geom(h);
   HIPCHK(h, hipMemcpyAsync(h->ucur, u, (size_t)h->B * h->D * sizeof(float), hipMemcpyHostToDevice, h->stream));
   const int64_t px = (int64_t)h->Bp * h->Dp;
@@ -943,6 +1146,7 @@ int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda, float* d
   std::lock_guard<std::mutex> g(h->mtx);
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
   if (int rc = ensure_adjoint(h)) return rc;
+  if (int rc = maybe_autotune(h)) return rc;
   const StateGeom sg = geom(h);
   const size_t sb = (size_t)h->B * h->D * sizeof(float);
   HIPCHK(h, hipMemcpyAsync(h->ucur, u, sb, hipMemcpyHostToDevice, h->stream));
@@ -958,6 +1162,56 @@ int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda, float* d
   HIPCHK(h, hipMemcpyAsync(p32.data(), h->dp32, h->P * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (int64_t i = 0; i < h->P; ++i) dp[i] = -(double)p32[i];  // epilogue stores -g
+  return 0;
+}
+
+int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (key == NODE_OPT_GRAPHS) {
+    h->use_graphs = value != 0;
+  } else if (key == NODE_OPT_AUTOTUNE) {
+    h->autotune = value != 0;
+    if (!h->autotune) {  // back to the heuristic plans
+      h->tuned = false;
+      for (auto& row : h->tcfg)
+        for (int& c : row) c = -1;
+      for (Layer& ly : h->layers) {
+        GemmPlan p = plan(ly.inp, ly.outp);
+        ly.wg_cfg = p.cfg; ly.wg_rt = p.rt; ly.wg_ct = p.ct; ly.wg_ks = ly.wg_ks0;
+      }
+    } else {
+      h->tuned = false;
+    }
+  } else if (key == NODE_OPT_GRAPH_ATTEMPTS) {
+    if (value < 1 || value > 16) return fail(h, NODE_ERR_INVALID, "graph attempts must be 1..16");
+    h->graph_attempts = (int)value;
+  } else {
+    return fail(h, NODE_ERR_INVALID, "unknown option");
+  }
+  if (h->fwd_graph) { (void)hipGraphExecDestroy(h->fwd_graph); h->fwd_graph = nullptr; }
+  if (h->adj_graph) { (void)hipGraphExecDestroy(h->adj_graph); h->adj_graph = nullptr; }
+  return 0;
+}
+
+int32_t node_get_plan(node_handle_t* h, int32_t role, int32_t layer, int32_t* tile_rows,
+                      int32_t* tile_cols, int32_t* ksplit) {
+  if (!h || role < 0 || role > 5 || layer < 0 || layer >= h->L) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int cfg, ks = 1;
+  if (role == PR_WGRAD) {
+    cfg = h->layers[layer].wg_cfg;
+    ks = h->layers[layer].wg_ks;
+  } else {
+    const Layer& ly = h->layers[layer];
+    int rows = h->Bp, cols = ly.outp, minc = 0;
+    if (role == PR_BWD_DATA) cols = ly.inp;
+    if (role == PR_FWD_STAGE || role == PR_BWD_LAST) { minc = 1; cols = role == PR_BWD_LAST ? h->Dp : ly.outp; }
+    cfg = plan_role(h, role, layer, rows, cols, minc).cfg;
+  }
+  if (tile_rows) *tile_rows = gemm_cfg_tile_rows(cfg);
+  if (tile_cols) *tile_cols = gemm_cfg_tile_cols(cfg);
+  if (ksplit) *ksplit = ks;
   return 0;
 }
 

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnode_hip.so")
 MAX_LAYERS = 8
 ACT = {"identity": 0, "tanh": 1, "softplus": 2, "relu": 3, "sigmoid": 4}
 MAP_IDENTITY, MAP_CUBE = 0, 1
+OPT_GRAPHS, OPT_GRAPH_ATTEMPTS, OPT_AUTOTUNE = 1, 2, 3
 PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 RET_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN",
              4: "MaxIters", 5: "DtLessThanMin", 6: "Unstable"}
@@ -69,6 +70,9 @@ SIGNATURES = [
     ("node_rhs", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("node_vjp", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.c_void_p]),
+    ("node_set_option", c_i32, [ctypes.c_void_p, c_i32, c_i64]),
+    ("node_get_plan", c_i32, [ctypes.c_void_p, c_i32, c_i32, ctypes.POINTER(c_i32),
+                              ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
     ("node_prof_enable", c_i32, [ctypes.c_void_p, ctypes.c_uint64]),
     ("node_prof_collect", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_i64,
                                   ctypes.POINTER(c_i64)]),

@@ -167,6 +167,32 @@ class Handle:
                     "node_vjp")
         return du, dp
 
+    def set_option(self, key, value):
+        """key: 'graphs' (bool) or 'graph_attempts' (1..16)."""
+        k = {"graphs": _lib.OPT_GRAPHS, "graph_attempts": _lib.OPT_GRAPH_ATTEMPTS,
+             "autotune": _lib.OPT_AUTOTUNE}[key]
+        self._check(self.lib.node_set_option(self.h, k, int(value)), "node_set_option")
+
+    def plans(self):
+        """{(role name, layer): (tile rows, tile cols, ksplit)} currently in use."""
+        out = {}
+        L = len(self.widths) - 1
+        for role, name in self.ROLES.items():
+            for layer in range(L):
+                if role in (0, 2) and layer == L - 1:
+                    continue
+                if role == 1 and layer != L - 1:
+                    continue
+                if role == 3 and layer == 0:
+                    continue
+                if role == 5 and layer != 0:
+                    continue
+                r, c, k = _lib.c_i32(), _lib.c_i32(), _lib.c_i32()
+                if self.lib.node_get_plan(self.h, role, layer, ctypes.byref(r), ctypes.byref(c),
+                                          ctypes.byref(k)) == 0:
+                    out[(name, layer + 1)] = (r.value, c.value, k.value)
+        return out
+
     # ------------------------------------------------------------ profiling
     ROLES = {0: "fwd_hidden", 1: "fwd_stage", 2: "adj_hidden", 3: "bwd_data",
              4: "wgrad", 5: "bwd_last"}

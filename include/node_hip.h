@@ -154,6 +154,21 @@ int32_t node_rhs(node_handle_t* h, const float* u, float* du);
 int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda,
                  float* du, double* dp);
 
+/* Execution options (no counterpart in the reference):
+ *   NODE_OPT_GRAPHS          1 (default): step attempts are captured once into a
+ *                            HIP graph and replayed; 0: launched one by one.
+ *   NODE_OPT_GRAPH_ATTEMPTS  step attempts per captured graph (default 4, 1..16). */
+/*   NODE_OPT_AUTOTUNE        1 (default): before its first solve a handle times every
+ *                            valid tile configuration (and split-K factor of the
+ *                            weight gradients) of each GEMM role on its own
+ *                            shapes and keeps the fastest; 0: fixed heuristic. */
+enum { NODE_OPT_GRAPHS = 1, NODE_OPT_GRAPH_ATTEMPTS = 2, NODE_OPT_AUTOTUNE = 3 };
+int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value);
+/* Tile plan chosen for a GEMM role (roles as for node_prof_enable below) and
+ * layer: tile rows x cols and split-K factor. */
+int32_t node_get_plan(node_handle_t* h, int32_t role, int32_t layer, int32_t* tile_rows,
+                      int32_t* tile_cols, int32_t* ksplit);
+
 /* Profiling hook (no counterpart in the reference; for measurement tools).
  * Records a HIP-event pair on the handle's stream around every GEMM launch
  * whose key (role * 8 + layer) has its bit set in key_mask; 0 disables.

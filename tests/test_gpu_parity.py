@@ -214,3 +214,47 @@ def test_reference_kwargs_matrix(ki, vec):
     print(f"  retcode {sol.retcode} Nt {len(sol.t)} grad rel err du0 {eu:.2e} dp {ep:.2e}",
           flush=True)
     assert eu < 5e-3 and ep < 5e-3
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_graph_replays_match_eager_launches(prec):
+    """Step attempts replayed from captured HIP graphs give bit-identical
+    results to launching the same kernels one by one, over several solves on
+    one handle (each solve replays its graph many times, with host polling and
+    other eager work in between)."""
+    from diffeqflux_hip import Handle
+    m, p, u0, h, (t0, t1) = setup_case("plain", prec)
+    opts = Handle.make_opts(t0, t1, abstol=1e-6, reltol=1e-5)
+    interior = list(np.linspace(t0, t1, 12)[1:-1])
+    c = _cotangents((12,) + u0.shape)
+    runs = []
+    for graphs, attempts in [(False, 1), (True, 4), (True, 1), (True, 4)]:
+        h.set_option("graphs", graphs)
+        h.set_option("graph_attempts", attempts)
+        out, ts, st = h.forward(u0, opts, interior)
+        du0, dp, sb = h.backward(c)
+        runs.append((out.copy(), du0.copy(), dp.copy(), st["naccept"], sb["naccept"]))
+        print(f"  graphs={graphs} x{attempts}: steps {st['naccept']}/{sb['naccept']}", flush=True)
+    for r in runs[1:]:
+        assert r[3] == runs[0][3] and r[4] == runs[0][4]
+        assert np.array_equal(r[0], runs[0][0])
+        assert np.array_equal(r[1], runs[0][1]) and np.array_equal(r[2], runs[0][2])
+
+
+def test_autotuned_plans_match_heuristic_results():
+    """The autotuner only changes tile shapes and split-K factors: the solve
+    agrees with the heuristic plan to FP32 rounding, and every chosen tile
+    divides its GEMM."""
+    from diffeqflux_hip import Handle
+    m, p, u0, h, (t0, t1) = setup_case("mixed", "bf16x3")
+    opts = Handle.make_opts(t0, t1, adaptive=False, dt=0.125)
+    c = _cotangents((3,) + u0.shape)
+    res = {}
+    for tuned in (False, True):
+        h.set_option("autotune", tuned)
+        out, ts, st = h.forward(u0, opts, [0.5])
+        du0, dp, _ = h.backward(c)
+        res[tuned] = (out, du0, dp)
+        print(f"  autotune={tuned}: plans {h.plans()}", flush=True)
+    for a, b in zip(res[False], res[True]):
+        assert rel(a, b) < 1e-5
