@@ -252,6 +252,7 @@ template <> struct Mma<float> {
 // ---------------------------------------------------------------- tile staging
 // Where a workgroup's tile sits and which wave block this thread owns.
 struct TileGeo {
+  int slot;         // tile index (partial-sum slot of the stage epilogues)
   int row0, col0;   // tile origin
   int rb, cb;       // this wave's block origin (absolute)
   int lr0, lc0;     // this wave's block origin inside the tile
@@ -459,7 +460,7 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
       }
     }
   EPI_LOOP_END
-  if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
+  if (e.part) block_reduce_store(sum, flag, e.part, geo.slot);
 }
 
 template <class PP, int BR, int BC, int NT, int MR, int MC>
@@ -552,7 +553,7 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
       }
     }
   EPI_LOOP_END
-  if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
+  if (e.part) block_reduce_store(sum, flag, e.part, geo.slot);
 }
 
 __device__ __forceinline__ void fold_one(const EpiWgrad& e, float* gcur, float* gnext,
@@ -717,7 +718,38 @@ __global__ void __launch_bounds__(WR * WC * 64)
   }
 #undef GEMM_GLOAD
 #undef GEMM_SSTORE
+  if (g.ws != nullptr && g.ksplit > 1) {
+    // split-K serial fixup: partial tile out (coalesced, thread-minor)
+    constexpr int TILE = MR * MC * 16 * NT;
+    float* mine = g.ws + ((int64_t)split * nt + bid) * TILE;
+#pragma unroll
+    for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+      for (int mj = 0; mj < MC; ++mj)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) mine[((mi * MC + mj) * 16 + v) * NT + tid] = acc[mi][mj][v];
+    __threadfence();
+    __syncthreads();
+    __shared__ int is_last;
+    if (tid == 0) is_last = atomicAdd(g.cnt + bid, 1) == g.ksplit - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+#pragma unroll
+    for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+      for (int mj = 0; mj < MC; ++mj)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int64_t o = (int64_t)((mi * MC + mj) * 16 + v) * NT + tid;
+          float sm = 0.f;
+          for (int sp = 0; sp < g.ksplit; ++sp) sm += g.ws[((int64_t)sp * nt + bid) * TILE + o];
+          acc[mi][mj][v] = sm;
+        }
+    if (tid == 0) g.cnt[bid] = 0;
+  }
   TileGeo geo;
+  geo.slot = bid;
   geo.row0 = row0;
   geo.col0 = col0;
   geo.lr0 = wr * (BR / WR);

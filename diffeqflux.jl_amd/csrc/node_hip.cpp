@@ -105,9 +105,12 @@ struct node_handle {
 
   // autotuned tile configuration per (role, layer); -1 = heuristic
   int tcfg[6][MAXL];
+  int tks[6][MAXL];       // split-K of the stage epilogues (serial fixup)
   bool tuned = false, autotune = true;
   int tune_role = -1, tune_layer = -1, tune_cfg = -1, tune_ks = 1;  // override while timing
   float* tune_wpart = nullptr;
+  float* fix_ws = nullptr;  // split-K fixup workspace / per-tile counters
+  int* fix_cnt = nullptr;
 
   // captured step-attempt graphs (re-captured when a captured pointer changes)
   bool use_graphs = true;
@@ -247,6 +250,13 @@ GemmPlan plan_role(node_handle* h, int role, int layer, int rows, int cols, int 
   return plan(rows, cols, min_cfg);
 }
 
+// split-K factor of a stage-epilogue role (1 without a fixup workspace)
+int stage_ks(node_handle* h, int role, int layer) {
+  if (!h->fix_ws) return 1;
+  if (h->tune_role == role && h->tune_layer == layer && h->tune_cfg >= 0) return h->tune_ks;
+  return h->tks[role][layer];
+}
+
 // hidden layer l: forward (Y) or adjoint recompute (Y, Y^T, act')
 void launch_hidden(node_handle* h, int l, int gate, bool adjoint) {
   const Layer& ly = h->layers[l];
@@ -274,6 +284,8 @@ void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* par
   g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
   GemmPlan p = plan_role(h, PR_FWD_STAGE, l, h->Bp, ly.outp, 1);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
+  g.ksplit = stage_ks(h, PR_FWD_STAGE, l);
+  g.ws = h->fix_ws; g.cnt = h->fix_cnt;
   EpiFwdStage e{};
   e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
   e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D; e.sv = (int64_t)h->Bp * h->D;
@@ -351,6 +363,8 @@ void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part
   g.pa = (int64_t)h->Bp * l0.outp; g.pb = (int64_t)l0.inp * l0.outp; g.ksplit = 1;
   GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 1);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
+  g.ksplit = stage_ks(h, PR_BWD_LAST, 0);
+  g.ws = h->fix_ws; g.cnt = h->fix_cnt;
   EpiBwdLast e{};
   e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
   e.kl = h->kl; e.sv = (int64_t)h->Bp * h->D;
@@ -492,12 +506,29 @@ int autotune_impl(node_handle* h) {
     tune_slot(PR_FWD_HIDDEN, l, h->Bp, ly.outp, 0, [&] { launch_hidden(h, l, GATE_ALWAYS, false); });
     tune_slot(PR_ADJ_HIDDEN, l, h->Bp, ly.outp, 0, [&] { launch_hidden(h, l, GATE_ALWAYS, true); });
   }
-  tune_slot(PR_FWD_STAGE, L - 1, h->Bp, h->layers[L - 1].outp, 1,
-            [&] { launch_fwd_stage(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr); });
+  auto tune_stage = [&](int role, int layer, int cols, int K, auto&& launch) {
+    float best = 1e30f;
+    int bc = -1, bks = 1;
+    const int kt = (int)((int64_t)K * h->esz / 128);
+    for (int c = 1; c < gemm_cfg_count(); ++c) {
+      if (h->Bp % gemm_cfg_tile_rows(c) || cols % gemm_cfg_tile_cols(c)) continue;
+      for (int ks = 1; ks <= 8; ks *= 2) {
+        if (kt % ks) continue;
+        h->tune_role = role; h->tune_layer = layer; h->tune_cfg = c; h->tune_ks = ks;
+        const float t = time_launches(h, launch, reps);
+        if (t < best) { best = t; bc = c; bks = ks; }
+      }
+    }
+    h->tune_role = h->tune_layer = h->tune_cfg = -1;
+    h->tcfg[role][layer] = bc;
+    h->tks[role][layer] = bks;
+  };
+  tune_stage(PR_FWD_STAGE, L - 1, h->layers[L - 1].outp, h->layers[L - 1].inp,
+             [&] { launch_fwd_stage(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr); });
   for (int l = 1; l < L; ++l)
     tune_slot(PR_BWD_DATA, l, h->Bp, h->layers[l].inp, 0, [&] { launch_bwd_data(h, l, GATE_ALWAYS); });
-  tune_slot(PR_BWD_LAST, 0, h->Bp, h->Dp, 1,
-            [&] { launch_bwd_last(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr, h->scratch); });
+  tune_stage(PR_BWD_LAST, 0, h->Dp, h->layers[0].outp,
+             [&] { launch_bwd_last(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr, h->scratch); });
   // weight gradients: tile x split-K; a split pays for its partial sums in the fold
   EpiWgrad wt{};
   wt.accb = h->accb; wt.acce = h->acce; wt.gf0 = h->gf0; wt.gf1 = h->gf1; wt.gptr = h->accb;
@@ -892,6 +923,8 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
   node_handle* h = new node_handle();
   for (auto& row : h->tcfg)
     for (int& c : row) c = -1;
+  for (auto& row : h->tks)
+    for (int& c : row) c = 1;
   h->desc = *desc;
   h->L = desc->n_layers;
   h->D = desc->widths[0];
@@ -950,6 +983,9 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
   if (!rc) rc = dalloc_t(h, &h->part, 4 * SEC * sizeof(double));
   if (!rc) rc = dalloc_t(h, &h->scratch, (size_t)h->Bp * h->D * sizeof(float));
   if (!rc) rc = dalloc_t(h, &h->ucur, (size_t)h->Bp * h->D * sizeof(float));
+  // split-K fixup workspace: up to 8 splits of a Bp x Dp (or last-layer) tile grid
+  if (!rc) rc = dalloc_t(h, &h->fix_ws, (size_t)8 * h->Bp * std::max(h->Dp, h->layers[h->L - 1].outp) * sizeof(float));
+  if (!rc) rc = dalloc_t(h, &h->fix_cnt, (size_t)(h->Bp / 32) * (std::max(h->Dp, h->layers[h->L - 1].outp) / 64) * sizeof(int));
   if (!rc && hipHostMalloc((void**)&h->hctl, sizeof(Ctl), 0) != hipSuccess) rc = NODE_ERR_HIP;
   if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = NODE_ERR_HIP;
   if (rc) {
@@ -1176,6 +1212,8 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
       h->tuned = false;
       for (auto& row : h->tcfg)
         for (int& c : row) c = -1;
+      for (auto& row : h->tks)
+        for (int& c : row) c = 1;
       for (Layer& ly : h->layers) {
         GemmPlan p = plan(ly.inp, ly.outp);
         ly.wg_cfg = p.cfg; ly.wg_rt = p.rt; ly.wg_ct = p.ct; ly.wg_ks = ly.wg_ks0;
@@ -1208,6 +1246,7 @@ int32_t node_get_plan(node_handle_t* h, int32_t role, int32_t layer, int32_t* ti
     if (role == PR_BWD_DATA) cols = ly.inp;
     if (role == PR_FWD_STAGE || role == PR_BWD_LAST) { minc = 1; cols = role == PR_BWD_LAST ? h->Dp : ly.outp; }
     cfg = plan_role(h, role, layer, rows, cols, minc).cfg;
+    if (role == PR_FWD_STAGE || role == PR_BWD_LAST) ks = h->tks[role][layer];
   }
   if (tile_rows) *tile_rows = gemm_cfg_tile_rows(cfg);
   if (tile_cols) *tile_cols = gemm_cfg_tile_cols(cfg);

@@ -53,6 +53,11 @@ struct GemmArgs {
   int K;                       // multiple of the K tile (times ksplit)
   int row_tiles, col_tiles;
   int ksplit;                  // split-K factor (weight gradients of narrow layers)
+  // split-K with serial fixup (stage epilogues): each split stores its partial
+  // tile in ws, the last split of a tile (per-tile counter) sums the partials in
+  // split order and runs the epilogue; counters return to zero afterwards
+  float* ws;
+  int* cnt;
 };
 
 // hidden layer forward: Y = act(acc + bias) (+ transposed copy, + act')

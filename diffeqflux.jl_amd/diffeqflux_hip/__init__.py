@@ -6,10 +6,11 @@ over the C ABI declared in ``include/node_hip.h``.
 """
 from .neural_ode import (NeuralODE, Chain, Dense, Cube, Tsit5, InterpolatingAdjoint,
                          ZygoteVJP, BacksolveAdjoint, TrackerAdjoint, QuadratureAdjoint,
-                         GaussAdjoint, ODESolution, save_plan)
+                         GaussAdjoint, ODESolution, save_plan, AugmentedNDELayer, DimMover)
 from .handle import Handle
 from ._lib import load, NodeHipError
 
 __all__ = ["NeuralODE", "Chain", "Dense", "Cube", "Tsit5", "InterpolatingAdjoint",
            "ZygoteVJP", "BacksolveAdjoint", "TrackerAdjoint", "QuadratureAdjoint",
-           "GaussAdjoint", "ODESolution", "save_plan", "Handle", "load", "NodeHipError"]
+           "GaussAdjoint", "ODESolution", "save_plan", "AugmentedNDELayer", "DimMover",
+           "Handle", "load", "NodeHipError"]

@@ -343,3 +343,70 @@ class _Lazy:
 
 
 _NeuralODEFn = _Lazy()
+
+
+# ---------------------------------------------------------------- wrappers
+class AugmentedNDELayer:
+    """``AugmentedNDELayer(nde, adim)`` (src/neural_de.jl:372-386): append
+    ``adim`` zero rows to the state (``__augment``: ``vcat(x, zeros)`` for a
+    vector, ``cat(x, zeros; dims = ndims(x) - 1)`` for a ``D x B`` matrix) and
+    run the wrapped NeuralODE on the augmented state.  The zeros carry no
+    gradient (``@ignore_derivatives``)."""
+
+    def __init__(self, nde, adim):
+        if not isinstance(nde, NeuralODE):
+            raise NotImplementedError("AugmentedNDELayer wraps a NeuralODE on the HIP path")
+        self.nde = nde
+        self.adim = int(adim)
+
+    def setup(self, rng, init="pytorch"):
+        return self.nde.setup(rng, init)
+
+    def augment(self, x):
+        try:
+            import torch
+            if isinstance(x, torch.Tensor):
+                z = torch.zeros(x.shape[:-1] + (self.adim,), dtype=x.dtype, device=x.device)
+                return torch.cat([x, z], dim=-1)
+        except ImportError:
+            pass
+        x = np.asarray(x)
+        return np.concatenate([x, np.zeros(x.shape[:-1] + (self.adim,), x.dtype)], axis=-1)
+
+    def __call__(self, x, p, st=None):
+        return self.nde(self.augment(x), p, st)
+
+
+class DimMover:
+    """``DimMover(from, to)`` (src/neural_de.jl:397-412): ``cat(eachslice(x;
+    dims = from)...; dims = to)`` with negative dims counted from the end in
+    Julia's column-major order.  Arrays here are the C-order mirror of the
+    Julia array (Julia dim ``k`` of an ``n``-dim array is numpy axis
+    ``n - k``), e.g. the ``(Nt, B, D)`` saved states of a ``D x B x Nt``
+    solution."""
+
+    def __init__(self, from_=-2, to=-1):
+        if from_ == 0 or to == 0:
+            raise ValueError("dims are 1-based (negative counts from the end)")
+        self.from_, self.to = from_, to
+
+    def __call__(self, x, ps=None, st=None):
+        n = x.ndim
+        f = self.from_ if self.from_ > 0 else n + 1 + self.from_
+        t = self.to if self.to > 0 else n + 1 + self.to
+        ax_f, ax_t = n - f, n - t          # numpy axes of the Julia dims
+        slices = [x.take(i, axis=ax_f) if hasattr(x, "take") and not _is_torch(x)
+                  else x.select(ax_f, i) for i in range(x.shape[ax_f])]
+        cat_ax = ax_t if ax_t < ax_f else ax_t - 1   # axis inside a slice
+        if _is_torch(x):
+            import torch
+            return torch.cat(slices, dim=cat_ax), st
+        return np.concatenate(slices, axis=cat_ax), st
+
+
+def _is_torch(x):
+    try:
+        import torch
+        return isinstance(x, torch.Tensor)
+    except ImportError:
+        return False
