@@ -5,8 +5,10 @@ Workload (the headline configuration): state D=64, MLP 64-1024-1024-1024-64
 with softplus, B=4096 trajectories u0 ~ N(0,1), tspan (0,1) with 32 saveat
 points, Tsit5 rtol=1e-4 (atol 1e-6), MSE loss against the saveat states of a
 random linear-plus-tanh teacher ODE, gradient by InterpolatingAdjoint.  GEMMs
-and VJPs in BF16 on MFMA (FP32 accumulation); state, stage sums and adjoint
-state in FP32; time / controller / dL/dp quadrature in FP64.
+and VJPs on BF16 MFMA (FP32 accumulation): by default split BF16 (hi + lo
+operand planes, three passes) for the right-hand side and the adjoint's data
+path, one BF16 pass for the weight-gradient products; state, stage sums and
+adjoint state in FP32; time / controller / dL/dp quadrature in FP64.
 
 One "step" = one forward solve + loss cotangent + one adjoint solve, all on
 inputs resident in HBM.  ``value`` = batched RHS evaluations (forward RHS +
@@ -86,21 +88,26 @@ def make_inputs(B, seed=1):
     return u0, ts, teacher_targets(u0, ts)
 
 
-def flops_per_launch(role, layer, B, precision):
+def flops_per_launch(role, layer, B, precision, wgrad_precision=None):
     """Algorithmic FLOPs of one launch of the (role, layer) GEMM: 2*B*in*out.
     For split BF16 the MFMA executes three BF16 passes; the roofline counts
-    the MFMA work actually issued (3x) against the BF16 peak."""
+    the MFMA work actually issued (3x) against the BF16 peak.  Role 4 (weight
+    gradients) runs in ``wgrad_precision`` when that is given."""
     i, o = WIDTHS[layer], WIDTHS[layer + 1]
-    return 2.0 * B * i * o * (3 if precision == "bf16x3" else 1)
+    prec = wgrad_precision if (role == 4 and wgrad_precision) else precision
+    return 2.0 * B * i * o * (3 if prec == "bf16x3" else 1)
 
 
 # ---------------------------------------------------------------- GPU legs
 class Runner:
-    def __init__(self, precision, B, p, u0, ts, targets):
+    def __init__(self, precision, B, p, u0, ts, targets, wgrad_precision=None):
         import torch
         from diffeqflux_hip import Handle
         self.torch = torch
         self.h = Handle(WIDTHS, ACTS, 0, precision, B)
+        if wgrad_precision:
+            self.h.set_option("wgrad_precision", wgrad_precision)
+        self.wgrad_precision = wgrad_precision
         self.h.set_params(p)
         self.opts = Handle.make_opts(TSPAN[0], TSPAN[1], abstol=ATOL, reltol=RTOL)
         self.interior = list(ts[1:-1])
@@ -157,7 +164,7 @@ def roofline_for(run, dom, precision, durations):
     thr = 0.25 * np.percentile(d, 90)
     real = d[d >= thr]
     avg_ms = float(real.mean())
-    fl = flops_per_launch(dom[0], dom[1], run.B, precision)
+    fl = flops_per_launch(dom[0], dom[1], run.B, precision, run.wgrad_precision)
     achieved = fl / (avg_ms * 1e-3) / 1e12
     peak = PEAK[precision]
     info = {"kernel": f"{ROLE_NAMES[dom[0]]}[layer {dom[1] + 1}]",
@@ -216,6 +223,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
+    ap.add_argument("--wgrad-precision", default="bf16", choices=["bf16x3", "bf16", "fp32", "same"],
+                    help="precision of the adjoint weight-gradient products (bf16 on a bf16x3 "
+                         "run; 'same' = --precision)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison leg")
     ap.add_argument("--no-bf16", action="store_true", help="skip the pure-BF16 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -230,9 +240,13 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     p = glorot_params()
     u0, ts, tgt = make_inputs(args.batch)
-    log(f"[bench] {torch.cuda.get_device_name(0)} B={args.batch} {args.precision}")
+    wgp = None if args.wgrad_precision in ("same", args.precision) else args.wgrad_precision
+    if wgp and args.precision != "bf16x3":
+        wgp = None
+    log(f"[bench] {torch.cuda.get_device_name(0)} B={args.batch} {args.precision}"
+        f" (weight gradients {wgp or args.precision})")
 
-    run = Runner(args.precision, args.batch, p, u0, ts, tgt)
+    run = Runner(args.precision, args.batch, p, u0, ts, tgt, wgp)
     run.step()  # warm the allocator / code objects; the handle autotunes its tiles here
     plans = {f"{k[0]}{k[1]}": f"{v[0]}x{v[1]}" + (f"/k{v[2]}" if v[2] > 1 else "")
              for k, v in run.h.plans().items()}
@@ -282,15 +296,19 @@ def main():
         Bs = args.cpu_batch
         us, tss, tgs = make_inputs(Bs)
         grads = {}
-        for prec in ("bf16x3", "bf16", "fp32"):
-            rs = Runner(prec, Bs, p, us, tss, tgs)
+        variants = [("bf16x3", "bf16x3", None), ("bf16x3_wgrad_bf16", "bf16x3", "bf16"),
+                    ("bf16", "bf16", None), ("fp32", "fp32", None)]
+        for name, prec, wg in variants:
+            rs = Runner(prec, Bs, p, us, tss, tgs, wg)
             _, _, du0, dp = rs.step()
-            grads[prec] = (du0.cpu().numpy(), dp.cpu().numpy())
+            grads[name] = (du0.cpu().numpy(), dp.cpu().numpy())
         cpu, errs = cpu_baseline_leg(p, Bs, grads)
-        for prec in ("bf16x3", "bf16", "fp32"):
+        for prec, _, _ in variants:
             extra[f"grad_rel_err_{prec}"] = float(f"{errs[prec]:.3e}")
             extra[f"grad_rel_err_du0_{prec}"] = float(f"{errs[prec + '_du0']:.3e}")
         extra["grad_rel_err_sample"] = f"||dL/dp - FP64 oracle|| / ||oracle||, B={Bs} sample"
+        head = "bf16x3_wgrad_bf16" if wgp == "bf16" else args.precision
+        extra["grad_rel_err"] = extra[f"grad_rel_err_{head}"]
         log(f"[bench] cpu baseline {cpu}; grad rel err {errs}")
 
     rec = {
@@ -298,6 +316,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+        "wgrad_dtype": wgp or args.precision,
         "dtype_note": {"bf16x3": "BF16 MFMA operands split hi+lo (3 passes), FP32 accumulate; "
                                  "FP32 state, FP64 time/controller/dL/dp quadrature",
                        "bf16": "BF16 MFMA operands, FP32 accumulate; FP32 state",
@@ -306,7 +325,8 @@ def main():
         "config": {"workload": "NeuralODE fwd+InterpolatingAdjoint, D=64, MLP "
                                "64-1024-1024-1024-64 softplus, B=4096, tspan (0,1), 32 saveat, "
                                "Tsit5 rtol=1e-4 atol=1e-6, MSE loss",
-                   "batch": args.batch, "precision": args.precision},
+                   "batch": args.batch, "precision": args.precision,
+                   "wgrad_precision": wgp or args.precision},
         "traj_per_s": round(args.batch * args.steps / dt, 2),
         "nf_fwd": sf["nf"], "nf_adj": sa["nf"],
         "steps_fwd": f"{sf['naccept']}+{sf['nreject']}",

@@ -929,7 +929,7 @@ __device__ double block_sum_parts(const double* part, int n, double* flag_out) {
   return out;
 }
 
-__device__ void interp_weights(double th, float* w) {
+__device__ __forceinline__ void interp_weights(double th, float* w) {
   const double th2 = th * th;
   w[0] = (float)(th * (dR1[0] + th * (dR1[1] + th * (dR1[2] + th * dR1[3]))));
   for (int i = 0; i < 6; ++i) w[i + 1] = (float)(th2 * (dRI[i][0] + th * (dRI[i][1] + th * dRI[i][2])));
@@ -957,8 +957,10 @@ __device__ void set_interp(Ctl* c, int slot, double tau, const double* ft) {
 }
 
 // loopheader + check_error for the next attempt; dt already holds the
-// proposal (after accept) or the shrunk step (after reject)
-__device__ void loopheader(Ctl* c, double tstop) {
+// proposal (after accept) or the shrunk step (after reject).  C: the shared
+// control block (Ctl) or one trajectory's registers (SlotCtl).
+template <class C>
+__device__ __forceinline__ void loopheader(C* c, double tstop) {
   c->iter += 1;
   double dt = c->dt;
   if (fabs(dt) > c->dtmax_abs) dt = c->tdir * c->dtmax_abs;
@@ -975,12 +977,13 @@ __device__ void loopheader(Ctl* c, double tstop) {
   }
 }
 
-__device__ double snap_t(double t, double ttmp, double tstop) {
+__device__ __forceinline__ double snap_t(double t, double ttmp, double tstop) {
   return fabs(ttmp - tstop) < 100.0 * dspacing(fmax(t, tstop)) ? tstop : ttmp;
 }
 
 // accept/reject decision common to forward and adjoint; returns accepted
-__device__ bool pi_control(Ctl* c, double eest) {
+template <class C>
+__device__ __forceinline__ bool pi_control(C* c, double eest) {
   c->eest = eest;
   if (!c->adaptive) return true;
   double q;
@@ -1420,5 +1423,7 @@ void launch_gather(const StateGeom& sg, const float* src, int64_t src_stride_flo
   hipLaunchKernelGGL(k_gather, dim3(EW_MAX_BLOCKS), dim3(EW_THREADS), 0, s, sg, src,
                      src_stride_floats, dst, n);
 }
+
+#include "ragged_kernels.h"
 
 }  // namespace nodehip

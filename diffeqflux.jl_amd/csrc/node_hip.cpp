@@ -55,6 +55,7 @@ struct GemmPlan {
 struct node_handle {
   node_desc desc{};
   int L = 0, D = 0, Dp = 0, B = 0, Bp = 0, prec = 0, esz = 4, planes = 1;
+  int wg_prec = 0;  // precision of the weight-gradient products (<= prec)
   int64_t P = 0;
   std::vector<Layer> layers;
   hipStream_t stream = nullptr;
@@ -118,6 +119,30 @@ struct node_handle {
   const void* fwd_graph_sig[3] = {nullptr, nullptr, nullptr};  // tape, save times, outputs
   const void* adj_graph_sig[3] = {nullptr, nullptr, nullptr};  // tape, save times, cotangents
   int graph_attempts = 4;
+
+  // ragged batches: flat parameters and the last per-trajectory forward
+  float* pflat = nullptr;
+  struct Ragged {
+    int64_t n = 0, n_cap = 0, ns = 0, ns_cap = 0;
+    double* t1 = nullptr;
+    float* u0 = nullptr;
+    int64_t* soff = nullptr;
+    double* st = nullptr;
+    float* out = nullptr;
+    int* ret = nullptr;
+    int* tstats = nullptr;
+    int* last = nullptr;
+    int* ctr = nullptr;                     // [0] queue, [1] tape records
+    unsigned long long* stats = nullptr;    // nf, naccept, nreject
+    double* rec_t = nullptr;
+    int* rec_prev = nullptr;
+    float* rec_data = nullptr;
+    int64_t rec_cap = 0;
+    bool have = false;
+    node_solve_opts opts{};
+    std::vector<int> h_ret;
+  } rg;
+  int n_cu = 0;
 
   // profiling (node_prof_enable)
   uint64_t prof_mask = 0;
@@ -349,7 +374,7 @@ void launch_wgrad(node_handle* h, int l, int gate, const EpiWgrad& wt) {
   e.in = ly.in; e.out = ly.out; e.w0 = ly.w0;
   e.wpart = ks > 1 ? wpart : nullptr;
   ProfScope ps(h, PR_WGRAD, l);
-  launch_gemm_wgrad(h->prec, cfg, g, e, h->ctl, gate, h->stream);
+  launch_gemm_wgrad(h->wg_prec, cfg, g, e, h->ctl, gate, h->stream);
 }
 
 // layer-1 data gradient with the adjoint stage epilogue
@@ -880,6 +905,171 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   return 0;
 }
 
+// ---------------------------------------------------------------- ragged batches
+// LDS layout of the per-trajectory kernels: per layer W rows [16*to][16*ti + 4]
+// then the bias [16*to]
+int ragged_model(node_handle* h, RaggedModel& m, std::string& why) {
+  const node_desc& d = h->desc;
+  m = RaggedModel{};
+  if (d.n_layers > RG_MAXL) { why = "per-trajectory solves support at most 4 Dense layers"; return -1; }
+  if (d.widths[0] > RG_MAXD) { why = "per-trajectory solves support state dimension D <= 32"; return -1; }
+  m.L = d.n_layers;
+  m.map = d.input_map;
+  int off = 0;
+  for (int l = 0; l < m.L; ++l) {
+    const int in = d.widths[l], out = d.widths[l + 1];
+    if (l < m.L - 1 && out > 16 * RG_HT) { why = "per-trajectory solves support hidden widths <= 128"; return -1; }
+    m.in[l] = in;
+    m.out[l] = out;
+    m.act[l] = d.acts[l];
+    m.ti[l] = (in + 15) / 16;
+    m.to[l] = (out + 15) / 16;
+    m.pitch[l] = 16 * m.ti[l] + 4;
+    m.lw[l] = off;
+    off += 16 * m.to[l] * m.pitch[l];
+    m.lb[l] = off;
+    off += 16 * m.to[l];
+    m.w0[l] = h->layers[l].w0;
+    m.b0[l] = h->layers[l].b0;
+  }
+  m.lds_floats = off;
+  if ((size_t)off * sizeof(float) > 160 * 1024) { why = "model weights exceed the 160 KB LDS of a workgroup"; return -1; }
+  m.P = h->P;
+  m.p = h->pflat;
+  return 0;
+}
+
+template <typename P>
+int grow(node_handle* h, P** p, int64_t need, int64_t& cap_elems, int64_t new_cap) {
+  if (*p && need <= cap_elems) return 0;
+  dfree(h, *p);
+  *p = nullptr;
+  cap_elems = new_cap;
+  return dalloc_t(h, p, (size_t)new_cap * sizeof(P));
+}
+
+int ragged_forward_impl(node_handle* h, const node_solve_opts* o, int64_t n, const double* t1,
+                        const float* u0, const int64_t* soff, const double* st, float* out,
+                        int32_t* ret, int32_t* tstats, node_ragged_stats* stats) {
+  if (!o || !t1 || !u0 || !soff || n < 1) return fail(h, NODE_ERR_INVALID, "null argument or n < 1");
+  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  RaggedModel m;
+  std::string why;
+  if (ragged_model(h, m, why)) return fail(h, NODE_ERR_UNSUPPORTED, why);
+  if (soff[0] != 0) return fail(h, NODE_ERR_INVALID, "save_offsets[0] must be 0");
+  if (!o->adaptive && !(o->dt > 0)) return fail(h, NODE_ERR_INVALID, "fixed-step solve needs dt > 0");
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(t1[i] > o->t0)) return fail(h, NODE_ERR_INVALID, "every trajectory needs t1 > t0");
+    if (soff[i + 1] < soff[i]) return fail(h, NODE_ERR_INVALID, "save_offsets must be non-decreasing");
+    for (int64_t k = soff[i]; k < soff[i + 1]; ++k) {
+      if (!(st[k] >= o->t0 && st[k] <= t1[i])) return fail(h, NODE_ERR_INVALID, "save time outside [t0, t1]");
+      if (k > soff[i] && !(st[k] > st[k - 1])) return fail(h, NODE_ERR_INVALID, "save times must increase");
+    }
+  }
+  const int64_t ns = soff[n];
+  if (ns > 0 && (!st || !out)) return fail(h, NODE_ERR_INVALID, "save times / output missing");
+  const int D = h->D, Dp = (D + 15) / 16 * 16;
+  auto& r = h->rg;
+  r.have = false;
+  int64_t c1 = r.n_cap, c2 = r.n_cap, c3 = r.n_cap, c4 = r.n_cap, c5 = r.n_cap, c6 = r.n_cap;
+  if (!r.t1 || n > r.n_cap) {
+    const int64_t cap = std::max<int64_t>(n, 64);
+    if (grow(h, &r.t1, n, c1, cap) || grow(h, &r.u0, n * D, c2, cap * D) ||
+        grow(h, &r.soff, n + 1, c3, cap + 1) || grow(h, &r.ret, n, c4, cap) ||
+        grow(h, &r.tstats, 4 * n, c5, 4 * cap) || grow(h, &r.last, n, c6, cap))
+      return NODE_ERR_HIP;
+    r.n_cap = cap;
+  }
+  if (!r.st || ns > r.ns_cap) {
+    const int64_t cap = std::max<int64_t>(ns, 64);
+    int64_t a1 = r.ns_cap, a2 = r.ns_cap;
+    if (grow(h, &r.st, ns, a1, cap) || grow(h, &r.out, ns * D, a2, cap * D)) return NODE_ERR_HIP;
+    r.ns_cap = cap;
+  }
+  if (!r.ctr) {
+    if (dalloc_t(h, &r.ctr, 4 * sizeof(int)) || dalloc_t(h, &r.stats, 4 * sizeof(unsigned long long)))
+      return NODE_ERR_HIP;
+  }
+  if (h->n_cu == 0) {
+    int dev = 0;
+    HIPCHK(h, hipGetDevice(&dev));
+    HIPCHK(h, hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  HIPCHK(h, hipMemcpyAsync(r.t1, t1, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(r.u0, u0, n * D * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(r.soff, soff, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+  if (ns > 0)
+    HIPCHK(h, hipMemcpyAsync(r.st, st, ns * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (r.rec_cap < n * 16) {
+    const int64_t cap = n * 64;
+    dfree(h, r.rec_t); dfree(h, r.rec_prev); dfree(h, r.rec_data);
+    r.rec_t = nullptr; r.rec_prev = nullptr; r.rec_data = nullptr;
+    if (dalloc_t(h, &r.rec_t, cap * sizeof(double)) || dalloc_t(h, &r.rec_prev, cap * sizeof(int)) ||
+        dalloc_t(h, &r.rec_data, (size_t)cap * 8 * Dp * sizeof(float)))
+      return NODE_ERR_HIP;
+    r.rec_cap = cap;
+  }
+  RaggedFwdArgs a{};
+  a.m = m;
+  a.D = D;
+  a.Dp = Dp;
+  a.n = n;
+  a.t1 = r.t1; a.u0 = r.u0; a.save_off = r.soff; a.save_t = r.st; a.out = r.out;
+  a.retcode = r.ret; a.traj_stats = r.tstats; a.last_rec = r.last;
+  a.queue = r.ctr; a.rec_count = r.ctr + 1;
+  a.t0 = o->t0;
+  a.abstol = o->abstol > 0 ? o->abstol : 1e-6;
+  a.reltol = o->reltol > 0 ? o->reltol : 1e-3;
+  a.dtmax = o->dtmax;
+  a.dtmin = o->dtmin;
+  a.dt_init = std::fabs(o->dt);
+  a.maxiters = o->maxiters > 0 ? o->maxiters : 100000;
+  a.adaptive = o->adaptive ? 1 : 0;
+  a.stats = r.stats;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(h->n_cu, (n + 63) / 64));
+  int ctr[4];
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    a.rec_t = r.rec_t; a.rec_prev = r.rec_prev; a.rec_data = r.rec_data; a.rec_cap = r.rec_cap;
+    HIPCHK(h, hipMemsetAsync(r.ctr, 0, 4 * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(r.stats, 0, 4 * sizeof(unsigned long long), h->stream));
+    if (ns > 0) HIPCHK(h, hipMemsetAsync(r.out, 0, ns * D * sizeof(float), h->stream));
+    launch_ragged_fwd(a, grid, h->stream);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(ctr, r.ctr, sizeof(ctr), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (ctr[1] <= r.rec_cap) break;
+    // tape pool too small: grow to what the solve needed and run again
+    const int64_t cap = (int64_t)ctr[1] + n;
+    dfree(h, r.rec_t); dfree(h, r.rec_prev); dfree(h, r.rec_data);
+    r.rec_t = nullptr; r.rec_prev = nullptr; r.rec_data = nullptr;
+    if (dalloc_t(h, &r.rec_t, cap * sizeof(double)) || dalloc_t(h, &r.rec_prev, cap * sizeof(int)) ||
+        dalloc_t(h, &r.rec_data, (size_t)cap * 8 * Dp * sizeof(float)))
+      return NODE_ERR_HIP;
+    r.rec_cap = cap;
+  }
+  if (ctr[1] > r.rec_cap) return fail(h, NODE_ERR_CAPACITY, "tape pool kept overflowing");
+  r.h_ret.resize(n);
+  HIPCHK(h, hipMemcpyAsync(r.h_ret.data(), r.ret, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (ns > 0) HIPCHK(h, hipMemcpyAsync(out, r.out, ns * D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (tstats) HIPCHK(h, hipMemcpyAsync(tstats, r.tstats, 4 * n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  unsigned long long hs[4];
+  HIPCHK(h, hipMemcpyAsync(hs, r.stats, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (ret) std::memcpy(ret, r.h_ret.data(), n * sizeof(int));
+  if (stats) {
+    node_ragged_stats s{};
+    s.nf = (int64_t)hs[0]; s.naccept = (int64_t)hs[1]; s.nreject = (int64_t)hs[2];
+    s.records = ctr[1];
+    for (int v : r.h_ret) s.n_success += v == NODE_RET_SUCCESS;
+    *stats = s;
+  }
+  r.n = n;
+  r.ns = ns;
+  r.opts = *o;
+  r.have = true;
+  return 0;
+}
+
 int copy_grads(node_handle* h, float* dLdu0, double* dLdp, hipMemcpyKind kind) {
   // lambda real rows are the first B*D floats (row stride D)
   if (dLdu0)
@@ -932,6 +1122,7 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
   h->B = (int)desc->batch;
   h->Bp = (int)rup(h->B, BPAD);
   h->prec = desc->precision;
+  h->wg_prec = h->prec;
   h->esz = h->prec == NODE_PREC_FP32 ? 4 : 2;
   h->planes = h->prec == NODE_PREC_BF16X3 ? 2 : 1;
   h->P = node_num_params(desc);
@@ -1056,8 +1247,11 @@ static int set_params_host(node_handle* h, const float* p, int64_t n) {
     }
     HIPCHK(h, hipMemcpy(ly.bias, b.data(), ly.outp * 4, hipMemcpyHostToDevice));
   }
+  if (!h->pflat && dalloc_t(h, &h->pflat, h->P * sizeof(float))) return NODE_ERR_HIP;
+  HIPCHK(h, hipMemcpy(h->pflat, p, h->P * sizeof(float), hipMemcpyHostToDevice));
   h->have_params = true;
   h->have_forward = false;
+  h->rg.have = false;
   return 0;
 }
 
@@ -1161,6 +1355,16 @@ int32_t node_backward(node_handle_t* h, const float* dLdout, float* dLdu0, doubl
   return copy_grads(h, dLdu0, dLdp, hipMemcpyDeviceToHost);
 }
 
+int32_t node_ragged_forward(node_handle_t* h, const node_solve_opts* opts, int64_t n,
+                            const double* t1, const float* u0, const int64_t* save_offsets,
+                            const double* save_times, float* out, int32_t* retcodes,
+                            int32_t* traj_stats, node_ragged_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return ragged_forward_impl(h, opts, n, t1, u0, save_offsets, save_times, out, retcodes,
+                             traj_stats, stats);
+}
+
 int32_t node_rhs(node_handle_t* h, const float* u, float* du) {
   if (!h || !u || !du) return NODE_ERR_INVALID;
   std::lock_guard<std::mutex> g(h->mtx);
@@ -1221,6 +1425,13 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
     } else {
       h->tuned = false;
     }
+  } else if (key == NODE_OPT_WGRAD_PRECISION) {
+    // BF16X3 operands hold the BF16 rounding in their first plane, so the
+    // weight-gradient products can read that plane alone
+    const bool ok = value == h->prec || (h->prec == NODE_PREC_BF16X3 && value == NODE_PREC_BF16);
+    if (!ok) return fail(h, NODE_ERR_INVALID, "weight-gradient precision must equal the handle's, or be BF16 on a BF16X3 handle");
+    h->wg_prec = (int)value;
+    h->tuned = false;
   } else if (key == NODE_OPT_GRAPH_ATTEMPTS) {
     if (value < 1 || value > 16) return fail(h, NODE_ERR_INVALID, "graph attempts must be 1..16");
     h->graph_attempts = (int)value;

@@ -202,4 +202,44 @@ void launch_prep_lambda(const StateGeom& sg, const float* lam, void* L, void* LT
 void launch_gather(const StateGeom& sg, const float* src, int64_t src_stride_floats,
                    float* dst, int64_t n, hipStream_t s);
 
+// ---------------------------------------------------------------- ragged batches
+constexpr int RG_MAXL = 4;
+constexpr int RG_HT = 8;        // hidden width <= 128 (16-feature tiles)
+constexpr int RG_MAXD = 32;     // state dimension <= 32
+struct RaggedModel {
+  int L, map;
+  int in[RG_MAXL], out[RG_MAXL], act[RG_MAXL];
+  int ti[RG_MAXL], to[RG_MAXL];       // 16-feature tiles of each layer's input / output
+  int pitch[RG_MAXL];                 // LDS row pitch (floats) of W_l: 16*ti + 4
+  int lw[RG_MAXL], lb[RG_MAXL];       // LDS offsets (floats) of W_l rows and bias
+  int64_t w0[RG_MAXL], b0[RG_MAXL];   // flat parameter offsets
+  int lds_floats;
+  int64_t P;
+  const float* p;                     // flat FP32 parameters (device)
+};
+struct RaggedFwdArgs {
+  RaggedModel m;
+  int D, Dp;
+  int64_t n;
+  const double* t1;                   // [n] tspan ends
+  const float* u0;                    // [n][D]
+  const int64_t* save_off;            // [n + 1]
+  const double* save_t;               // [save_off[n]]
+  float* out;                         // [save_off[n]][D]
+  int* retcode;                       // [n]
+  int* traj_stats;                    // [n][4]: nf, naccept, nreject, records
+  int* last_rec;                      // [n] last tape record of each trajectory
+  int* queue;                         // next trajectory to start
+  double* rec_t;                      // tape pool: start time of each record
+  int* rec_prev;                      // previous record of the same trajectory (-1: first)
+  float* rec_data;                    // [rec_cap][8][Dp]: u, k1..k7
+  int64_t rec_cap;
+  int* rec_count;
+  double t0, abstol, reltol, dtmax, dtmin, dt_init;
+  int64_t maxiters;
+  int adaptive;
+  unsigned long long* stats;          // nf, naccept, nreject over all trajectories
+};
+void launch_ragged_fwd(const RaggedFwdArgs& a, int grid, hipStream_t s);
+
 }  // namespace nodehip

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnode_hip.so")
 MAX_LAYERS = 8
 ACT = {"identity": 0, "tanh": 1, "softplus": 2, "relu": 3, "sigmoid": 4}
 MAP_IDENTITY, MAP_CUBE = 0, 1
-OPT_GRAPHS, OPT_GRAPH_ATTEMPTS, OPT_AUTOTUNE = 1, 2, 3
+OPT_GRAPHS, OPT_GRAPH_ATTEMPTS, OPT_AUTOTUNE, OPT_WGRAD_PRECISION = 1, 2, 3, 4
 PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 RET_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN",
              4: "MaxIters", 5: "DtLessThanMin", 6: "Unstable"}
@@ -46,6 +46,14 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class RaggedStats(ctypes.Structure):
+    _fields_ = [("nf", c_i64), ("naccept", c_i64), ("nreject", c_i64), ("records", c_i64),
+                ("n_success", c_i64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every entry point declared in node_hip.h
 SIGNATURES = [
     ("node_abi_version", c_i32, []),
@@ -67,6 +75,10 @@ SIGNATURES = [
                               ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("node_backward_device", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    ("node_ragged_forward", c_i32, [ctypes.c_void_p, ctypes.POINTER(SolveOpts), c_i64,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.POINTER(RaggedStats)]),
     ("node_rhs", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("node_vjp", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.c_void_p]),

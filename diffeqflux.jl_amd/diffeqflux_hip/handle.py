@@ -152,6 +152,33 @@ class Handle:
             self._check(rc, "node_backward")
         return du0, dp, st.as_dict()
 
+    def ragged_forward(self, u0, t1, save_times, t0=0.0, abstol=1e-6, reltol=1e-3, adaptive=True,
+                       dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=0):
+        """Per-trajectory solves.  u0: (n, D); t1: (n,); save_times: list of n
+        increasing sequences.  Returns (list of (len_i, D) saved states,
+        retcodes (n,), per-trajectory stats (n, 4) [nf, naccept, nreject,
+        records], totals dict)."""
+        u0 = _f32(u0).reshape(-1, self.D)
+        n = u0.shape[0]
+        t1 = np.ascontiguousarray(np.asarray(t1, dtype=np.float64).reshape(n))
+        lens = [len(s) for s in save_times]
+        off = np.zeros(n + 1, np.int64)
+        off[1:] = np.cumsum(lens)
+        st = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float64) for s in save_times])
+                                  if off[-1] else np.zeros(1))
+        out = np.zeros((max(int(off[-1]), 1), self.D), np.float32)
+        ret = np.zeros(n, np.int32)
+        ts = np.zeros((n, 4), np.int32)
+        o = self.make_opts(t0, t0, abstol=abstol, reltol=reltol, adaptive=adaptive, dt=dt,
+                           dtmax=dtmax, dtmin=dtmin, maxiters=maxiters)
+        rs = _lib.RaggedStats()
+        self._check(self.lib.node_ragged_forward(self.h, ctypes.byref(o), n, _ptr(t1), _ptr(u0),
+                                                 _ptr(off), _ptr(st), _ptr(out), _ptr(ret),
+                                                 _ptr(ts), ctypes.byref(rs)),
+                    "node_ragged_forward")
+        self.ragged_offsets = off
+        return [out[off[i]:off[i + 1]] for i in range(n)], ret, ts, rs.as_dict()
+
     def rhs(self, u):
         u = _f32(u).reshape(self.B, self.D)
         du = np.empty_like(u)
@@ -168,9 +195,12 @@ class Handle:
         return du, dp
 
     def set_option(self, key, value):
-        """key: 'graphs' (bool) or 'graph_attempts' (1..16)."""
+        """key: 'graphs' (bool), 'graph_attempts' (1..16), 'autotune' (bool) or
+        'wgrad_precision' ('fp32' / 'bf16' / 'bf16x3')."""
         k = {"graphs": _lib.OPT_GRAPHS, "graph_attempts": _lib.OPT_GRAPH_ATTEMPTS,
-             "autotune": _lib.OPT_AUTOTUNE}[key]
+             "autotune": _lib.OPT_AUTOTUNE, "wgrad_precision": _lib.OPT_WGRAD_PRECISION}[key]
+        if key == "wgrad_precision" and isinstance(value, str):
+            value = _lib.PREC[value]
         self._check(self.lib.node_set_option(self.h, k, int(value)), "node_set_option")
 
     def plans(self):

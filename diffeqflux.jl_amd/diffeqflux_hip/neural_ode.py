@@ -217,10 +217,13 @@ _SOLVER_KW = {"saveat", "save_everystep", "save_start", "save_end", "abstol", "r
 class NeuralODE:
     """``NeuralODE(model, tspan, alg=Tsit5(); kwargs...)`` (src/neural_de.jl:42-45).
 
-    Extra keyword ``precision`` ('fp32' or 'bf16') selects the GEMM precision
-    of the right-hand side and its VJPs on the GPU."""
+    Extra keywords (no counterpart in the reference): ``precision`` ('fp32',
+    'bf16' or 'bf16x3') selects the GEMM precision of the right-hand side and
+    its VJPs on the GPU; ``wgrad_precision`` (default: ``precision``) that of
+    the adjoint's weight-gradient products ('bf16' on a 'bf16x3' layer)."""
 
-    def __init__(self, model, tspan, alg=None, *args, precision="fp32", **kwargs):
+    def __init__(self, model, tspan, alg=None, *args, precision="fp32", wgrad_precision=None,
+                 **kwargs):
         if alg is None:
             alg = Tsit5()
         if not isinstance(alg, Tsit5):
@@ -236,6 +239,7 @@ class NeuralODE:
         self.args = args
         self.kwargs = dict(kwargs)
         self.precision = precision
+        self.wgrad_precision = wgrad_precision
         self._handles = {}
 
     @property
@@ -251,6 +255,8 @@ class NeuralODE:
         if h is None:
             h = Handle(self.model.widths, self.model.acts, self.model.input_map,
                        self.precision, B)
+            if self.wgrad_precision is not None:
+                h.set_option("wgrad_precision", self.wgrad_precision)
             self._handles[B] = h
         return h
 

@@ -154,6 +154,38 @@ int32_t node_rhs(node_handle_t* h, const float* u, float* du);
 int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda,
                  float* du, double* dp);
 
+/* ---- Per-trajectory ("ragged") solves ----------------------------------
+ * n independent trajectories of the handle's model, each its own ODE problem
+ * -- what n separate NeuralODE solves of batch 1 (src/neural_de.jl:47-61, one
+ * call per trajectory) compute: own tspan end t1[i], own save times, own error
+ * norm (RMS over its own D entries), own initial dt and PI-controlled steps --
+ * in one persistent launch (no counterpart in the reference, which has no
+ * batched form of per-trajectory solves).
+ *  opts     : t0 (shared start), tolerances, adaptive / dt / dtmax / dtmin /
+ *             maxiters as for node_forward; t1 and the save flags are ignored.
+ *  t1       : [n] tspan ends (> t0).
+ *  u0       : D x n.
+ *  save_offsets : [n + 1], save_offsets[0] = 0; trajectory i saves at
+ *             save_times[save_offsets[i] .. save_offsets[i+1]) (increasing,
+ *             within [t0, t1[i]]; t0 and t1[i] are saved only when listed, the
+ *             saveat-vector rule of OrdinaryDiffEq).
+ *  out      : D x save_offsets[n] saved states.
+ *  retcodes : [n] NODE_RET_* per trajectory.  traj_stats (optional): [n][4]
+ *             RHS evaluations, accepted steps, rejected steps, tape records.
+ * Host pointers.  Computes in FP32 (MFMA 16x16x4 FP32) whatever the handle's
+ * precision: at the tolerances these solves are run with, BF16 operands push
+ * the error estimate onto their rounding floor.  Model limits: D <= 32, hidden
+ * widths <= 128, at most 4 Dense layers. */
+typedef struct node_ragged_stats {
+    int64_t nf, naccept, nreject;   /* summed over trajectories */
+    int64_t records;                /* forward tape records (accepted steps + 1 each) */
+    int64_t n_success;              /* trajectories with retcode Success */
+} node_ragged_stats;
+int32_t node_ragged_forward(node_handle_t* h, const node_solve_opts* opts, int64_t n,
+                            const double* t1, const float* u0, const int64_t* save_offsets,
+                            const double* save_times, float* out, int32_t* retcodes,
+                            int32_t* traj_stats, node_ragged_stats* stats);
+
 /* Execution options (no counterpart in the reference):
  *   NODE_OPT_GRAPHS          1 (default): step attempts are captured once into a
  *                            HIP graph and replayed; 0: launched one by one.
@@ -162,7 +194,14 @@ int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda,
  *                            valid tile configuration (and split-K factor of the
  *                            weight gradients) of each GEMM role on its own
  *                            shapes and keeps the fastest; 0: fixed heuristic. */
-enum { NODE_OPT_GRAPHS = 1, NODE_OPT_GRAPH_ATTEMPTS = 2, NODE_OPT_AUTOTUNE = 3 };
+/*   NODE_OPT_WGRAD_PRECISION NODE_PREC_* of the adjoint's weight-gradient products
+ *                            (lambda^T df/dp, a third of the adjoint's GEMM work).
+ *                            Default: the handle's precision.  A BF16X3 handle
+ *                            accepts NODE_PREC_BF16: one MFMA pass instead of three
+ *                            on those products; the data path (state, adjoint state,
+ *                            and so the step sizes) keeps BF16X3. */
+enum { NODE_OPT_GRAPHS = 1, NODE_OPT_GRAPH_ATTEMPTS = 2, NODE_OPT_AUTOTUNE = 3,
+       NODE_OPT_WGRAD_PRECISION = 4 };
 int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value);
 /* Tile plan chosen for a GEMM role (roles as for node_prof_enable below) and
  * layer: tile rows x cols and split-K factor. */

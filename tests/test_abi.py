@@ -45,13 +45,16 @@ def test_struct_layouts_match_c(tmp_path):
         '#include <stdio.h>\n#include <stddef.h>\n#include "node_hip.h"\n'
         "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\", sizeof(node_desc),"
         " offsetof(node_desc, batch), sizeof(node_solve_opts), offsetof(node_solve_opts, maxiters),"
-        " sizeof(node_stats), offsetof(node_stats, t_final)); return 0;}\n")
+        " sizeof(node_stats), offsetof(node_stats, t_final));"
+        " printf(\"%zu %zu\\n\", sizeof(node_ragged_stats), offsetof(node_ragged_stats, n_success));"
+        " return 0;}\n")
     exe = tmp_path / "probe"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(probe), "-o", str(exe)])
     got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
     want = [ctypes.sizeof(_lib.NodeDesc), _lib.NodeDesc.batch.offset,
             ctypes.sizeof(_lib.SolveOpts), _lib.SolveOpts.maxiters.offset,
-            ctypes.sizeof(_lib.Stats), _lib.Stats.t_final.offset]
+            ctypes.sizeof(_lib.Stats), _lib.Stats.t_final.offset,
+            ctypes.sizeof(_lib.RaggedStats), _lib.RaggedStats.n_success.offset]
     assert got == want
 
 

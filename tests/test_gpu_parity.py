@@ -151,6 +151,44 @@ def test_adaptive_forward_and_adjoint(name, tol, prec):
     assert eu < 50 * tol + 1e-5 and ep < 50 * tol + 1e-5
 
 
+def test_bf16_weight_gradients_on_split_handle():
+    """BF16X3 data path with BF16 weight-gradient products: same forward, the
+    adjoint takes the BF16X3 step sequence (within a few steps), gradients
+    within 1e-3 of the FP64 oracle; invalid combinations are refused."""
+    from diffeqflux_hip import Handle, NodeHipError
+    m, p, u0, h, (t0, t1) = setup_case("mixed", "bf16x3")
+    tol = 1e-4
+    ts_req = list(np.linspace(t0, t1, 9))
+    opts = Handle.make_opts(t0, t1, abstol=1e-6, reltol=tol)
+    c = None
+    res = {}
+    for wg in ("bf16x3", "bf16"):
+        h.set_option("wgrad_precision", wg)
+        out, ts, st = h.forward(u0, opts, ts_req[1:-1])
+        if c is None:
+            c = _cotangents(out.shape)
+        du0, dp, sb = h.backward(c)
+        res[wg] = (out, du0, dp, st, sb)
+    ref = OA.forward(m, p, u0.astype(np.float64), t0, t1, saveat=ts_req, abstol=1e-6, reltol=tol)
+    gu0, gp, adj = OA.backward(m, p, ref, [ci.astype(np.float64) for ci in c],
+                               abstol=1e-6, reltol=tol)
+    o3, _, _, st3, sb3 = res["bf16x3"]
+    o1, du1, dp1, st1, sb1 = res["bf16"]
+    print(f"  adjoint steps bf16x3 {sb3['naccept']}+{sb3['nreject']} wgrad-bf16 "
+          f"{sb1['naccept']}+{sb1['nreject']}; grad rel err du0 {rel(du1, gu0):.2e} "
+          f"dp {rel(dp1, gp):.2e} (l2 {np.linalg.norm(dp1 - gp) / np.linalg.norm(gp):.2e})",
+          flush=True)
+    assert np.array_equal(o1, o3) and st1["naccept"] == st3["naccept"]
+    assert sb1["retcode"] == 1 and abs(sb1["naccept"] - sb3["naccept"]) <= 2
+    assert np.linalg.norm(dp1 - gp) / np.linalg.norm(gp) < 1e-3
+    assert rel(du1, gu0) < 1e-3
+    with pytest.raises(NodeHipError):
+        h.set_option("wgrad_precision", "fp32")
+    hf = Handle([2, 50, 2], ["tanh", "identity"], MAP_CUBE, "fp32", 1)
+    with pytest.raises(NodeHipError):
+        hf.set_option("wgrad_precision", "bf16")
+
+
 REF_KWARGS = [  # test/BasicNeuralDE/neural_de_tests.jl:27-42
     dict(save_everystep=False, save_start=False),
     dict(abstol=1e-12, reltol=1e-12, save_everystep=False, save_start=False),

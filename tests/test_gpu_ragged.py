@@ -1,0 +1,107 @@
+"""GPU parity of the per-trajectory ("ragged") solves: every trajectory of one
+launch against the oracle solving that trajectory alone (batch 1), on the
+same FP32 parameters.
+
+Tolerances: saved states within 20 * (abstol + reltol*|u|) of the FP64 oracle
+at every save time (the FP32 error estimate may take a different, equally
+valid step sequence); accepted-step counts within max(2, n/5) of the oracle
+run in FP32 (the reference's own precision).
+"""
+import numpy as np
+import pytest
+
+from oracle import adjoint as OA
+from oracle.mlp import MLP, MAP_CUBE
+
+pytestmark = pytest.mark.gpu
+
+
+def ragged_case(widths, acts, imap, n, seed, tmax=3.0, nsave=(3, 12)):
+    m = MLP(widths, acts, input_map=imap)
+    rng = np.random.default_rng(seed)
+    p32 = m.glorot_init(rng).astype(np.float32)
+    p32[m.offsets[0][1]:m.offsets[0][2]] = rng.uniform(-0.1, 0.1, widths[1])
+    u0 = rng.standard_normal((n, widths[0])).astype(np.float32)
+    if imap == MAP_CUBE:
+        u0 *= 0.7
+    t1 = rng.uniform(0.1, tmax, n)
+    saves = []
+    for i in range(n):
+        k = int(rng.integers(nsave[0], nsave[1] + 1))
+        s = np.linspace(0.0, t1[i], k)
+        if i % 3 == 1:
+            s = s[1:]          # t0 not saved
+        if i % 4 == 2:
+            s = s[:-1]         # t1 not saved
+        saves.append(s)
+    return m, p32, u0, t1, saves
+
+
+@pytest.mark.parametrize("case", ["config4", "spiral", "identity_last2"])
+def test_ragged_forward_matches_per_trajectory_oracle(case):
+    from diffeqflux_hip import Handle
+    if case == "config4":
+        widths, acts, imap, n, tol = [16, 128, 128, 16], ["tanh", "tanh", "identity"], 0, 24, 1e-5
+    elif case == "spiral":
+        widths, acts, imap, n, tol = [2, 50, 2], ["tanh", "identity"], MAP_CUBE, 20, 1e-6
+    else:
+        widths, acts, imap, n, tol = [32, 64, 32], ["softplus", "identity"], 0, 17, 1e-4
+    m, p32, u0, t1, saves = ragged_case(widths, acts, imap, n, seed=3)
+    h = Handle(widths, acts, imap, "fp32", 1)
+    h.set_params(p32)
+    outs, ret, ts, tot = h.ragged_forward(u0, t1, saves, abstol=tol, reltol=tol)
+    print(f"\n[ragged {case} n={n}] totals {tot}", flush=True)
+    assert np.all(ret == 1), ret
+    p = p32.astype(np.float64)
+    worst = 0.0
+    for i in range(n):
+        s = list(saves[i])
+        ref = OA.forward(m, p, u0[i:i + 1].astype(np.float64), 0.0, float(t1[i]), saveat=s,
+                         abstol=tol, reltol=tol)
+        ref32 = OA.forward(m, p32, u0[i:i + 1], 0.0, float(t1[i]), saveat=s, abstol=tol,
+                           reltol=tol)
+        R = np.array(ref.u)[:, 0, :]
+        assert outs[i].shape == R.shape, (i, outs[i].shape, R.shape)
+        err = np.abs(outs[i] - R) / (tol + tol * np.abs(R))
+        worst = max(worst, float(err.max()))
+        assert err.max() < 20, (i, err.max())
+        slack = max(2, ref32.naccept // 5)
+        assert abs(int(ts[i, 1]) - ref32.naccept) <= slack, (i, ts[i], ref32.naccept)
+        assert ts[i, 3] == ts[i, 1] + 1
+    print(f"  worst scaled err {worst:.3f}; steps gpu {ts[:6, 1].tolist()} ...", flush=True)
+    assert tot["naccept"] == int(ts[:, 1].sum()) and tot["n_success"] == n
+
+
+def test_ragged_fixed_dt_matches_oracle_closely():
+    """Fixed dt: same step sequence as the oracle, so FP32 rounding only."""
+    from diffeqflux_hip import Handle
+    widths, acts = [16, 128, 128, 16], ["tanh", "tanh", "identity"]
+    m, p32, u0, t1, saves = ragged_case(widths, acts, 0, 9, seed=5)
+    t1 = np.round(t1 * 8) / 8 + 0.25          # multiples of dt
+    saves = [np.linspace(0.0, t, 5) for t in t1]
+    h = Handle(widths, acts, 0, "fp32", 1)
+    h.set_params(p32)
+    outs, ret, ts, _ = h.ragged_forward(u0, t1, saves, adaptive=False, dt=1 / 32)
+    assert np.all(ret == 1)
+    p = p32.astype(np.float64)
+    for i in range(len(t1)):
+        ref = OA.forward(m, p, u0[i:i + 1].astype(np.float64), 0.0, float(t1[i]),
+                         saveat=list(saves[i]), adaptive=False, dt=1 / 32)
+        R = np.array(ref.u)[:, 0, :]
+        e = np.abs(outs[i] - R).max() / np.abs(R).max()
+        assert e < 2e-5, (i, e)
+        assert ts[i, 1] == ref.naccept
+
+
+def test_ragged_rejects_unsupported_and_invalid():
+    from diffeqflux_hip import Handle, NodeHipError
+    h = Handle([64, 256, 64], ["tanh", "identity"], 0, "fp32", 1)
+    h.set_params(np.zeros(h.P, np.float32))
+    with pytest.raises(NodeHipError, match="D <= 32"):
+        h.ragged_forward(np.zeros((2, 64)), [1.0, 1.0], [[0.5], [0.5]])
+    h2 = Handle([16, 32, 16], ["tanh", "identity"], 0, "fp32", 1)
+    h2.set_params(np.zeros(h2.P, np.float32))
+    with pytest.raises(NodeHipError, match="t1 > t0"):
+        h2.ragged_forward(np.zeros((2, 16)), [1.0, 0.0], [[0.5], [0.0]])
+    with pytest.raises(NodeHipError, match="increase"):
+        h2.ragged_forward(np.zeros((1, 16)), [1.0], [[0.5, 0.2]])
