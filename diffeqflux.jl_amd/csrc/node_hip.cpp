@@ -143,6 +143,14 @@ struct node_handle {
     std::vector<int> h_ret;
   } rg;
   int n_cu = 0;
+  struct RaggedAdj {
+    float* dLdout = nullptr; int64_t dLdout_cap = 0;
+    float* dLdu0 = nullptr; int64_t dLdu0_cap = 0;
+    float* rows = nullptr; int64_t rows_cap = 0;
+    float* tmp = nullptr; int64_t tmp_cap = 0;
+    float* sbuf = nullptr; int64_t sbuf_cap = 0;
+    double* dp = nullptr; int64_t dp_cap = 0;
+  } ra;
 
   // profiling (node_prof_enable)
   uint64_t prof_mask = 0;
@@ -934,6 +942,14 @@ int ragged_model(node_handle* h, RaggedModel& m, std::string& why) {
   }
   m.lds_floats = off;
   if ((size_t)off * sizeof(float) > 160 * 1024) { why = "model weights exceed the 160 KB LDS of a workgroup"; return -1; }
+  int so = 0;
+  for (int l = 0; l < m.L; ++l) {
+    m.sh[l] = so;
+    so += 16 * m.ti[l];
+    m.sd[l] = so;
+    so += 16 * m.to[l];
+  }
+  m.sv = so;
   m.P = h->P;
   m.p = h->pflat;
   return 0;
@@ -946,6 +962,10 @@ int grow(node_handle* h, P** p, int64_t need, int64_t& cap_elems, int64_t new_ca
   *p = nullptr;
   cap_elems = new_cap;
   return dalloc_t(h, p, (size_t)new_cap * sizeof(P));
+}
+
+int ragged_grid(node_handle* h, int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(h->n_cu, (n + 63) / 64));
 }
 
 int ragged_forward_impl(node_handle* h, const node_solve_opts* o, int64_t n, const double* t1,
@@ -1026,7 +1046,7 @@ int ragged_forward_impl(node_handle* h, const node_solve_opts* o, int64_t n, con
   a.maxiters = o->maxiters > 0 ? o->maxiters : 100000;
   a.adaptive = o->adaptive ? 1 : 0;
   a.stats = r.stats;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(h->n_cu, (n + 63) / 64));
+  const int grid = ragged_grid(h, n);
   int ctr[4];
   for (int attempt = 0; attempt < 3; ++attempt) {
     a.rec_t = r.rec_t; a.rec_prev = r.rec_prev; a.rec_data = r.rec_data; a.rec_cap = r.rec_cap;
@@ -1067,6 +1087,78 @@ int ragged_forward_impl(node_handle* h, const node_solve_opts* o, int64_t n, con
   r.ns = ns;
   r.opts = *o;
   r.have = true;
+  return 0;
+}
+
+int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, double* dLdp,
+                         float* dLdp_traj, int32_t* ret, int32_t* tstats,
+                         node_ragged_stats* stats) {
+  auto& r = h->rg;
+  auto& b = h->ra;
+  if (!r.have) return fail(h, NODE_ERR_STATE, "no per-trajectory forward solve to differentiate");
+  for (int v : r.h_ret)
+    if (v != NODE_RET_SUCCESS)
+      return fail(h, NODE_ERR_STATE, "a trajectory's forward solve did not succeed (retcode != Success)");
+  if (r.ns > 0 && !dLdout) return fail(h, NODE_ERR_INVALID, "dLdout is NULL");
+  RaggedModel m;
+  std::string why;
+  if (ragged_model(h, m, why)) return fail(h, NODE_ERR_UNSUPPORTED, why);
+  const int64_t n = r.n, ns = r.ns, P = h->P;
+  const int D = h->D, Dp = (D + 15) / 16 * 16;
+  const int grid = ragged_grid(h, n);
+  const int64_t slots = (int64_t)grid * 4 * 16;
+  if (grow(h, &b.dLdout, std::max<int64_t>(ns * D, 1), b.dLdout_cap, std::max<int64_t>(ns * D, 64)) ||
+      grow(h, &b.dLdu0, n * D, b.dLdu0_cap, n * D) ||
+      grow(h, &b.rows, n * P, b.rows_cap, n * P) ||
+      grow(h, &b.tmp, slots * P, b.tmp_cap, slots * P) ||
+      grow(h, &b.sbuf, slots * 8 * m.sv, b.sbuf_cap, slots * 8 * m.sv) ||
+      grow(h, &b.dp, P, b.dp_cap, P))
+    return NODE_ERR_HIP;
+  if (ns > 0)
+    HIPCHK(h, hipMemcpyAsync(b.dLdout, dLdout, ns * D * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(b.rows, 0, (size_t)n * P * sizeof(float), h->stream));
+  HIPCHK(h, hipMemsetAsync(r.ctr, 0, 4 * sizeof(int), h->stream));
+  HIPCHK(h, hipMemsetAsync(r.stats, 0, 4 * sizeof(unsigned long long), h->stream));
+  RaggedAdjArgs a{};
+  a.m = m;
+  a.D = D;
+  a.Dp = Dp;
+  a.n = n;
+  a.t1 = r.t1; a.save_off = r.soff; a.save_t = r.st; a.dLdout = b.dLdout;
+  a.last_rec = r.last; a.rec_t = r.rec_t; a.rec_prev = r.rec_prev; a.rec_data = r.rec_data;
+  a.dLdu0 = b.dLdu0; a.dLdp_traj = b.rows; a.mu_tmp = b.tmp; a.sbuf = b.sbuf;
+  a.queue = r.ctr; a.retcode = r.ret; a.traj_stats = r.tstats;
+  const node_solve_opts* o = &r.opts;
+  a.t0 = o->t0;
+  a.abstol = o->abstol > 0 ? o->abstol : 1e-6;
+  a.reltol = o->reltol > 0 ? o->reltol : 1e-3;
+  a.dtmax = o->dtmax;
+  a.dtmin = o->dtmin;
+  a.dt_init = std::fabs(o->dt);
+  a.maxiters = o->maxiters > 0 ? o->maxiters : 100000;
+  a.adaptive = o->adaptive ? 1 : 0;
+  a.stats = r.stats;
+  launch_ragged_adj(a, grid, h->stream);
+  HIPCHK(h, hipGetLastError());
+  launch_ragged_sum(b.rows, n, P, b.dp, h->stream);
+  HIPCHK(h, hipGetLastError());
+  std::vector<int> hret(n);
+  unsigned long long hs[4];
+  HIPCHK(h, hipMemcpyAsync(hret.data(), r.ret, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(hs, r.stats, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+  if (dLdu0) HIPCHK(h, hipMemcpyAsync(dLdu0, b.dLdu0, n * D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (dLdp) HIPCHK(h, hipMemcpyAsync(dLdp, b.dp, P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (dLdp_traj)
+    HIPCHK(h, hipMemcpyAsync(dLdp_traj, b.rows, (size_t)n * P * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (tstats) HIPCHK(h, hipMemcpyAsync(tstats, r.tstats, 4 * n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (ret) std::memcpy(ret, hret.data(), n * sizeof(int));
+  if (stats) {
+    node_ragged_stats s{};
+    s.nf = (int64_t)hs[0]; s.naccept = (int64_t)hs[1]; s.nreject = (int64_t)hs[2];
+    for (int v : hret) s.n_success += v == NODE_RET_SUCCESS;
+    *stats = s;
+  }
   return 0;
 }
 
@@ -1363,6 +1455,14 @@ int32_t node_ragged_forward(node_handle_t* h, const node_solve_opts* opts, int64
   std::lock_guard<std::mutex> g(h->mtx);
   return ragged_forward_impl(h, opts, n, t1, u0, save_offsets, save_times, out, retcodes,
                              traj_stats, stats);
+}
+
+int32_t node_ragged_backward(node_handle_t* h, const float* dLdout, float* dLdu0, double* dLdp,
+                             float* dLdp_traj, int32_t* retcodes, int32_t* traj_stats,
+                             node_ragged_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return ragged_backward_impl(h, dLdout, dLdu0, dLdp, dLdp_traj, retcodes, traj_stats, stats);
 }
 
 int32_t node_rhs(node_handle_t* h, const float* u, float* du) {

@@ -214,6 +214,8 @@ struct RaggedModel {
   int lw[RG_MAXL], lb[RG_MAXL];       // LDS offsets (floats) of W_l rows and bias
   int64_t w0[RG_MAXL], b0[RG_MAXL];   // flat parameter offsets
   int lds_floats;
+  int sh[RG_MAXL], sd[RG_MAXL];       // adjoint stage block: offsets of h_l (layer input) and delta_l
+  int sv;                             // floats per stage block
   int64_t P;
   const float* p;                     // flat FP32 parameters (device)
 };
@@ -241,5 +243,31 @@ struct RaggedFwdArgs {
   unsigned long long* stats;          // nf, naccept, nreject over all trajectories
 };
 void launch_ragged_fwd(const RaggedFwdArgs& a, int grid, hipStream_t s);
+struct RaggedAdjArgs {
+  RaggedModel m;
+  int D, Dp;
+  int64_t n;
+  const double* t1;
+  const int64_t* save_off;
+  const double* save_t;
+  const float* dLdout;                // [save_off[n]][D]
+  const int* last_rec;
+  const double* rec_t;
+  const int* rec_prev;
+  const float* rec_data;
+  float* dLdu0;                       // [n][D]
+  float* dLdp_traj;                   // [n][P] per-trajectory parameter gradients (zeroed)
+  float* mu_tmp;                      // [slots][P]
+  float* sbuf;                        // [slots][8][sv] stage blocks
+  int* queue;
+  int* retcode;                       // [n]
+  int* traj_stats;                    // [n][4]
+  double t0, abstol, reltol, dtmax, dtmin, dt_init;
+  int64_t maxiters;
+  int adaptive;
+  unsigned long long* stats;
+};
+void launch_ragged_adj(const RaggedAdjArgs& a, int grid, hipStream_t s);
+void launch_ragged_sum(const float* rows, int64_t n, int64_t P, double* out, hipStream_t s);
 
 }  // namespace nodehip

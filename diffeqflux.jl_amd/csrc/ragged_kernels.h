@@ -472,3 +472,562 @@ void launch_ragged_fwd(const RaggedFwdArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_ragged_fwd<2>, dim3(grid), dim3(256), lds, s, a);
   }
 }
+
+// ---------------------------------------------------------------- adjoint
+// One VJP for the 16 slot columns: x0 = map(u) (first ti[0] tiles), lam (first
+// to[L-1] tiles).  Stores the layer inputs h_l and the pre-activation
+// gradients delta_l of this evaluation in the slot's stage block `blk`
+// (act' is parked there during the forward sweep) and returns
+// g = W_0^T delta_0 (the input-map derivative is applied by the caller).
+__device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
+                                       const f32x4 (&x0)[RG_HT], const f32x4 (&lam)[RG_HT],
+                                       float* blk, f32x4 (&gout)[RG_HT], int lane) {
+  const int r = lane & 15, g4 = 4 * (lane >> 4);
+  const int L = m.L;
+  f32x4 ha[RG_HT], hb[RG_HT];
+#pragma unroll
+  for (int t = 0; t < RG_HT; ++t) {
+    ha[t] = x0[t];
+    hb[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (t < m.ti[0]) *(f32x4*)(blk + m.sh[0] + 16 * t + g4) = x0[t];
+  }
+#pragma unroll
+  for (int l = 0; l < RG_MAXL; ++l) {
+    if (l < L) {
+      f32x4 (&in)[RG_HT] = (l & 1) ? hb : ha;
+      f32x4 (&out)[RG_HT] = (l & 1) ? ha : hb;
+      const float* W = lds + m.lw[l];
+      const float* bias = lds + m.lb[l];
+      const int pitch = m.pitch[l], nti = m.ti[l], nto = m.to[l], act = m.act[l];
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        if (ot < nto) {
+          f32x4 acc = *(const f32x4*)(bias + 16 * ot + g4);
+          const float* wrow = W + (16 * ot + r) * pitch + g4;
+#pragma unroll
+          for (int it = 0; it < RG_HT; ++it) {
+            if (it < nti) {
+              const f32x4 a = *(const f32x4*)(wrow + 16 * it);
+              acc = rg_mfma(a.x, in[it].x, acc);
+              acc = rg_mfma(a.y, in[it].y, acc);
+              acc = rg_mfma(a.z, in[it].z, acc);
+              acc = rg_mfma(a.w, in[it].w, acc);
+            }
+          }
+          f32x4 y, sd;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            y[v] = act_f(act, acc[v]);
+            sd[v] = act_d(act, acc[v], y[v]);
+          }
+          *(f32x4*)(blk + m.sd[l] + 16 * ot + g4) = sd;
+          if (l < L - 1) *(f32x4*)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
+          out[ot] = y;
+        }
+      }
+    }
+  }
+  // backward sweep: g entering layer l lives in (l & 1) ? gb : ga
+  f32x4 ga[RG_HT], gb[RG_HT];
+  const bool odd = ((L - 1) & 1) != 0;
+#pragma unroll
+  for (int t = 0; t < RG_HT; ++t) {
+    ga[t] = odd ? f32x4{0.f, 0.f, 0.f, 0.f} : lam[t];
+    gb[t] = odd ? lam[t] : f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+#pragma unroll
+  for (int l = RG_MAXL - 1; l >= 0; --l) {
+    if (l < L) {
+      f32x4 (&gi)[RG_HT] = (l & 1) ? gb : ga;
+      f32x4 (&go)[RG_HT] = (l & 1) ? ga : gb;
+      const int nti = m.ti[l], nto = m.to[l], pitch = m.pitch[l];
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        if (ot < nto) {
+          float* pd = blk + m.sd[l] + 16 * ot + g4;
+          const f32x4 d = gi[ot] * *(const f32x4*)pd;
+          *(f32x4*)pd = d;
+          gi[ot] = d;
+        }
+      }
+      const float* W = lds + m.lw[l];
+#pragma unroll
+      for (int it = 0; it < RG_HT; ++it) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        if (it < nti) {
+#pragma unroll
+          for (int ot = 0; ot < RG_HT; ++ot) {
+            if (ot < nto) {
+              const float* wc = W + (16 * ot + g4) * pitch + 16 * it + r;
+              acc = rg_mfma(wc[0], gi[ot].x, acc);
+              acc = rg_mfma(wc[pitch], gi[ot].y, acc);
+              acc = rg_mfma(wc[2 * pitch], gi[ot].z, acc);
+              acc = rg_mfma(wc[3 * pitch], gi[ot].w, acc);
+            }
+          }
+        }
+        go[it] = acc;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < RG_HT; ++t) gout[t] = gb[t];
+}
+
+// Parameter part of one trajectory's adjoint state, run by the whole wave for
+// slot block `sbs` (8 stage blocks of m.sv floats; logical stage j of the
+// attempt sits in block j, except j = 0 in block p0 and j = 6 in block 7-p0):
+//  MODE 0: stage sums kb = sum_j b_j g_j, ke = sum_j btilde_j g_j with
+//          g_j = -(delta_j h_j^T) (weights) and -delta_j (bias); writes
+//          mu_new = mu + dt kb; returns sum (dt ke / (atol + max|mu|,|mu_new| rtol))^2
+//  MODE 1: returns sum (g_0 / (atol + |mu| rtol))^2           (initial dt, d1)
+//  MODE 2: returns sum ((g_1 - g_0) / (atol + |mu| rtol))^2   (initial dt, d2;
+//          g_1 in block 7 - p0), *flag = 1 where g_1 != g_0
+template <int MODE>
+__device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, double dt,
+                             double atol, double rtol, const float* mu, float* mu_new,
+                             double* flag, int lane) {
+  constexpr int NJ = MODE == 0 ? 7 : (MODE == 1 ? 1 : 2);
+  int blkj[NJ];
+  float cb[NJ], ce[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    if (MODE == 0) {
+      blkj[j] = j == 0 ? p0 : (j == 6 ? 7 - p0 : j);
+      cb[j] = j < 6 ? cA[6][j] : 0.f;
+      ce[j] = cBT[j];
+    } else {
+      blkj[j] = j == 0 ? p0 : 7 - p0;
+      cb[j] = j == 0 ? 1.f : 0.f;   // g_0
+      ce[j] = j == 0 ? -1.f : 1.f;  // g_1 - g_0
+    }
+  }
+  double sum = 0.0, fl = 0.0;
+  for (int l = 0; l < m.L; ++l) {
+    const int in = m.in[l], out = m.out[l];
+    for (int o0 = 0; o0 < out; o0 += 64) {
+      const int o = o0 + lane;
+      const bool ok = o < out;
+      float bd[NJ], ed[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const float d = ok ? sbs[blkj[j] * m.sv + m.sd[l] + o] : 0.f;
+        bd[j] = cb[j] * d;
+        ed[j] = ce[j] * d;
+      }
+      // bias entry (h = 1), then the weight column entries W[o, i]
+      for (int i = -1; i < in; ++i) {
+        float kb = 0.f, ke = 0.f;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const float hj = i < 0 ? 1.f : sbs[blkj[j] * m.sv + m.sh[l] + i];
+          kb += bd[j] * hj;
+          ke += ed[j] * hj;
+        }
+        if (!ok) continue;
+        const int64_t idx = i < 0 ? m.b0[l] + o : m.w0[l] + (int64_t)i * out + o;
+        const float gb = -kb, ge = -ke;
+        const float m0 = mu[idx];
+        double e;
+        if (MODE == 0) {
+          const float mn = (float)((double)m0 + dt * (double)gb);
+          mu_new[idx] = mn;
+          e = dt * (double)ge / (atol + fmax(fabs((double)m0), fabs((double)mn)) * rtol);
+        } else if (MODE == 1) {
+          e = (double)gb / (atol + fabs((double)m0) * rtol);
+        } else {
+          e = (double)ge / (atol + fabs((double)m0) * rtol);
+          if (ge != 0.f) fl = 1.0;
+        }
+        sum += e * e;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += __shfl_xor(sum, off);
+    fl = fmax(fl, __shfl_xor(fl, off));
+  }
+  if (flag) *flag = fl;
+  return sum;
+}
+
+// u(tau) of the slot's trajectory from its forward tape: the stored state when
+// tau is a step time, else the step's free interpolant (dt = t_{j+1} - t_j);
+// (cj, tcj, tnx): a record at or before tau's step and the time after it
+template <int DT>
+__device__ __forceinline__ void rg_tape_u(const RaggedAdjArgs& a, int cj, double tcj, double tnx,
+                                          double tau, f32x4 (&uu)[DT], int lane) {
+  const int g4 = 4 * (lane >> 4);
+  int j = cj;
+  double tj = tcj, tn = tnx;
+  while (tj > tau && a.rec_prev[j] >= 0) {
+    tn = tj;
+    j = a.rec_prev[j];
+    tj = a.rec_t[j];
+  }
+  const float* rd = a.rec_data + (int64_t)j * 8 * a.Dp;
+  if (tj == tau || !(tn > tj)) {
+#pragma unroll
+    for (int t = 0; t < DT; ++t) uu[t] = *(const f32x4*)(rd + 16 * t + g4);
+    return;
+  }
+  const double dts = tn - tj;
+  float w[7];
+  interp_weights((tau - tj) / dts, w);
+#pragma unroll
+  for (int t = 0; t < DT; ++t) {
+    f32x4 sm = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 7; ++i) sm += w[i] * *(const f32x4*)(rd + (1 + i) * a.Dp + 16 * t + g4);
+    uu[t] = *(const f32x4*)(rd + 16 * t + g4) + (float)dts * sm;
+  }
+}
+
+template <int DT>
+__global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
+  extern __shared__ __attribute__((aligned(16))) float rg_lds[];
+  const RaggedModel& m = a.m;
+  rg_load_weights(m, rg_lds);
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const int64_t wslot0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RG_SLOTS;
+  float* sb = a.sbuf + (wslot0 + r) * 8 * m.sv;
+  const int D = a.D;
+  const int64_t P = m.P;
+  const double nel = (double)D + (double)P;
+  const float atol = (float)a.abstol, rtol = (float)a.reltol;
+
+  int traj = -1, phase = 0, stage = 0, p0 = 0, pad = 0, fsal_re = 0, muflip = 0, cj = 0;
+  bool need_start = true;
+  SlotCtl c{};
+  f32x4 lam[DT], lnew[DT], kl[7][DT], uc[DT];
+#pragma unroll
+  for (int t = 0; t < DT; ++t) {
+    lam[t] = lnew[t] = uc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 7; ++j) kl[j][t] = lam[t];
+  }
+  int64_t s_lo = 0, snext = -1, nf = 0, nacc = 0, nrej = 0;
+  double tcj = 0.0, tnx = 0.0, dt0 = 0.0, d0 = 0.0, d1 = 0.0, s1l = 0.0;
+
+  for (;;) {
+    if (need_start) {
+      need_start = false;
+      int next = 0;
+      if (g == 0) next = atomicAdd(a.queue, 1);
+      next = rg_bcast(next, lane);
+      traj = next < a.n ? next : -1;
+      if (traj >= 0) {
+        c = SlotCtl{};
+        c.t = a.t1[traj];
+        c.t_end = a.t0;
+        c.tdir = -1.0;
+        c.dtmax_abs = a.dtmax > 0 ? a.dtmax : fabs(c.t - c.t_end);
+        c.dtmin = a.dtmin > 0 ? a.dtmin : fmax(dspacing(c.t), dspacing(c.t_end));
+        c.maxiters = a.maxiters;
+        c.adaptive = a.adaptive;
+        c.dt_fixed = a.dt_init;
+        c.qold = QOLDINIT;
+        c.q11 = 1.0;
+        s_lo = a.save_off[traj];
+        snext = a.save_off[traj + 1] - 1;
+        int64_t jump = -1;
+        if (snext >= s_lo && a.save_t[snext] == c.t) { jump = snext; --snext; }
+        double s0 = 0.0;
+#pragma unroll
+        for (int t = 0; t < DT; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int f = 16 * t + 4 * g + v;
+            const float l = (jump >= 0 && f < D) ? a.dLdout[jump * D + f] : 0.f;
+            lam[t][v] = l;
+            if (f < D) {
+              const double q = (double)l / (a.abstol + fabs((double)l) * a.reltol);
+              s0 += q * q;
+            }
+          }
+        s0 = rg_slot_sum(s0);
+        d0 = sqrt(s0 / nel);
+        cj = a.last_rec[traj];
+        tcj = a.rec_t[cj];
+        tnx = INFINITY;
+        p0 = 0;
+        muflip = 0;
+        fsal_re = 0;
+        phase = 0;
+        stage = 0;
+        nf = nacc = nrej = 0;
+      }
+    }
+    if (!__any(traj >= 0)) break;
+    // ---- VJP input of this round: stage time and adjoint state
+    double tau = c.t;
+    f32x4 ls[DT];
+    const float h = (float)c.dt;
+    int blkid = p0;
+#pragma unroll
+    for (int t = 0; t < DT; ++t) ls[t] = lam[t];
+    if (traj >= 0) {
+      if (phase == 1) {
+        tau = c.t + c.tdir * dt0;
+        blkid = 7 - p0;
+#pragma unroll
+        for (int t = 0; t < DT; ++t) ls[t] = lam[t] + (float)(c.tdir * dt0) * kl[0][t];
+      } else if (phase == 2) {
+        tau = c.t + dC[stage] * c.dt;
+        blkid = stage == 0 ? p0 : (stage == 6 ? 7 - p0 : stage);
+        if (stage > 0) {
+#pragma unroll
+          for (int t = 0; t < DT; ++t) ls[t] = lam[t] + h * rg_stage_sum(kl, stage, t);
+        }
+      }
+      rg_tape_u<DT>(a, cj, tcj, tnx, tau, uc, lane);
+    }
+    f32x4 x0[RG_HT], lg[RG_HT], gv[RG_HT];
+#pragma unroll
+    for (int t = 0; t < RG_HT; ++t) {
+      x0[t] = lg[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (t < DT && traj >= 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x0[t][q] = rg_map(m.map, uc[t][q]);
+        lg[t] = ls[t];
+      }
+    }
+    rg_vjp(m, rg_lds, x0, lg, sb + blkid * m.sv, gv, lane);
+    // k = -(lambda^T df/du): input-map derivative 3u^2 for x.^3
+    f32x4 kn[DT];
+#pragma unroll
+    for (int t = 0; t < DT; ++t) {
+      f32x4 v = gv[t];
+      if (m.map == 1) v *= 3.f * uc[t] * uc[t];
+      kn[t] = -v;
+    }
+    // ---- consume, part 1 (adjoint-state terms); mu-pass requests
+    int mode = -1;   // mu pass: 0 step, 1 d1, 2 d2
+    double sl = 0.0, fll = 0.0;
+    if (traj >= 0) {
+      ++nf;
+      if (phase == 0) {
+#pragma unroll
+        for (int t = 0; t < DT; ++t) {
+          kl[0][t] = kn[t];
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (16 * t + 4 * g + v < D) {
+              const float sk = atol + fabsf(lam[t][v]) * rtol;
+              const double q = (double)(kn[t][v] / sk);
+              sl += q * q;
+            }
+        }
+        mode = 1;
+      } else if (phase == 1) {
+#pragma unroll
+        for (int t = 0; t < DT; ++t) {
+          kl[6][t] = kn[t];
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (16 * t + 4 * g + v < D) {
+              const float sk = atol + fabsf(lam[t][v]) * rtol;
+              const double q = (double)((kn[t][v] - kl[0][t][v]) / sk);
+              sl += q * q;
+              if (kn[t][v] != kl[0][t][v]) fll = 1.0;
+            }
+        }
+        mode = 2;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 7; ++j)
+          if (j == stage)
+#pragma unroll
+            for (int t = 0; t < DT; ++t) kl[j][t] = kn[t];
+        if (stage == 6) {
+#pragma unroll
+          for (int t = 0; t < DT; ++t) {
+            lnew[t] = ls[t];
+            f32x4 ut = cBT[6] * kl[6][t];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) ut += cBT[j] * kl[j][t];
+            ut *= h;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              if (16 * t + 4 * g + v < D) {
+                const float sk = atol + fmaxf(fabsf(lam[t][v]), fabsf(lnew[t][v])) * rtol;
+                const double q = (double)(ut[v] / sk);
+                sl += q * q;
+                if (isnan(lnew[t][v])) fll = 1.0;
+              }
+          }
+          mode = 0;
+        }
+      }
+      sl = rg_slot_sum(sl);
+      fll = rg_slot_sum(fll);
+    }
+    // ---- parameter part, slot by slot with the whole wave
+    __threadfence_block();
+    double smu = 0.0, flmu = 0.0;
+    for (int s = 0; s < RG_SLOTS; ++s) {
+      const int ms = __shfl(mode, s);
+      if (ms < 0) continue;
+      const int trs = __shfl(traj, s);
+      const int p0s = __shfl(p0, s);
+      const int mfs = __shfl(muflip, s);
+      const double dts = __shfl(c.dt, s);
+      const float* sbs = a.sbuf + (wslot0 + s) * 8 * m.sv;
+      float* row = a.dLdp_traj + (int64_t)trs * P;
+      float* tmp = a.mu_tmp + (wslot0 + s) * P;
+      const float* mu = mfs ? tmp : row;
+      float* mun = mfs ? row : tmp;
+      double fl = 0.0, sm;
+      if (ms == 0) sm = rg_mu_pass<0>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, mun, nullptr, lane);
+      else if (ms == 1) sm = rg_mu_pass<1>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, nullptr, lane);
+      else sm = rg_mu_pass<2>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, &fl, lane);
+      if (r == s) { smu = sm; flmu = fl; }
+    }
+    if (traj < 0) continue;
+    // ---- consume, part 2: control
+    bool fin = false;
+    if (phase == 0) {
+      s1l = sl;
+      d1 = sqrt((sl + smu) / nel);
+      if (a.dt_init > 0.0) {
+        c.dt = c.tdir * a.dt_init;
+        c.dtpropose = c.dt;
+        c.last_accept = 0;
+        loopheader(&c, snext >= s_lo ? a.save_t[snext] : c.t_end);
+        phase = 2;
+        stage = 1;
+        fin = c.done;
+      } else {
+        const double dtmin = nextafter(fmax(c.dtmin, dspacing(c.t)), INFINITY);
+        const double smalldt = fmax(dtmin, 1e-6);
+        dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : 0.01 * (d0 / d1);
+        dt0 = fmin(dt0, c.dtmax_abs);
+        pad = 0;
+        if (!(dt0 >= 10 * 2.220446049250313e-16)) { dt0 = smalldt; pad = 1; }
+        phase = 1;
+      }
+    } else if (phase == 1) {
+      const double dtmin = nextafter(fmax(c.dtmin, dspacing(c.t)), INFINITY);
+      if (pad) {
+        c.dt = c.tdir * dt0;
+      } else if (fmax(fll, flmu) == 0.0) {
+        c.dt = c.tdir * fmax(dtmin, 100 * dt0);
+      } else {
+        const double d2 = sqrt((sl + smu) / nel) / dt0;
+        const double mx = fmax(d1, d2);
+        const double dt1 = mx <= 1e-15 ? fmax(1e-6, dt0 * 1e-3) : pow(10.0, -(2.0 + log10(mx)) / 5.0);
+        c.dt = c.tdir * fmax(dtmin, fmin(fmin(100 * dt0, dt1), c.dtmax_abs));
+      }
+      c.dtpropose = c.dt;
+      c.last_accept = 0;
+      loopheader(&c, snext >= s_lo ? a.save_t[snext] : c.t_end);
+      phase = 2;
+      stage = 1;
+      fin = c.done;
+    } else if (stage < 6) {
+      ++stage;
+    } else {
+      const double eest = c.adaptive ? sqrt((sl + smu) / nel) : 0.0;
+      fsal_re = 0;
+      if (c.adaptive && isnan(eest)) {
+        c.retcode = 3;
+        c.done = 1;
+      } else if (pi_control(&c, eest)) {
+        ++nacc;
+        const double tstop = snext >= s_lo ? a.save_t[snext] : c.t_end;
+        c.t = snap_t(c.t, c.t + c.dt, tstop);
+        muflip ^= 1;
+        p0 = 7 - p0;  // FSAL: the k7 block is the next attempt's k1 block
+#pragma unroll
+        for (int t = 0; t < DT; ++t) { lam[t] = lnew[t]; kl[0][t] = kl[6][t]; }
+        if (snext >= s_lo && c.t == a.save_t[snext]) {
+#pragma unroll
+          for (int t = 0; t < DT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int f = 16 * t + 4 * g + v;
+              if (f < D) lam[t][v] += a.dLdout[snext * D + f];
+            }
+          --snext;
+          fsal_re = 1;
+        }
+        // tape cursor back to the record holding the new time
+        while (tcj > c.t && a.rec_prev[cj] >= 0) {
+          tnx = tcj;
+          cj = a.rec_prev[cj];
+          tcj = a.rec_t[cj];
+        }
+        if (!c.adaptive) c.dtpropose = c.dt;
+        c.last_accept = 1;
+        if (fll != 0.0) { c.retcode = 6; c.done = 1; }
+        else if (c.t == c.t_end) { c.retcode = 1; c.done = 1; }
+        else c.dt = c.dtpropose;
+      } else {
+        ++nrej;
+        c.last_accept = 0;
+        c.dt = c.dt / fmin(1.0 / QMIN, c.q11 / GAMMA);
+      }
+      if (!c.done) loopheader(&c, snext >= s_lo ? a.save_t[snext] : c.t_end);
+      stage = fsal_re ? 0 : 1;
+      fin = c.done;
+    }
+    if (fin) {
+#pragma unroll
+      for (int t = 0; t < DT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int f = 16 * t + 4 * g + v;
+          if (f < D) a.dLdu0[(int64_t)traj * D + f] = lam[t][v];
+        }
+      if (g == 0) {
+        a.retcode[traj] = c.retcode;
+        int* ts = a.traj_stats + 4 * (int64_t)traj;
+        ts[0] = (int)nf; ts[1] = (int)nacc; ts[2] = (int)nrej; ts[3] = 0;
+        atomicAdd(a.stats + 0, (unsigned long long)nf);
+        atomicAdd(a.stats + 1, (unsigned long long)nacc);
+        atomicAdd(a.stats + 2, (unsigned long long)nrej);
+      }
+      need_start = true;
+    }
+    // the finished trajectory's parameter gradient must end in its own row
+    const int fin_any = __any(fin && muflip);
+    if (fin_any) {
+      for (int s = 0; s < RG_SLOTS; ++s) {
+        const int cp = __shfl((int)(fin && muflip), s);
+        if (!cp) continue;
+        const int trs = __shfl(traj, s);
+        const float* tmp = a.mu_tmp + (wslot0 + s) * P;
+        float* row = a.dLdp_traj + (int64_t)trs * P;
+        for (int64_t i = lane; i < P; i += 64) row[i] = tmp[i];
+      }
+    }
+  }
+}
+
+void launch_ragged_adj(const RaggedAdjArgs& a, int grid, hipStream_t s) {
+  const size_t lds = (size_t)a.m.lds_floats * sizeof(float);
+  if (a.Dp <= 16) {
+    (void)hipFuncSetAttribute((const void*)k_ragged_adj<1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_ragged_adj<1>, dim3(grid), dim3(256), lds, s, a);
+  } else {
+    (void)hipFuncSetAttribute((const void*)k_ragged_adj<2>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_ragged_adj<2>, dim3(grid), dim3(256), lds, s, a);
+  }
+}
+
+// dL/dp = sum over trajectories of their FP32 rows, in trajectory order, in FP64
+__global__ void k_ragged_sum(const float* rows, int64_t n, int64_t P, double* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int64_t k = 0; k < n; ++k) s += (double)rows[k * P + i];
+    out[i] = s;
+  }
+}
+void launch_ragged_sum(const float* rows, int64_t n, int64_t P, double* out, hipStream_t s) {
+  const int64_t nb = std::min<int64_t>((P + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_ragged_sum, dim3((int)nb), dim3(256), 0, s, rows, n, P, out);
+}

@@ -179,6 +179,29 @@ class Handle:
         self.ragged_offsets = off
         return [out[off[i]:off[i + 1]] for i in range(n)], ret, ts, rs.as_dict()
 
+    def ragged_backward(self, dLdout, per_trajectory=False):
+        """Adjoint of the last ragged_forward.  dLdout: list of (len_i, D)
+        cotangents (or one stacked (total, D) array).  Returns (dL/du0 (n, D),
+        dL/dp (P,) FP64, per-trajectory dL/dp (n, P) FP32 or None, retcodes,
+        per-trajectory stats (n, 4), totals)."""
+        if isinstance(dLdout, (list, tuple)):
+            dLdout = np.concatenate([np.asarray(x, np.float32).reshape(-1, self.D) for x in dLdout])
+        g = _f32(dLdout).reshape(-1, self.D)
+        if g.shape[0] == 0:
+            g = np.zeros((1, self.D), np.float32)
+        n = len(self.ragged_offsets) - 1
+        du0 = np.zeros((n, self.D), np.float32)
+        dp = np.zeros(self.P, np.float64)
+        rows = np.zeros((n, self.P), np.float32) if per_trajectory else None
+        ret = np.zeros(n, np.int32)
+        ts = np.zeros((n, 4), np.int32)
+        rs = _lib.RaggedStats()
+        self._check(self.lib.node_ragged_backward(self.h, _ptr(g), _ptr(du0), _ptr(dp),
+                                                  _ptr(rows) if rows is not None else None,
+                                                  _ptr(ret), _ptr(ts), ctypes.byref(rs)),
+                    "node_ragged_backward")
+        return du0, dp, rows, ret, ts, rs.as_dict()
+
     def rhs(self, u):
         u = _f32(u).reshape(self.B, self.D)
         du = np.empty_like(u)

@@ -186,6 +186,20 @@ int32_t node_ragged_forward(node_handle_t* h, const node_solve_opts* opts, int64
                             const double* save_times, float* out, int32_t* retcodes,
                             int32_t* traj_stats, node_ragged_stats* stats);
 
+/* InterpolatingAdjoint of every trajectory of the last node_ragged_forward
+ * (each its own adjoint problem [lambda; mu] with its own error norm over its
+ * D + P entries, steps landing on its save times with the cotangent jumps),
+ * in one persistent launch.  dLdout: D x save_offsets[n] cotangents of the
+ * saved states.  Writes dL/du0 (D x n), dL/dp summed over trajectories (P,
+ * FP64, summed in trajectory order) and optionally each trajectory's own
+ * dL/dp (P x n, FP32, the adjoint state mu of that trajectory).  All forward
+ * retcodes must be Success.  Host pointers.
+ * Replaces: per trajectory, the pullback of src/neural_de.jl:55-58 with
+ * sensealg InterpolatingAdjoint(ZygoteVJP()). */
+int32_t node_ragged_backward(node_handle_t* h, const float* dLdout, float* dLdu0, double* dLdp,
+                             float* dLdp_traj, int32_t* retcodes, int32_t* traj_stats,
+                             node_ragged_stats* stats);
+
 /* Execution options (no counterpart in the reference):
  *   NODE_OPT_GRAPHS          1 (default): step attempts are captured once into a
  *                            HIP graph and replayed; 0: launched one by one.

@@ -105,3 +105,45 @@ def test_ragged_rejects_unsupported_and_invalid():
         h2.ragged_forward(np.zeros((2, 16)), [1.0, 0.0], [[0.5], [0.0]])
     with pytest.raises(NodeHipError, match="increase"):
         h2.ragged_forward(np.zeros((1, 16)), [1.0], [[0.5, 0.2]])
+
+
+@pytest.mark.parametrize("case", ["config4", "spiral"])
+def test_ragged_adjoint_matches_per_trajectory_oracle(case):
+    """dL/du0 and each trajectory's dL/dp against the oracle's
+    InterpolatingAdjoint of that trajectory alone; dL/dp summed over
+    trajectories in FP64."""
+    from diffeqflux_hip import Handle
+    if case == "config4":
+        widths, acts, imap, n, tol = [16, 128, 128, 16], ["tanh", "tanh", "identity"], 0, 10, 1e-5
+    else:
+        widths, acts, imap, n, tol = [2, 50, 2], ["tanh", "identity"], MAP_CUBE, 8, 1e-6
+    m, p32, u0, t1, saves = ragged_case(widths, acts, imap, n, seed=7, nsave=(2, 9))
+    h = Handle(widths, acts, imap, "fp32", 1)
+    h.set_params(p32)
+    outs, ret, ts, _ = h.ragged_forward(u0, t1, saves, abstol=tol, reltol=tol)
+    assert np.all(ret == 1)
+    rng = np.random.default_rng(11)
+    cots = [rng.standard_normal(o.shape).astype(np.float32) for o in outs]
+    du0, dp, rows, aret, ats, tot = h.ragged_backward(cots, per_trajectory=True)
+    print(f"\n[ragged adjoint {case} n={n}] totals {tot}", flush=True)
+    assert np.all(aret == 1), aret
+    p = p32.astype(np.float64)
+    gp_sum = np.zeros_like(p)
+    worst_u = worst_p = 0.0
+    for i in range(n):
+        s = list(saves[i])
+        ref = OA.forward(m, p, u0[i:i + 1].astype(np.float64), 0.0, float(t1[i]), saveat=s,
+                         abstol=tol, reltol=tol)
+        c = [cots[i][k:k + 1].astype(np.float64) for k in range(len(s))]
+        gu0, gp, adj = OA.backward(m, p, ref, c, abstol=tol, reltol=tol)
+        gp_sum += gp
+        eu = float(np.abs(du0[i] - gu0[0]).max() / max(np.abs(gu0).max(), 1e-30))
+        ep = float(np.abs(rows[i] - gp).max() / max(np.abs(gp).max(), 1e-30))
+        worst_u, worst_p = max(worst_u, eu), max(worst_p, ep)
+        print(f"  traj {i}: saves {len(s)} adj steps gpu {ats[i, 1]}+{ats[i, 2]} oracle "
+              f"{adj.naccept}+{adj.nreject}; rel err du0 {eu:.2e} dp {ep:.2e}", flush=True)
+        assert eu < 50 * tol + 1e-5 and ep < 50 * tol + 1e-5, (i, eu, ep)
+    e = float(np.abs(dp - gp_sum).max() / np.abs(gp_sum).max())
+    print(f"  worst du0 {worst_u:.2e} dp {worst_p:.2e}; summed dL/dp rel err {e:.2e}", flush=True)
+    assert e < 50 * tol + 1e-5
+    assert np.allclose(dp, rows.astype(np.float64).sum(0), rtol=1e-6, atol=1e-9)
