@@ -6,9 +6,9 @@ with softplus, B=4096 trajectories u0 ~ N(0,1), tspan (0,1) with 32 saveat
 points, Tsit5 rtol=1e-4 (atol 1e-6), MSE loss against the saveat states of a
 random linear-plus-tanh teacher ODE, gradient by InterpolatingAdjoint.  GEMMs
 and VJPs on BF16 MFMA (FP32 accumulation): by default split BF16 (hi + lo
-operand planes, three passes) for the right-hand side and the adjoint's data
-path, one BF16 pass for the weight-gradient products; state, stage sums and
-adjoint state in FP32; time / controller / dL/dp quadrature in FP64.
+operand planes, three passes; --wgrad-precision bf16 runs the adjoint's
+weight-gradient products in one BF16 pass); state, stage sums and adjoint
+state in FP32; time / controller / dL/dp quadrature in FP64.
 
 One "step" = one forward solve + loss cotangent + one adjoint solve, all on
 inputs resident in HBM.  ``value`` = batched RHS evaluations (forward RHS +
@@ -223,9 +223,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
-    ap.add_argument("--wgrad-precision", default="bf16", choices=["bf16x3", "bf16", "fp32", "same"],
-                    help="precision of the adjoint weight-gradient products (bf16 on a bf16x3 "
-                         "run; 'same' = --precision)")
+    ap.add_argument("--wgrad-precision", default="same", choices=["bf16x3", "bf16", "fp32", "same"],
+                    help="precision of the adjoint weight-gradient products ('bf16' is allowed "
+                         "on a bf16x3 run; 'same' = --precision)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison leg")
     ap.add_argument("--no-bf16", action="store_true", help="skip the pure-BF16 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

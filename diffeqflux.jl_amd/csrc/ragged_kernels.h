@@ -884,10 +884,11 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
       else sm = rg_mu_pass<2>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, &fl, lane);
       if (r == s) { smu = sm; flmu = fl; }
     }
-    if (traj < 0) continue;
     // ---- consume, part 2: control
     bool fin = false;
-    if (phase == 0) {
+    if (traj < 0) {
+      // idle slot: nothing to decide
+    } else if (phase == 0) {
       s1l = sl;
       d1 = sqrt((sl + smu) / nel);
       if (a.dt_init > 0.0) {
@@ -972,7 +973,7 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
       stage = fsal_re ? 0 : 1;
       fin = c.done;
     }
-    if (fin) {
+    if (fin) {  // (only slots with a trajectory reach fin)
 #pragma unroll
       for (int t = 0; t < DT; ++t)
 #pragma unroll
@@ -991,6 +992,7 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
       need_start = true;
     }
     // the finished trajectory's parameter gradient must end in its own row
+    // (whole wave, converged)
     const int fin_any = __any(fin && muflip);
     if (fin_any) {
       for (int s = 0; s < RG_SLOTS; ++s) {
