@@ -139,39 +139,141 @@ class Runner:
         return dt, nf, nfa, last
 
 
+def gemm_shape(role, layer, B):
+    """(M, N, K) of the GEMM of a (role, layer) launch."""
+    i, o = WIDTHS[layer], WIDTHS[layer + 1]
+    if role in (0, 1, 2):      # forward / recompute of layer l: B x out, K = in
+        return (B, o, i)
+    if role in (3, 5):         # data gradient of layer l: B x in, K = out
+        return (B, i, o)
+    return (i, o, B)           # weight gradient: in x out, K = batch
+
+
+# HBM bytes per launch of a kernel, from two rocprofv3 --pmc runs of this
+# bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the same
+# counters read for a plain 256 MiB device copy (FETCH_SIZE 0.500x of the bytes
+# read, WRITE_SIZE 1.000x of the bytes written): bwd_data at K = 1024 fetches
+# 49,210 KiB x 2.0 and writes 32,768 KiB per launch.
+TRAFFIC = {("bf16x3", 3, (4096, 1024, 1024)): (49210 * 2.0 + 32768) * 1024}
+
+
 def profile_dominant(run):
-    """One profiled step with every GEMM timed: the (role, layer) with the
-    largest total time is the dominant kernel."""
+    """One profiled step with every GEMM timed.  Launches are grouped by
+    (role, GEMM shape) -- the layers of one shape run the same kernel -- and
+    the group with the largest total time is the dominant kernel."""
     run.h.prof_enable(None)
     run.step()
     recs = run.h.prof_collect()
     run.h.prof_enable([])
-    tot = {}
+    tot, grp = {}, {}
     for k, ms in recs:
         tot[k] = tot.get(k, 0.0) + ms
-    dom = max(tot, key=tot.get)
-    return dom, tot
+        g = (k[0], gemm_shape(k[0], k[1], run.B))
+        grp[g] = grp.get(g, 0.0) + ms
+    dom = max(grp, key=grp.get)
+    keys = sorted(k for k in tot if (k[0], gemm_shape(k[0], k[1], run.B)) == dom)
+    return dom, keys, tot
 
 
-def roofline_for(run, dom, precision, durations):
+def roofline_for(run, dom, keys, precision, durations):
     """achieved = algorithmic FLOPs per launch / average duration of the
     launches that did work (launches gated off on the device -- FSAL stage
     without re-evaluation, attempts enqueued after the end -- return at once
-    and are excluded by duration)."""
+    and are excluded by duration).  Durations come from HIP event pairs
+    around each launch on the library's stream; they include the event
+    commands themselves (a few microseconds above rocprof's kernel-only
+    duration of the same launches)."""
     d = np.array(durations, dtype=np.float64)
     if d.size == 0:
         return None, {}
     thr = 0.25 * np.percentile(d, 90)
     real = d[d >= thr]
     avg_ms = float(real.mean())
-    fl = flops_per_launch(dom[0], dom[1], run.B, precision, run.wgrad_precision)
+    role, layer = keys[0]
+    fl = flops_per_launch(role, layer, run.B, precision, run.wgrad_precision)
     achieved = fl / (avg_ms * 1e-3) / 1e12
     peak = PEAK[precision]
-    info = {"kernel": f"{ROLE_NAMES[dom[0]]}[layer {dom[1] + 1}]",
+    M, N, K = dom[1]
+    info = {"kernel": f"{ROLE_NAMES[role]}[layers {','.join(str(k[1] + 1) for k in keys)}] "
+                      f"{M}x{N}, K={K}",
             "launches": int(d.size), "launches_with_work": int(real.size),
             "avg_ms": avg_ms, "flop_per_launch": fl}
+    traffic = TRAFFIC.get((precision if not run.wgrad_precision or role != 4
+                           else run.wgrad_precision, role, dom[1]))
+    if traffic is not None:
+        info["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch, "
+                                  "copy-kernel calibrated")
     return {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None}, info
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": round(traffic) if traffic is not None else None}, info
+
+
+# ---------------------------------------------------------------- ragged leg
+RAGGED_WIDTHS = [16, 128, 128, 16]
+RAGGED_ACTS = ["tanh", "tanh", "identity"]
+
+
+def ragged_inputs(n, seed=4):
+    """Per-trajectory workload: u0 ~ N(0,1), t1 ~ U(0.1, 5), save count ~
+    U{5..200} on a uniform grid over [0, t1] (endpoints included)."""
+    rng = np.random.default_rng(seed)
+    u0 = rng.standard_normal((n, RAGGED_WIDTHS[0])).astype(np.float32)
+    t1 = rng.uniform(0.1, 5.0, n)
+    saves = [np.linspace(0.0, t1[i], int(rng.integers(5, 201))) for i in range(n)]
+    return u0, t1, saves
+
+
+def ragged_leg(n=10000, reps=3, cpu_sample=8):
+    """Per-trajectory solves (10,000 trajectories, D=16, MLP 16-128-128-16
+    tanh, own t1 / save times / adaptive steps, rtol 1e-5, atol 1e-6) forward
+    + adjoint of an MSE-style loss, FP32 MFMA; the oracle on a few of the same
+    trajectories (batch 1 each) as the CPU baseline."""
+    from diffeqflux_hip import Handle
+    from oracle.mlp import MLP
+    m = MLP(RAGGED_WIDTHS, RAGGED_ACTS)
+    p = m.glorot_init(np.random.default_rng(2)).astype(np.float32)
+    u0, t1, saves = ragged_inputs(n)
+    h = Handle(RAGGED_WIDTHS, RAGGED_ACTS, 0, "fp32", 1)
+    h.set_params(p)
+    kw = dict(abstol=1e-6, reltol=1e-5)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        outs, ret, ts, tot = h.ragged_forward(u0, t1, saves, **kw)
+        t1f = time.perf_counter()
+        cots = [o / (0.5 * len(o) * n) for o in outs]      # d/du of sum ||u - u/2||^2 / N
+        t1c = time.perf_counter()
+        du0, dp, _, aret, ats, atot = h.ragged_backward(cots)
+        t2 = time.perf_counter()
+        rec = (t2 - t0 - (t1c - t1f), t1f - t0, t2 - t1c, tot, atot)
+        if best is None or rec[0] < best[0]:
+            best = rec
+    wall, tf, ta, tot, atot = best
+    evals = tot["nf"] + atot["nf"]
+    res = {"workload": "10,000 trajectories, D=16, MLP 16-128-128-16 tanh, t1~U(0.1,5), "
+                       "saveat count~U{5..200}, rtol 1e-5 atol 1e-6, per-trajectory adaptive dt, "
+                       "FP32 MFMA",
+           "traj_per_s": round(n / wall, 1), "rhs_evals_per_s_traj": round(evals / wall, 1),
+           "fwd_ms": round(1e3 * tf, 2), "adj_ms": round(1e3 * ta, 2),
+           "fwd_steps": tot["naccept"], "fwd_rejects": tot["nreject"],
+           "adj_steps": atot["naccept"], "adj_rejects": atot["nreject"],
+           "all_success": bool(np.all(ret == 1) and np.all(aret == 1)),
+           "timing": "host entry points incl. PCIe copies of inputs/outputs"}
+    # CPU baseline: the oracle (FP32) on the first cpu_sample trajectories
+    from oracle import adjoint as OA
+    tc = time.perf_counter()
+    ev = 0
+    for i in range(cpu_sample):
+        f = OA.forward(m, p, u0[i:i + 1], 0.0, float(t1[i]), saveat=list(saves[i]), **kw)
+        c = [x / (0.5 * len(saves[i]) * n) for x in np.array(f.u)]
+        _, _, a = OA.backward(m, p, f, c, **kw)
+        ev += f.nf + a.nf
+    cs = time.perf_counter() - tc
+    res["cpu_baseline"] = {"value": round(cpu_sample / cs, 3), "unit": "trajectories/s",
+                           "cores": 1, "kind": "port",
+                           "sample": f"oracle FP32, trajectories 0..{cpu_sample - 1} solved one "
+                                     f"by one (batch 1), forward+adjoint, {ev} evals in {cs:.1f} s"}
+    return res
 
 
 # ---------------------------------------------------------------- CPU leg
@@ -230,6 +332,8 @@ def main():
     ap.add_argument("--no-bf16", action="store_true", help="skip the pure-BF16 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-batch", type=int, default=256)
+    ap.add_argument("--no-ragged", action="store_true",
+                    help="skip the per-trajectory (ragged) workload leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); "
@@ -256,14 +360,14 @@ def main():
     value = (nf + nfa) / dt
     # roofline pass: the same steps again with HIP events around every launch
     # of the dominant GEMM (event hooks launch kernels one by one)
-    dom, tot = profile_dominant(run)
+    dom, keys, tot = profile_dominant(run)
     log("[bench] per-role GEMM time (ms, one step, eager launches):",
         {f"{ROLE_NAMES[k[0]]}{k[1] + 1}": round(v, 2) for k, v in sorted(tot.items())})
-    run.h.prof_enable([dom])
+    run.h.prof_enable(keys)
     dtp, _, _, _ = run.timed(args.steps, 0)
     durs = [ms for _, ms in run.h.prof_collect()]
     run.h.prof_enable([])
-    roof, rinfo = roofline_for(run, dom, args.precision, durs)
+    roof, rinfo = roofline_for(run, dom, keys, args.precision, durs)
     rinfo["pass_ms_per_step"] = round(1e3 * dtp / args.steps, 2)
     sf, sa = last[0], last[1]
     log(f"[bench] {args.precision}: {value:.1f} evals/s, {1e3 * dt / args.steps:.2f} ms/step, "
@@ -310,6 +414,10 @@ def main():
         head = "bf16x3_wgrad_bf16" if wgp == "bf16" else args.precision
         extra["grad_rel_err"] = extra[f"grad_rel_err_{head}"]
         log(f"[bench] cpu baseline {cpu}; grad rel err {errs}")
+
+    if not args.no_ragged:
+        extra["ragged"] = ragged_leg()
+        log(f"[bench] ragged: {extra['ragged']}")
 
     rec = {
         "metric": METRIC, "value": round(value, 2), "unit": "RHS evals/s",

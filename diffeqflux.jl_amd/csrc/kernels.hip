@@ -407,6 +407,63 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
   const int s = e.stage;
   T* X0 = (T*)e.X0;
   double sum = 0.0, flag = 0.0;
+  if (e.mode == SM_STEP) {
+    // pass 1: loads and arithmetic only (no stores between the loads, which
+    // the compiler could not reorder across: the stage slots share one array)
+    f32x16 uv[MR][MC];
+    EPI_LOOP_BEGIN
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p, v = q * 4 + p;
+        uv[mi][mj][v] = 0.f;
+        if (c < e.D) {
+          const int64_t idx = (int64_t)r * e.D + c;
+          const float k = act_f(e.act, acc[mi][mj][v] + e.bias[c]);
+          acc[mi][mj][v] = k;
+          if (s < 6) {
+            float sm = cA[s + 1][s] * k;
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+              if (j < s) sm += cA[s + 1][j] * ent[(1 + j) * e.sv + idx];
+            uv[mi][mj][v] = ent[idx] + dt * sm;
+          } else {
+            float ut = cBT[6] * k;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) ut += cBT[j] * ent[(1 + j) * e.sv + idx];
+            ut *= dt;
+            const float up = ent[idx], un = ent[e.ent + idx];
+            const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
+            if (r < e.B) {
+              const double a = (double)(ut / sk);
+              sum += a * a;
+              if (isnan(un)) flag = 1.0;
+            }
+          }
+        }
+      }
+    EPI_LOOP_END
+    // pass 2: stores
+    EPI_LOOP_BEGIN
+      if (c < e.D) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int r = r0 + p, v = q * 4 + p;
+          const int64_t idx = (int64_t)r * e.D + c;
+          const float k = acc[mi][mj][v];
+          ent[(1 + s) * e.sv + idx] = k;
+          if (s < 6) {
+            const float u = uv[mi][mj][v];
+            put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, map_f(e.map, u));
+            if (s == 5) ent[e.ent + idx] = u;            // candidate tape[n+1].uprev
+          } else {
+            ent[e.ent + e.sv + idx] = k;                  // FSAL: tape[n+1].k1
+          }
+        }
+      }
+    EPI_LOOP_END
+    if (e.part) block_reduce_store(sum, flag, e.part, geo.slot);
+    return;
+  }
   EPI_LOOP_BEGIN
     if (c < e.D) {
       const float bias = e.bias[c];
@@ -466,10 +523,20 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
 template <class PP, int BR, int BC, int NT, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR][MC],
                                           const TileGeo& geo, int lane, const Ctl* ctl,
-                                          int , unsigned char*) {
+                                          int , unsigned char* lds) {
   const int rb = geo.rb, cb = geo.cb;
   using T = typename PP::T;
   const float dt = (float)ctl->dt;
+  // next-stage inputs (lambda_s and x = map(u(t_s))), written through LDS
+  // as row-major + transposed 16-byte stores after the element loop
+  const bool stage_out = e.mode == SM_STEP && e.stage < 6;
+  f32x16 lsv[MR][MC], xv[MR][MC];
+#pragma unroll
+  for (int i = 0; i < MR; ++i)
+#pragma unroll
+    for (int j = 0; j < MC; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) lsv[i][j][v] = xv[i][j][v] = 0.f;
   const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
   const int s = e.stage;
   T* L = (T*)e.L;
@@ -487,6 +554,75 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
     tdt = ctl->st_dt[slot];
 #pragma unroll
     for (int i = 0; i < 7; ++i) w[i] = ctl->st_w[slot][i];
+  }
+  if (e.mode == SM_STEP) {
+    // pass 1: loads and arithmetic; pass 2: stores (see the forward stage)
+    f32x16 uv[MR][MC];
+    EPI_LOOP_BEGIN
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p, v = q * 4 + p;
+        uv[mi][mj][v] = 0.f;
+        if (c < e.D) {
+          const int64_t idx = (int64_t)r * e.D + c;
+          float g = acc[mi][mj][v];
+          if (e.map == 1) { const float u = e.Ucur[idx]; g *= 3.f * u * u; }
+          const float k = -g;
+          acc[mi][mj][v] = k;
+          if (s < 6) {
+            float sm = cA[s + 1][s] * k;
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+              if (j < s) sm += cA[s + 1][j] * e.kl[j * e.sv + idx];
+            lsv[mi][mj][v] = e.lam[idx] + dt * sm;
+            const float* te = e.tape + (int64_t)tj * e.ent;
+            float u = te[idx];
+            if (!ex) {
+              float sm2 = 0.f;
+#pragma unroll
+              for (int i = 0; i < 7; ++i) sm2 += w[i] * te[(1 + i) * e.sv + idx];
+              u += tdt * sm2;
+            }
+            uv[mi][mj][v] = u;
+            xv[mi][mj][v] = map_f(e.map, u);
+          } else {
+            float ut = cBT[6] * k;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) ut += cBT[j] * e.kl[j * e.sv + idx];
+            ut *= dt;
+            const float lp = e.lam[idx], ln = e.lam_new[idx];
+            const float sk = atol + fmaxf(fabsf(lp), fabsf(ln)) * rtol;
+            if (r < e.B) {
+              const double a = (double)(ut / sk);
+              sum += a * a;
+              if (isnan(ln)) flag = 1.0;
+            }
+          }
+        }
+      }
+    EPI_LOOP_END
+    EPI_LOOP_BEGIN
+      if (c < e.D) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int r = r0 + p, v = q * 4 + p;
+          const int64_t idx = (int64_t)r * e.D + c;
+          e.kl[s * e.sv + idx] = acc[mi][mj][v];
+          if (s < 6) {
+            if (s == 5) e.lam_new[idx] = lsv[mi][mj][v];
+            e.Ucur[idx] = uv[mi][mj][v];
+          }
+        }
+      }
+    EPI_LOOP_END
+    if (stage_out) {
+      tile_write<T, PP::SPLIT, BR, BC, NT, MR, MC>(lds, lsv, geo, lane, L, e.ldl, e.pl, LT, e.ldlt,
+                                                   e.plt);
+      tile_write<T, PP::SPLIT, BR, BC, NT, MR, MC>(lds, xv, geo, lane, X0, e.ldx, e.px, X0T,
+                                                   e.ldxt, e.pxt);
+    }
+    if (e.part) block_reduce_store(sum, flag, e.part, geo.slot);
+    return;
   }
   EPI_LOOP_BEGIN
     if (c < e.D) {
@@ -522,8 +658,7 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
               if (j < s) sm += cA[s + 1][j] * e.kl[j * e.sv + idx];
             const float ls = e.lam[idx] + dt * sm;
             if (s == 5) e.lam_new[idx] = ls;
-            put<PP>(L + (int64_t)r * e.ldl + c, e.pl, ls);
-            put<PP>(LT + (int64_t)c * e.ldlt + r, e.plt, ls);
+            lsv[mi][mj][q * 4 + p] = ls;
             const float* te = e.tape + (int64_t)tj * e.ent;
             float u = te[idx];
             if (!ex) {
@@ -532,9 +667,7 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
               for (int i = 0; i < 7; ++i) sm2 += w[i] * te[(1 + i) * e.sv + idx];
               u += tdt * sm2;
             }
-            const float x = map_f(e.map, u);
-            put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, x);
-            put<PP>(X0T + (int64_t)c * e.ldxt + r, e.pxt, x);
+            xv[mi][mj][q * 4 + p] = map_f(e.map, u);
             e.Ucur[idx] = u;
           } else {
             float ut = cBT[6] * k;
@@ -553,6 +686,12 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
       }
     }
   EPI_LOOP_END
+  if (stage_out) {
+    tile_write<T, PP::SPLIT, BR, BC, NT, MR, MC>(lds, lsv, geo, lane, L, e.ldl, e.pl, LT, e.ldlt,
+                                                 e.plt);
+    tile_write<T, PP::SPLIT, BR, BC, NT, MR, MC>(lds, xv, geo, lane, X0, e.ldx, e.px, X0T, e.ldxt,
+                                                 e.pxt);
+  }
   if (e.part) block_reduce_store(sum, flag, e.part, geo.slot);
 }
 

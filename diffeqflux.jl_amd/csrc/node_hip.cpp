@@ -394,7 +394,7 @@ void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part
   g.A = L == 1 ? h->Lm : h->Dl[0];
   g.lda = l0.outp; g.B = l0.Wc; g.ldb = l0.outp; g.K = l0.outp;
   g.pa = (int64_t)h->Bp * l0.outp; g.pb = (int64_t)l0.inp * l0.outp; g.ksplit = 1;
-  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 1);
+  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 2);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   g.ksplit = stage_ks(h, PR_BWD_LAST, 0);
   g.ws = h->fix_ws; g.cnt = h->fix_cnt;
@@ -412,7 +412,7 @@ void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part
 }
 
 int vjp_last_tiles(node_handle* h) {
-  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 1);
+  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 2);
   return p.rt * p.ct;
 }
 
@@ -539,11 +539,13 @@ int autotune_impl(node_handle* h) {
     tune_slot(PR_FWD_HIDDEN, l, h->Bp, ly.outp, 0, [&] { launch_hidden(h, l, GATE_ALWAYS, false); });
     tune_slot(PR_ADJ_HIDDEN, l, h->Bp, ly.outp, 0, [&] { launch_hidden(h, l, GATE_ALWAYS, true); });
   }
+  // (the adjoint stage epilogue keeps five value tiles live: its 128-row
+  // form spills, so it starts at the 64-row tiles)
   auto tune_stage = [&](int role, int layer, int cols, int K, auto&& launch) {
     float best = 1e30f;
     int bc = -1, bks = 1;
     const int kt = (int)((int64_t)K * h->esz / 128);
-    for (int c = 1; c < gemm_cfg_count(); ++c) {
+    for (int c = role == PR_BWD_LAST ? 2 : 1; c < gemm_cfg_count(); ++c) {
       if (h->Bp % gemm_cfg_tile_rows(c) || cols % gemm_cfg_tile_cols(c)) continue;
       for (int ks = 1; ks <= 8; ks *= 2) {
         if (kt % ks) continue;

@@ -584,10 +584,30 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
 //  MODE 2: returns sum ((g_1 - g_0) / (atol + |mu| rtol))^2   (initial dt, d2;
 //          g_1 in block 7 - p0), *flag = 1 where g_1 != g_0
 template <int MODE>
+__device__ __forceinline__ void rg_mu_entry(int64_t idx, float kb, float ke, float dtf, float atf,
+                                            float rtf, const float* mu, float* mu_new, float m0,
+                                            float& acc, float& fl) {
+  const float gb = -kb, ge = -ke;
+  float e;
+  if (MODE == 0) {
+    const float mn = m0 + dtf * gb;
+    mu_new[idx] = mn;
+    e = __fdividef(dtf * ge, atf + fmaxf(fabsf(m0), fabsf(mn)) * rtf);
+  } else if (MODE == 1) {
+    e = __fdividef(gb, atf + fabsf(m0) * rtf);
+  } else {
+    e = __fdividef(ge, atf + fabsf(m0) * rtf);
+    if (ge != 0.f) fl = 1.f;
+  }
+  acc += e * e;
+}
+
+template <int MODE>
 __device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, double dt,
                              double atol, double rtol, const float* mu, float* mu_new,
                              double* flag, int lane) {
   constexpr int NJ = MODE == 0 ? 7 : (MODE == 1 ? 1 : 2);
+  constexpr int CB = 8;  // weight columns per batch of loads
   int blkj[NJ];
   float cb[NJ], ce[NJ];
 #pragma unroll
@@ -602,53 +622,69 @@ __device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, dou
       ce[j] = j == 0 ? -1.f : 1.f;  // g_1 - g_0
     }
   }
-  double sum = 0.0, fl = 0.0;
+  const float dtf = (float)dt, atf = (float)atol, rtf = (float)rtol;
+  float acc = 0.f, fl = 0.f;
   for (int l = 0; l < m.L; ++l) {
     const int in = m.in[l], out = m.out[l];
+    const int64_t w0 = m.w0[l];
     for (int o0 = 0; o0 < out; o0 += 64) {
       const int o = o0 + lane;
       const bool ok = o < out;
       float bd[NJ], ed[NJ];
+      float kb = 0.f, ke = 0.f;
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
         const float d = ok ? sbs[blkj[j] * m.sv + m.sd[l] + o] : 0.f;
         bd[j] = cb[j] * d;
         ed[j] = ce[j] * d;
+        kb += bd[j];
+        ke += ed[j];
       }
-      // bias entry (h = 1), then the weight column entries W[o, i]
-      for (int i = -1; i < in; ++i) {
-        float kb = 0.f, ke = 0.f;
+      if (ok) {  // bias entry (layer input 1)
+        const int64_t idx = m.b0[l] + o;
+        rg_mu_entry<MODE>(idx, kb, ke, dtf, atf, rtf, mu, mu_new, mu[idx], acc, fl);
+      }
+      for (int i0 = 0; i0 < in; i0 += 64) {
+        // this block's layer inputs h_j[i0 + lane]; entry i is broadcast by readlane
+        float hl[NJ];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          const float hj = i < 0 ? 1.f : sbs[blkj[j] * m.sv + m.sh[l] + i];
-          kb += bd[j] * hj;
-          ke += ed[j] * hj;
+        for (int j = 0; j < NJ; ++j)
+          hl[j] = (i0 + lane < in) ? sbs[blkj[j] * m.sv + m.sh[l] + i0 + lane] : 0.f;
+        const int ni = min(64, in - i0);
+        for (int ii = 0; ii < ni; ii += CB) {
+          float m0v[CB];
+#pragma unroll
+          for (int k = 0; k < CB; ++k) {
+            const int64_t idx = w0 + (int64_t)(i0 + ii + k) * out + o;
+            m0v[k] = (ok && ii + k < ni) ? mu[idx] : 0.f;
+          }
+#pragma unroll
+          for (int k = 0; k < CB; ++k) {
+            if (ii + k < ni) {
+              float kbw = 0.f, kew = 0.f;
+#pragma unroll
+              for (int j = 0; j < NJ; ++j) {
+                const float hj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hl[j]), ii + k));
+                kbw += bd[j] * hj;
+                kew += ed[j] * hj;
+              }
+              if (ok) {
+                const int64_t idx = w0 + (int64_t)(i0 + ii + k) * out + o;
+                rg_mu_entry<MODE>(idx, kbw, kew, dtf, atf, rtf, mu, mu_new, m0v[k], acc, fl);
+              }
+            }
+          }
         }
-        if (!ok) continue;
-        const int64_t idx = i < 0 ? m.b0[l] + o : m.w0[l] + (int64_t)i * out + o;
-        const float gb = -kb, ge = -ke;
-        const float m0 = mu[idx];
-        double e;
-        if (MODE == 0) {
-          const float mn = (float)((double)m0 + dt * (double)gb);
-          mu_new[idx] = mn;
-          e = dt * (double)ge / (atol + fmax(fabs((double)m0), fabs((double)mn)) * rtol);
-        } else if (MODE == 1) {
-          e = (double)gb / (atol + fabs((double)m0) * rtol);
-        } else {
-          e = (double)ge / (atol + fabs((double)m0) * rtol);
-          if (ge != 0.f) fl = 1.0;
-        }
-        sum += e * e;
       }
     }
   }
+  double sum = (double)acc, flg = (double)fl;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     sum += __shfl_xor(sum, off);
-    fl = fmax(fl, __shfl_xor(fl, off));
+    flg = fmax(flg, __shfl_xor(flg, off));
   }
-  if (flag) *flag = fl;
+  if (flag) *flag = flg;
   return sum;
 }
 
