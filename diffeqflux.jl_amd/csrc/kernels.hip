@@ -695,6 +695,114 @@ __device__ __forceinline__ void epi_apply(const EpiBwdLast& e, f32x16 (&acc)[MR]
   if (e.part) block_reduce_store(sum, flag, e.part, geo.slot);
 }
 
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiCnfStage& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& geo, int lane, const Ctl* ctl,
+                                          int , unsigned char* lds) {
+  const int rb = geo.rb, cb = geo.cb;
+  using T = typename PP::T;
+  const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
+  float* ent = e.tape + n * e.ent;
+  const float dt = (float)ctl->dt;
+  const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
+  const int s = e.stage;
+  T* X0 = (T*)e.X0;
+  // per-sample trace e^T (df/dz) e: products staged in LDS, summed per row
+  // (one tile spans every column of a row: the plan keeps BC >= the width)
+  float* pl = (float*)lds;
+  float* tr = pl + BR * (BC + 1);
+  __syncthreads();
+  EPI_LOOP_BEGIN
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int r = r0 + p, v = q * 4 + p;
+      const float pr = c < e.dm ? acc[mi][mj][v] * e.e[(int64_t)r * e.dm + c] : 0.f;
+      pl[(r - geo.row0) * (BC + 1) + (c - geo.col0)] = pr;
+    }
+  EPI_LOOP_END
+  __syncthreads();
+  for (int i = threadIdx.x; i < BR; i += NT) {
+    float sm = 0.f;
+    for (int j = 0; j < BC; ++j) sm += pl[i * (BC + 1) + j];
+    tr[i] = sm;
+  }
+  __syncthreads();
+  // stage values k = [f; -trace] (pass 1: loads / arithmetic, pass 2: stores)
+  double sum = 0.0, flag = 0.0;
+  f32x16 uv[MR][MC];
+  EPI_LOOP_BEGIN
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int r = r0 + p, v = q * 4 + p;
+      uv[mi][mj][v] = 0.f;
+      acc[mi][mj][v] = 0.f;
+      if (c < e.D) {
+        const int64_t idx = (int64_t)r * e.D + c;
+        const float k = c < e.dm ? e.f[(int64_t)r * e.dm + c] : -tr[r - geo.row0];
+        acc[mi][mj][v] = k;
+        const bool live = r < e.B;
+        if (e.mode == SM_STEP) {
+          if (s < 6) {
+            float sm = cA[s + 1][s] * k;
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+              if (j < s) sm += cA[s + 1][j] * ent[(1 + j) * e.sv + idx];
+            uv[mi][mj][v] = ent[idx] + dt * sm;
+          } else {
+            float ut = cBT[6] * k;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) ut += cBT[j] * ent[(1 + j) * e.sv + idx];
+            ut *= dt;
+            const float up = ent[idx], un = ent[e.ent + idx];
+            const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
+            if (live) {
+              const double a = (double)(ut / sk);
+              sum += a * a;
+              if (isnan(un)) flag = 1.0;
+            }
+          }
+        } else if (e.mode == SM_INIT_F0) {
+          const float sk = atol + fabsf(ent[idx]) * rtol;
+          if (live) { const double a = (double)(k / sk); sum += a * a; }
+        } else if (e.mode == SM_INIT_F1) {
+          const float f0 = ent[e.sv + idx];
+          const float sk = atol + fabsf(ent[idx]) * rtol;
+          if (live) {
+            const double a = (double)((k - f0) / sk);
+            sum += a * a;
+            if (k != f0) flag = 1.0;
+          }
+        }
+      }
+    }
+  EPI_LOOP_END
+  const double tnext = e.mode == SM_STEP && s < 6 ? ctl->t + dC[s + 1] * ctl->dt : 0.0;
+  EPI_LOOP_BEGIN
+    if (c < e.D) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p, v = q * 4 + p;
+        const int64_t idx = (int64_t)r * e.D + c;
+        const float k = acc[mi][mj][v];
+        if (e.mode == SM_STEP) {
+          ent[(1 + s) * e.sv + idx] = k;
+          if (s < 6) {
+            const float u = uv[mi][mj][v];
+            if (c < e.dm) put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, u);
+            else if (e.tcat && r < e.B) put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, (float)tnext);
+            if (s == 5) ent[e.ent + idx] = u;            // candidate tape[n+1].uprev
+          } else {
+            ent[e.ent + e.sv + idx] = k;                  // FSAL: tape[n+1].k1
+          }
+        } else if (e.mode == SM_INIT_F0) {
+          ent[e.sv + idx] = k;
+        }
+      }
+    }
+  EPI_LOOP_END
+  if (e.part) block_reduce_store(sum, flag, e.part, geo.slot);
+}
+
 __device__ __forceinline__ void fold_one(const EpiWgrad& e, float* gcur, float* gnext,
                                          int64_t idx, float g) {
   if (e.flags & WG_STORE_CUR) gcur[idx] = g;
@@ -918,7 +1026,8 @@ static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, i
   if (ga.ksplit < 1) ga.ksplit = 1;
   // the stage epilogues (per-element Tsit5 / adjoint stage work) are planned
   // with at most 64 output columns per tile; their 128x128 form is not built
-  constexpr bool kStage = std::is_same<Epi, EpiFwdStage>::value || std::is_same<Epi, EpiBwdLast>::value;
+  constexpr bool kStage = std::is_same<Epi, EpiFwdStage>::value ||
+                         std::is_same<Epi, EpiBwdLast>::value || std::is_same<Epi, EpiCnfStage>::value;
   if (kStage && cfg == 0) cfg = 1;
   switch (cfg) {
     case 0:
@@ -936,6 +1045,8 @@ void launch_gemm_fwdstage(int prec, int cfg, const GemmArgs& g, const EpiFwdStag
                           const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_bwddata(int prec, int cfg, const GemmArgs& g, const EpiBwdData& e,
                          const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
+void launch_gemm_cnfstage(int prec, int cfg, const GemmArgs& g, const EpiCnfStage& e,
+                          const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_bwdlast(int prec, int cfg, const GemmArgs& g, const EpiBwdLast& e,
                          const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
@@ -950,6 +1061,16 @@ int state_blocks(const StateGeom& sg) {
   return (int)(b < EW_MAX_BLOCKS ? b : EW_MAX_BLOCKS);
 }
 
+// model input of state entry (r, c): map(u) for the model's columns; for a
+// time-concatenated CNF model, column dm carries the stage time instead of
+// log p (only for real rows: padded rows stay zero)
+template <class PP>
+__device__ __forceinline__ void put_x0(const StateGeom& sg, typename PP::T* X0, int64_t px,
+                                       int r, int c, float u, double t) {
+  if (c < sg.dm) put<PP>(X0 + (int64_t)r * sg.x0w + c, px, map_f(sg.map, u));
+  else if (sg.tcat && c == sg.dm && r < sg.B) put<PP>(X0 + (int64_t)r * sg.x0w + c, px, (float)t);
+}
+
 template <class PP>
 __global__ void k_fwd_init(StateGeom sg, const float* __restrict__ u0, float* tape,
                            typename PP::T* X0, int64_t px, const Ctl* ctl, double* part) {
@@ -961,7 +1082,7 @@ __global__ void k_fwd_init(StateGeom sg, const float* __restrict__ u0, float* ta
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float u = r < sg.B ? u0[i] : 0.f;
     tape[i] = u;
-    put<PP>(X0 + (int64_t)r * sg.Dp + c, px, map_f(sg.map, u));
+    put_x0<PP>(sg, X0, px, r, c, u, ctl->t);
     if (r < sg.B) { const double a = (double)u / (atol + fabs((double)u) * rtol); sum += a * a; }
   }
   block_reduce_store(sum, 0.0, part, blockIdx.x);
@@ -981,7 +1102,7 @@ __global__ void k_fwd_initdt_prep(StateGeom sg, const float* __restrict__ tape,
        i += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float u = tape[i] + h * tape[sg.sv + i];
-    put<PP>(X0 + (int64_t)r * sg.Dp + c, px, map_f(sg.map, u));
+    put_x0<PP>(sg, X0, px, r, c, u, ctl->t + ctl->tdir * ctl->dt0);
   }
 }
 void launch_fwd_initdt_prep(const StateGeom& sg, const float* tape, void* X0, int64_t px,
@@ -1035,7 +1156,7 @@ __global__ void k_fwd_step_begin(StateGeom sg, const float* __restrict__ tape,
        i += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float u = ecur[i] + h * ecur[sg.sv + i];
-    put<PP>(X0 + (int64_t)r * sg.Dp + c, px, map_f(sg.map, u));
+    put_x0<PP>(sg, X0, px, r, c, u, ctl->t + dC[1] * ctl->dt);
   }
 }
 void launch_fwd_step_begin(const StateGeom& sg, const float* tape, void* X0, int64_t px,

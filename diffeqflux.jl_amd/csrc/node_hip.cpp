@@ -56,6 +56,10 @@ struct node_handle {
   node_desc desc{};
   int L = 0, D = 0, Dp = 0, B = 0, Bp = 0, prec = 0, esz = 4, planes = 1;
   int wg_prec = 0;  // precision of the weight-gradient products (<= prec)
+  bool cnf = false;        // continuous normalizing flow: state [z; log p]
+  bool cnf_active = false; // inside node_cnf_forward (probe set)
+  float* probe = nullptr;  // CNF probe e, FP32 [Bp][dm]
+  float* fbuf = nullptr;   // CNF model output f, FP32 [Bp][dm]
   int64_t P = 0;
   std::vector<Layer> layers;
   hipStream_t stream = nullptr;
@@ -253,6 +257,9 @@ StateGeom geom(const node_handle* h) {
   g.map = h->desc.input_map; g.prec = h->prec;
   g.sv = (int64_t)h->Bp * h->D;
   g.ent = 8 * g.sv;
+  g.dm = h->cnf ? h->D - 1 : h->D;
+  g.tcat = h->desc.input_map == NODE_MAP_CNF_TIME ? 1 : 0;
+  g.x0w = h->layers[0].inp;
   return g;
 }
 
@@ -261,10 +268,18 @@ int validate_desc(const node_desc* d, std::string& msg) {
   if (d->n_layers < 1 || d->n_layers > NODE_HIP_MAX_LAYERS) { msg = "n_layers must be 1..8"; return NODE_ERR_INVALID; }
   for (int i = 0; i <= d->n_layers; ++i)
     if (d->widths[i] < 1 || d->widths[i] > 65536) { msg = "widths must be 1..65536"; return NODE_ERR_INVALID; }
-  if (d->widths[0] != d->widths[d->n_layers]) { msg = "output width must equal input width (ODE state dimension)"; return NODE_ERR_INVALID; }
   for (int i = 0; i < d->n_layers; ++i)
     if (d->acts[i] < 0 || d->acts[i] > 4) { msg = "unknown activation"; return NODE_ERR_INVALID; }
-  if (d->input_map != NODE_MAP_IDENTITY && d->input_map != NODE_MAP_CUBE) { msg = "unknown input_map"; return NODE_ERR_INVALID; }
+  if (d->input_map == NODE_MAP_CNF || d->input_map == NODE_MAP_CNF_TIME) {
+    const int dm = d->widths[d->n_layers];
+    if (d->widths[0] != dm + (d->input_map == NODE_MAP_CNF_TIME ? 1 : 0)) { msg = "CNF model: input width must be D (or D + 1 with time)"; return NODE_ERR_INVALID; }
+    if (d->n_layers < 2) { msg = "CNF model needs at least 2 Dense layers"; return NODE_ERR_INVALID; }
+    if (d->acts[d->n_layers - 1] != NODE_ACT_IDENTITY) { msg = "CNF model: the last layer must be linear"; return NODE_ERR_INVALID; }
+    if (dm + 1 > 64) { msg = "CNF state dimension D + 1 must be <= 64"; return NODE_ERR_INVALID; }
+  } else {
+    if (d->widths[0] != d->widths[d->n_layers]) { msg = "output width must equal input width (ODE state dimension)"; return NODE_ERR_INVALID; }
+    if (d->input_map != NODE_MAP_IDENTITY && d->input_map != NODE_MAP_CUBE) { msg = "unknown input_map"; return NODE_ERR_INVALID; }
+  }
   if (d->precision < NODE_PREC_FP32 || d->precision > NODE_PREC_BF16X3) { msg = "unknown precision"; return NODE_ERR_INVALID; }
   if (d->batch < 1 || d->batch > (int64_t)1 << 24) { msg = "batch must be 1..2^24"; return NODE_ERR_INVALID; }
   return 0;
@@ -330,12 +345,17 @@ void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* par
 }
 
 // One batched right-hand side: X0 -> layers -> last-layer epilogue.
+void cnf_eval(node_handle* h, int stage, int mode, int gate, double* part);
+int cnf_last_tiles(node_handle* h);
+
 void rhs_eval(node_handle* h, int stage, int mode, int gate, double* part) {
+  if (h->cnf) { cnf_eval(h, stage, mode, gate, part); return; }
   for (int l = 0; l < h->L - 1; ++l) launch_hidden(h, l, gate, false);
   launch_fwd_stage(h, stage, mode, gate, part);
 }
 
 int rhs_last_tiles(node_handle* h) {
+  if (h->cnf) return cnf_last_tiles(h);
   const Layer& ly = h->layers[h->L - 1];
   GemmPlan p = plan_role(h, PR_FWD_STAGE, h->L - 1, h->Bp, ly.outp, 1);
   return p.rt * p.ct;
@@ -729,7 +749,7 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
   double* p0 = h->part;
   double* px = h->part + SEC;
   const int nb = state_blocks(sg);
-  const int64_t xpl = (int64_t)h->Bp * h->Dp;  // X0 plane stride
+  const int64_t xpl = (int64_t)h->Bp * h->layers[0].inp;  // X0 plane stride
   launch_fwd_init(sg, d_u0, h->tape, h->X0, xpl, h->ctl, p0, nb, h->stream);
   rhs_eval(h, 0, SM_INIT_F0, GATE_ALWAYS, px);
   const int nx = rhs_last_tiles(h);
@@ -812,9 +832,58 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
   return 0;
 }
 
+// ---------------------------------------------------------------- CNF
+// One right-hand side of the continuous normalizing flow: hidden layers
+// (keeping act'), the last layer f (plain, FP32), the probe's backprop
+// e^T df/dx through the data-gradient GEMMs, and the fused stage GEMM whose
+// epilogue forms e^T (df/dz) e and the Tsit5 stage of [z; log p].
+void cnf_eval(node_handle* h, int stage, int mode, int gate, double* part) {
+  const int L = h->L;
+  for (int l = 0; l < L - 1; ++l) launch_hidden(h, l, gate, true);
+  {  // f = last layer (identity activation), FP32
+    const int l = L - 1;
+    const Layer& ly = h->layers[l];
+    GemmArgs g{};
+    g.A = h->H[l - 1];
+    g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
+    g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
+    GemmPlan p = plan_role(h, PR_FWD_STAGE, l, h->Bp, ly.outp, 1);
+    g.row_tiles = p.rt; g.col_tiles = p.ct;
+    EpiFwdStage e{};
+    e.bias = ly.bias; e.act = ly.act; e.D = ly.out; e.B = h->B; e.stage = 0; e.mode = SM_PLAIN;
+    e.scratch = h->fbuf;
+    ProfScope ps(h, PR_FWD_STAGE, l);
+    launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  }
+  for (int l = L - 1; l >= 1; --l) launch_bwd_data(h, l, gate);
+  const Layer& l0 = h->layers[0];
+  GemmArgs g{};
+  g.A = h->Dl[0];
+  g.lda = l0.outp; g.B = l0.Wc; g.ldb = l0.outp; g.K = l0.outp;
+  g.pa = (int64_t)h->Bp * l0.outp; g.pb = (int64_t)l0.inp * l0.outp; g.ksplit = 1;
+  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, l0.inp, 2);
+  g.row_tiles = p.rt; g.col_tiles = p.ct;
+  const StateGeom sg = geom(h);
+  EpiCnfStage e{};
+  e.D = h->D; e.B = h->B; e.dm = sg.dm; e.tcat = sg.tcat; e.stage = stage; e.mode = mode;
+  e.tape = h->tape; e.ent = sg.ent; e.sv = sg.sv;
+  e.X0 = h->X0; e.ldx = l0.inp; e.px = (int64_t)h->Bp * l0.inp;
+  e.f = h->fbuf; e.e = h->probe; e.part = part;
+  ProfScope ps(h, PR_BWD_LAST, 0);
+  launch_gemm_cnfstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+}
+
+int cnf_last_tiles(node_handle* h) {
+  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->layers[0].inp, 2);
+  return p.rt * p.ct;
+}
+
 // ---------------------------------------------------------------- backward
 int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   if (!h->have_forward) return fail(h, NODE_ERR_STATE, "no forward solve to differentiate");
+  if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "the adjoint of a CNF solve is not on the HIP path");
+  if (h->layers[h->L - 1].act != NODE_ACT_IDENTITY)
+    return fail(h, NODE_ERR_UNSUPPORTED, "adjoint: the last Dense layer must be linear");
   if (h->fwd_stats.retcode != NODE_RET_SUCCESS)
     return fail(h, NODE_ERR_STATE, "forward solve did not succeed (retcode != Success)");
   if (!(h->fwd_opts.t1 > h->fwd_opts.t0))
@@ -1211,7 +1280,8 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
     for (int& c : row) c = 1;
   h->desc = *desc;
   h->L = desc->n_layers;
-  h->D = desc->widths[0];
+  h->cnf = desc->input_map == NODE_MAP_CNF || desc->input_map == NODE_MAP_CNF_TIME;
+  h->D = h->cnf ? desc->widths[desc->n_layers] + 1 : desc->widths[0];  // ODE state width
   h->Dp = (int)rup(h->D, KPAD);
   h->B = (int)desc->batch;
   h->Bp = (int)rup(h->B, BPAD);
@@ -1388,6 +1458,7 @@ int32_t node_forward(node_handle_t* h, const node_solve_opts* opts, const float*
   if (!h) return NODE_ERR_INVALID;
   std::lock_guard<std::mutex> g(h->mtx);
   if (!u0) return fail(h, NODE_ERR_INVALID, "u0 is NULL");
+  if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "CNF handle: use node_cnf_forward");
   if (n_saveat < 0 || (n_saveat > 0 && !saveat)) return fail(h, NODE_ERR_INVALID, "bad saveat");
   if (int rc = maybe_autotune(h)) return rc;  // before u0 is staged in the scratch buffer
   float* du0 = h->scratch;  // u0 staged through the scratch state buffer
@@ -1398,6 +1469,57 @@ int32_t node_forward(node_handle_t* h, const node_solve_opts* opts, const float*
     if (h->nsaved > out_cap) return fail(h, NODE_ERR_CAPACITY, "out_cap smaller than the number of saved states");
     HIPCHK(h, hipMemcpy(out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float), hipMemcpyDeviceToHost));
   }
+  return 0;
+}
+
+int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const float* x,
+                         const float* probe, float* z, float* delta_logp, float* logpx,
+                         node_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "not a CNF handle (input_map NODE_MAP_CNF*)");
+  if (!opts || !x || !probe) return fail(h, NODE_ERR_INVALID, "null argument");
+  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  if (int rc = ensure_adjoint(h)) return rc;  // Lm holds the probe for the data-gradient GEMMs
+  if (int rc = maybe_autotune(h)) return rc;
+  const int dm = h->D - 1, B = h->B;
+  if (!h->probe && dalloc_t(h, &h->probe, (size_t)h->Bp * dm * sizeof(float))) return NODE_ERR_HIP;
+  if (!h->fbuf && dalloc_t(h, &h->fbuf, (size_t)h->Bp * dm * sizeof(float))) return NODE_ERR_HIP;
+  HIPCHK(h, hipMemcpyAsync(h->probe, probe, (size_t)B * dm * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  StateGeom sgm = geom(h);
+  sgm.D = dm;
+  const Layer& ll = h->layers[h->L - 1];
+  const int64_t pl = (int64_t)h->Bp * ll.outp;
+  launch_prep_lambda(sgm, h->probe, h->Lm, h->LmT, ll.outp, h->Bp, pl, pl, h->stream);
+  // u0 = [x; 0] staged through the scratch state buffer
+  std::vector<float> u0((size_t)B * h->D, 0.f);
+  for (int b = 0; b < B; ++b)
+    for (int c = 0; c < dm; ++c) u0[(size_t)b * h->D + c] = x[(size_t)b * dm + c];
+  HIPCHK(h, hipMemcpyAsync(h->scratch, u0.data(), u0.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  node_solve_opts o = *opts;
+  o.save_start = 0; o.save_end = 1; o.save_everystep = 0;
+  node_stats st{};
+  h->cnf_active = true;
+  int rc = forward_impl(h, &o, h->scratch, nullptr, 0, &st);
+  h->cnf_active = false;
+  if (rc) return rc;
+  std::vector<float> fin((size_t)B * h->D);
+  if (st.nsaved >= 1)
+    HIPCHK(h, hipMemcpy(fin.data(), h->out_dev, fin.size() * sizeof(float), hipMemcpyDeviceToHost));
+  const double lc = 0.5 * dm * std::log(2.0 * M_PI);
+  for (int b = 0; b < B; ++b) {
+    double ss = 0.0;
+    for (int c = 0; c < dm; ++c) {
+      const float v = fin[(size_t)b * h->D + c];
+      if (z) z[(size_t)b * dm + c] = v;
+      ss += (double)v * v;
+    }
+    const float dl = fin[(size_t)b * h->D + dm];
+    if (delta_logp) delta_logp[b] = dl;
+    if (logpx) logpx[b] = (float)(-0.5 * ss - lc - (double)dl);
+  }
+  h->have_forward = false;  // no adjoint of this solve
+  if (stats) *stats = st;
   return 0;
 }
 

@@ -77,6 +77,19 @@ struct EpiFwdStage {
   int64_t px;                              // plane stride of X0
 };
 
+// continuous normalizing flow stage (state [z; log p], D = dm + 1 columns):
+// the GEMM computes w = (e^T df/dx) per sample (x the model input); the
+// epilogue forms trace = sum_{c<dm} w e, the stage k = [f; -trace] (f from
+// the last layer, FP32 [Bp][dm]) and the Tsit5 stage combination as
+// EpiFwdStage does, writing the next model input [z_s; t_s].
+struct EpiCnfStage {
+  int D; int B; int dm; int tcat; int stage; int mode;
+  float* tape; int64_t ent; int64_t sv;
+  void* X0; int ldx; int64_t px;
+  const float* f; const float* e;          // [Bp][dm] FP32 model output / probe
+  double* part;
+};
+
 // hidden data gradient: delta = acc * act'(z)  (row-major + transposed)
 struct EpiBwdData {
   int C; int ldo; int ldt; const void* S; void* Dl; void* DlT;
@@ -134,6 +147,8 @@ void launch_gemm_fwdstage(int prec, int cfg, const GemmArgs& g, const EpiFwdStag
                           const Ctl* ctl, int gate, hipStream_t s);
 void launch_gemm_bwddata(int prec, int cfg, const GemmArgs& g, const EpiBwdData& e,
                          const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_cnfstage(int prec, int cfg, const GemmArgs& g, const EpiCnfStage& e,
+                          const Ctl* ctl, int gate, hipStream_t s);
 void launch_gemm_bwdlast(int prec, int cfg, const GemmArgs& g, const EpiBwdLast& e,
                          const Ctl* ctl, int gate, hipStream_t s);
 void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
@@ -148,6 +163,9 @@ struct StateGeom {
   int map, prec;             // prec: 0 fp32, 1 bf16, 2 split bf16 (hi/lo planes)
   int64_t sv;                // floats per state vector (Bp * D)
   int64_t ent;               // floats per tape entry (8 * sv)
+  int dm;                    // state columns fed to the model (D; D - 1 for a CNF: not log p)
+  int tcat;                  // 1: the model input is [z; t] (column dm of X0 holds t)
+  int x0w;                   // row stride of the model-input matrix X0
 };
 
 void launch_fwd_init(const StateGeom& sg, const float* u0, float* tape, void* X0,

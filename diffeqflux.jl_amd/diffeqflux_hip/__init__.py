@@ -9,8 +9,9 @@ from .neural_ode import (NeuralODE, Chain, Dense, Cube, Tsit5, InterpolatingAdjo
                          GaussAdjoint, ODESolution, save_plan, AugmentedNDELayer, DimMover)
 from .handle import Handle
 from ._lib import load, NodeHipError
+from .ffjord import FFJORD
 
 __all__ = ["NeuralODE", "Chain", "Dense", "Cube", "Tsit5", "InterpolatingAdjoint",
            "ZygoteVJP", "BacksolveAdjoint", "TrackerAdjoint", "QuadratureAdjoint",
            "GaussAdjoint", "ODESolution", "save_plan", "AugmentedNDELayer", "DimMover",
-           "Handle", "load", "NodeHipError"]
+           "Handle", "load", "NodeHipError", "FFJORD"]

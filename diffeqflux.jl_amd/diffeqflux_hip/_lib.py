@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libnode_hip.so")
 
 MAX_LAYERS = 8
 ACT = {"identity": 0, "tanh": 1, "softplus": 2, "relu": 3, "sigmoid": 4}
-MAP_IDENTITY, MAP_CUBE = 0, 1
+MAP_IDENTITY, MAP_CUBE, MAP_CNF, MAP_CNF_TIME = 0, 1, 2, 3
 OPT_GRAPHS, OPT_GRAPH_ATTEMPTS, OPT_AUTOTUNE, OPT_WGRAD_PRECISION = 1, 2, 3, 4
 PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 RET_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN",
@@ -69,6 +69,9 @@ SIGNATURES = [
     ("node_forward_device", c_i32, [ctypes.c_void_p, ctypes.POINTER(SolveOpts),
                                     ctypes.c_void_p, ctypes.c_void_p, c_i64,
                                     ctypes.c_void_p, c_i64, ctypes.POINTER(Stats)]),
+    ("node_cnf_forward", c_i32, [ctypes.c_void_p, ctypes.POINTER(SolveOpts), ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("node_saved_times", c_i32, [ctypes.c_void_p, ctypes.c_void_p, c_i64]),
     ("node_copy_saved", c_i32, [ctypes.c_void_p, ctypes.c_void_p, c_i64]),
     ("node_backward", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

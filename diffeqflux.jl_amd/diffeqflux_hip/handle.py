@@ -152,6 +152,24 @@ class Handle:
             self._check(rc, "node_backward")
         return du0, dp, st.as_dict()
 
+    def cnf_forward(self, x, probe, t0, t1, abstol=1e-6, reltol=1e-3, adaptive=True, dt=0.0,
+                    dtmax=0.0, dtmin=0.0, maxiters=0):
+        """FFJORD forward on a CNF handle.  x, probe: (B, D).  Returns
+        (logpx (B,), z(T) (B, D), delta_logp (B,), stats dict)."""
+        D = self.widths[-1]
+        x = _f32(x).reshape(self.B, D)
+        e = _f32(probe).reshape(self.B, D)
+        z = np.empty((self.B, D), np.float32)
+        dl = np.empty(self.B, np.float32)
+        lp = np.empty(self.B, np.float32)
+        o = self.make_opts(t0, t1, abstol=abstol, reltol=reltol, adaptive=adaptive, dt=dt,
+                           dtmax=dtmax, dtmin=dtmin, maxiters=maxiters)
+        st = _lib.Stats()
+        self._check(self.lib.node_cnf_forward(self.h, ctypes.byref(o), _ptr(x), _ptr(e), _ptr(z),
+                                              _ptr(dl), _ptr(lp), ctypes.byref(st)),
+                    "node_cnf_forward")
+        return lp, z, dl, st.as_dict()
+
     def ragged_forward(self, u0, t1, save_times, t0=0.0, abstol=1e-6, reltol=1e-3, adaptive=True,
                        dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=0):
         """Per-trajectory solves.  u0: (n, D); t1: (n,); save_times: list of n

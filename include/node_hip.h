@@ -45,7 +45,12 @@ enum { NODE_ACT_IDENTITY = 0, NODE_ACT_TANH = 1, NODE_ACT_SOFTPLUS = 2,
        NODE_ACT_RELU = 3, NODE_ACT_SIGMOID = 4 };
 /* parameter-free first layer: identity or x -> x.^3 (WrappedFunction,
  * docs/src/examples/neural_ode.md:32) */
-enum { NODE_MAP_IDENTITY = 0, NODE_MAP_CUBE = 1 };
+enum { NODE_MAP_IDENTITY = 0, NODE_MAP_CUBE = 1,
+/* continuous normalizing flow handles (node_cnf_forward): the model maps z
+ * (NODE_MAP_CNF, widths[0] == widths[n_layers] == D) or the time-concatenated
+ * [z; t] (NODE_MAP_CNF_TIME, widths[0] == D + 1) to dz/dt; the ODE state is
+ * [z; log p] (D + 1 <= 64 entries per sample), the last layer is linear. */
+       NODE_MAP_CNF = 2, NODE_MAP_CNF_TIME = 3 };
 /* GEMM precision of the right-hand side and its VJPs.  State, stage sums and
  * adjoint state stay FP32 in all modes.
  *  FP32   : FP32 operands on FP32 MFMA.
@@ -131,6 +136,19 @@ int32_t node_forward(node_handle_t* h, const node_solve_opts* opts,
 int32_t node_forward_device(node_handle_t* h, const node_solve_opts* opts,
                             const float* d_u0, const double* saveat, int64_t n_saveat,
                             float* d_out, int64_t out_cap, node_stats* stats);
+
+/* FFJORD forward (src/ffjord.jl:81-178, __ffjord + __forward_ffjord with
+ * monte_carlo = true): solves d[z; log p]/dt = [f(z[, t]); -e^T (df/dz) e]
+ * from [x; 0] over opts->t0..t1 with Tsit5 (error norm over all (D+1) x B
+ * entries, one step size for the batch), the Hutchinson probe e (D x B) held
+ * fixed for the whole solve (the reference redraws randn! on every call:
+ * here the caller draws it, e.g. Rademacher, once per solve).  Writes z(T)
+ * (D x B), delta_logp = log p(T) (B) and logpx = logpz(z(T)) - delta_logp
+ * with the standard-normal base density (B).  Any output may be NULL.
+ * Handle: input_map NODE_MAP_CNF or NODE_MAP_CNF_TIME.  Host pointers. */
+int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const float* x,
+                         const float* probe, float* z, float* delta_logp, float* logpx,
+                         node_stats* stats);
 
 /* Times of the states saved by the last forward (n = stats->nsaved). */
 int32_t node_saved_times(node_handle_t* h, double* t_out, int64_t cap);

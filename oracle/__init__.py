@@ -26,5 +26,6 @@ Modules:
   mlp       -- Lux Chain of Dense layers: forward, VJP, flat parameter layout
   tsit5     -- the OrdinaryDiffEq integrator loop (adaptive + fixed dt)
   adjoint   -- SciMLSensitivity InterpolatingAdjoint (continuous adjoint)
+  ffjord    -- FFJORD augmented RHS (Hutchinson / exact trace) and log-likelihood
   problems  -- the five workload definitions (synthetic data generators)
 """
