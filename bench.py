@@ -276,6 +276,62 @@ def ragged_leg(n=10000, reps=3, cpu_sample=8):
     return res
 
 
+# ---------------------------------------------------------------- FFJORD leg
+def gaussian_ring(n, D=16, seed=6):
+    """8-component Gaussian mixture, means on a radius-4 ring in the first two
+    coordinates, std 0.5; N(0,1) in the other coordinates."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, 8, n)
+    x = rng.standard_normal((n, D)).astype(np.float32)
+    ang = 2 * math.pi * k / 8
+    x[:, 0] = 4 * np.cos(ang) + 0.5 * rng.standard_normal(n)
+    x[:, 1] = 4 * np.sin(ang) + 0.5 * rng.standard_normal(n)
+    return x
+
+
+def ffjord_leg(B=8192, reps=3, precision="bf16x3", cpu_batch=64):
+    """FFJORD log-likelihood forward (D=16, time-concatenated MLP 17-512-512-16
+    tanh, Hutchinson trace with one Rademacher probe per sample, tspan (0,1),
+    Tsit5 rtol 1e-5 atol 1e-5) on the Gaussian-ring data; the oracle (FP32) on
+    a slice of the batch as the CPU baseline."""
+    from diffeqflux_hip import Handle, _lib
+    from oracle.mlp import MLP
+    widths, acts = [17, 512, 512, 16], ["tanh", "tanh", "identity"]
+    m = MLP(widths, acts)
+    p = m.glorot_init(np.random.default_rng(3)).astype(np.float32)
+    x = gaussian_ring(B)
+    e = np.random.default_rng(7).choice(np.array([-1.0, 1.0], np.float32), size=(B, 16))
+    h = Handle(widths, acts, _lib.MAP_CNF_TIME, precision, B)
+    h.set_params(p)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        lp, z, dl, st = h.cnf_forward(x, e, 0.0, 1.0, abstol=1e-5, reltol=1e-5)
+        dt = time.perf_counter() - t0
+        if best is None or dt < best[0]:
+            best = (dt, st, lp)
+    dt, st, lp = best
+    fl = 2.0 * B * sum(i * o for i, o in zip(widths[:-1], widths[1:])) * 2  # f + probe VJP
+    res = {"workload": "FFJORD forward, B=8192, D=16, MLP 17-512-512-16 tanh (time-concat), "
+                       "Hutchinson 1 Rademacher probe, tspan (0,1), rtol 1e-5 atol 1e-5",
+           "precision": precision, "ms": round(1e3 * dt, 2), "nf": st["nf"],
+           "steps": f"{st['naccept']}+{st['nreject']}",
+           "rhs_evals_per_s": round(st["nf"] / dt, 1), "samples_per_s": round(B / dt, 1),
+           "tflops_issued": round(st["nf"] * fl * (3 if precision == "bf16x3" else 1) / dt / 1e12, 1),
+           "mean_logpx": float(np.mean(lp)),
+           "timing": "host entry point incl. PCIe copies of x / probe / results"}
+    from oracle import ffjord as FF
+    tc = time.perf_counter()
+    rlp, _, _, sol = FF.forward(m, p, x[:cpu_batch], 0.0, 1.0, probe=e[:cpu_batch],
+                                time_concat=True, abstol=1e-5, reltol=1e-5)
+    cs = time.perf_counter() - tc
+    res["cpu_baseline"] = {"value": round(cpu_batch / cs, 2), "unit": "samples/s", "cores": 1,
+                           "kind": "port",
+                           "sample": f"oracle FP32 FFJORD forward on {cpu_batch} of the samples "
+                                     f"(same probe), {sol.nf} evals in {cs:.1f} s"}
+    return res
+
+
 # ---------------------------------------------------------------- CPU leg
 def cpu_baseline_leg(p, Bs, gpu_grads):
     """The oracle (numpy FP32 restatement of the reference algorithm) timed on
@@ -334,6 +390,7 @@ def main():
     ap.add_argument("--cpu-batch", type=int, default=256)
     ap.add_argument("--no-ragged", action="store_true",
                     help="skip the per-trajectory (ragged) workload leg")
+    ap.add_argument("--no-ffjord", action="store_true", help="skip the FFJORD workload leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); "
@@ -418,6 +475,9 @@ def main():
     if not args.no_ragged:
         extra["ragged"] = ragged_leg()
         log(f"[bench] ragged: {extra['ragged']}")
+    if not args.no_ffjord:
+        extra["ffjord"] = ffjord_leg()
+        log(f"[bench] ffjord: {extra['ffjord']}")
 
     rec = {
         "metric": METRIC, "value": round(value, 2), "unit": "RHS evals/s",
