@@ -113,29 +113,24 @@ __device__ __forceinline__ void rg_layer(const float* W, int pitch, const float*
 }
 
 // The model's forward pass for the 16 slot columns: x (DT state tiles, input
-// map applied) -> f (DT tiles).
+// map applied) -> f (DT tiles).  One rolled loop body over the layers (the
+// output is copied into place after each layer) keeps the code small enough
+// for the instruction cache.
 template <int DT>
 __device__ __forceinline__ void rg_mlp(const RaggedModel& m, const float* lds,
                                        const f32x4 (&x)[DT], f32x4 (&f)[DT], int lane) {
-  const int L = m.L;
-  if (L == 1) {
-    rg_layer<DT, DT>(lds + m.lw[0], m.pitch[0], lds + m.lb[0], m.act[0], x, m.ti[0], m.to[0], f, lane);
-    return;
-  }
-  f32x4 ha[RG_HT], hb[RG_HT];
-  rg_layer<DT, RG_HT>(lds + m.lw[0], m.pitch[0], lds + m.lb[0], m.act[0], x, m.ti[0], m.to[0], ha, lane);
+  f32x4 hin[RG_HT], hout[RG_HT];
 #pragma unroll
-  for (int l = 1; l < RG_MAXL - 1; ++l) {
-    if (l < L - 1) {
-      if (l & 1)
-        rg_layer<RG_HT, RG_HT>(lds + m.lw[l], m.pitch[l], lds + m.lb[l], m.act[l], ha, m.ti[l], m.to[l], hb, lane);
-      else
-        rg_layer<RG_HT, RG_HT>(lds + m.lw[l], m.pitch[l], lds + m.lb[l], m.act[l], hb, m.ti[l], m.to[l], ha, lane);
-    }
+  for (int t = 0; t < RG_HT; ++t) hin[t] = t < DT ? x[t < DT ? t : 0] : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int l = 0; l < m.L; ++l) {
+    rg_layer<RG_HT, RG_HT>(lds + m.lw[l], m.pitch[l], lds + m.lb[l], m.act[l], hin, m.ti[l],
+                           m.to[l], hout, lane);
+#pragma unroll
+    for (int t = 0; t < RG_HT; ++t) hin[t] = hout[t];
   }
-  const int l = L - 1;
-  if (l & 1) rg_layer<RG_HT, DT>(lds + m.lw[l], m.pitch[l], lds + m.lb[l], m.act[l], ha, m.ti[l], m.to[l], f, lane);
-  else rg_layer<RG_HT, DT>(lds + m.lw[l], m.pitch[l], lds + m.lb[l], m.act[l], hb, m.ti[l], m.to[l], f, lane);
+#pragma unroll
+  for (int t = 0; t < DT; ++t) f[t] = hin[t];
 }
 
 // weights of every layer into LDS rows [out_p][in_p + 4], bias [out_p];
@@ -484,94 +479,89 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
                                        float* blk, f32x4 (&gout)[RG_HT], int lane) {
   const int r = lane & 15, g4 = 4 * (lane >> 4);
   const int L = m.L;
-  f32x4 ha[RG_HT], hb[RG_HT];
+  // one rolled loop body per sweep (the layer's input is copied into place
+  // after each layer): the unrolled-over-layers form did not fit the
+  // instruction cache
+  f32x4 hin[RG_HT], hout[RG_HT];
 #pragma unroll
   for (int t = 0; t < RG_HT; ++t) {
-    ha[t] = x0[t];
-    hb[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    hin[t] = x0[t];
     if (t < m.ti[0]) *(f32x4*)(blk + m.sh[0] + 16 * t + g4) = x0[t];
   }
+#pragma unroll 1
+  for (int l = 0; l < L; ++l) {
+    const float* W = lds + m.lw[l];
+    const float* bias = lds + m.lb[l];
+    const int pitch = m.pitch[l], nti = m.ti[l], nto = m.to[l], act = m.act[l];
 #pragma unroll
-  for (int l = 0; l < RG_MAXL; ++l) {
-    if (l < L) {
-      f32x4 (&in)[RG_HT] = (l & 1) ? hb : ha;
-      f32x4 (&out)[RG_HT] = (l & 1) ? ha : hb;
-      const float* W = lds + m.lw[l];
-      const float* bias = lds + m.lb[l];
-      const int pitch = m.pitch[l], nti = m.ti[l], nto = m.to[l], act = m.act[l];
+    for (int ot = 0; ot < RG_HT; ++ot) {
+      hout[ot] = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (ot < nto) {
+        f32x4 acc = *(const f32x4*)(bias + 16 * ot + g4);
+        const float* wrow = W + (16 * ot + r) * pitch + g4;
 #pragma unroll
-      for (int ot = 0; ot < RG_HT; ++ot) {
-        if (ot < nto) {
-          f32x4 acc = *(const f32x4*)(bias + 16 * ot + g4);
-          const float* wrow = W + (16 * ot + r) * pitch + g4;
-#pragma unroll
-          for (int it = 0; it < RG_HT; ++it) {
-            if (it < nti) {
-              const f32x4 a = *(const f32x4*)(wrow + 16 * it);
-              acc = rg_mfma(a.x, in[it].x, acc);
-              acc = rg_mfma(a.y, in[it].y, acc);
-              acc = rg_mfma(a.z, in[it].z, acc);
-              acc = rg_mfma(a.w, in[it].w, acc);
-            }
+        for (int it = 0; it < RG_HT; ++it) {
+          if (it < nti) {
+            const f32x4 a = *(const f32x4*)(wrow + 16 * it);
+            acc = rg_mfma(a.x, hin[it].x, acc);
+            acc = rg_mfma(a.y, hin[it].y, acc);
+            acc = rg_mfma(a.z, hin[it].z, acc);
+            acc = rg_mfma(a.w, hin[it].w, acc);
           }
-          f32x4 y, sd;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            y[v] = act_f(act, acc[v]);
-            sd[v] = act_d(act, acc[v], y[v]);
-          }
-          *(f32x4*)(blk + m.sd[l] + 16 * ot + g4) = sd;
-          if (l < L - 1) *(f32x4*)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
-          out[ot] = y;
         }
+        f32x4 y, sd;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          y[v] = act_f(act, acc[v]);
+          sd[v] = act_d(act, acc[v], y[v]);
+        }
+        *(f32x4*)(blk + m.sd[l] + 16 * ot + g4) = sd;
+        if (l < L - 1) *(f32x4*)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
+        hout[ot] = y;
       }
     }
+#pragma unroll
+    for (int t = 0; t < RG_HT; ++t) hin[t] = hout[t];
   }
-  // backward sweep: g entering layer l lives in (l & 1) ? gb : ga
-  f32x4 ga[RG_HT], gb[RG_HT];
-  const bool odd = ((L - 1) & 1) != 0;
+  // backward sweep: gi = gradient entering layer l
+  f32x4 gi[RG_HT], go[RG_HT];
 #pragma unroll
-  for (int t = 0; t < RG_HT; ++t) {
-    ga[t] = odd ? f32x4{0.f, 0.f, 0.f, 0.f} : lam[t];
-    gb[t] = odd ? lam[t] : f32x4{0.f, 0.f, 0.f, 0.f};
-  }
+  for (int t = 0; t < RG_HT; ++t) gi[t] = lam[t];
+#pragma unroll 1
+  for (int l = L - 1; l >= 0; --l) {
+    const int nti = m.ti[l], nto = m.to[l], pitch = m.pitch[l];
 #pragma unroll
-  for (int l = RG_MAXL - 1; l >= 0; --l) {
-    if (l < L) {
-      f32x4 (&gi)[RG_HT] = (l & 1) ? gb : ga;
-      f32x4 (&go)[RG_HT] = (l & 1) ? ga : gb;
-      const int nti = m.ti[l], nto = m.to[l], pitch = m.pitch[l];
-#pragma unroll
-      for (int ot = 0; ot < RG_HT; ++ot) {
-        if (ot < nto) {
-          float* pd = blk + m.sd[l] + 16 * ot + g4;
-          const f32x4 d = gi[ot] * *(const f32x4*)pd;
-          *(f32x4*)pd = d;
-          gi[ot] = d;
-        }
-      }
-      const float* W = lds + m.lw[l];
-#pragma unroll
-      for (int it = 0; it < RG_HT; ++it) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        if (it < nti) {
-#pragma unroll
-          for (int ot = 0; ot < RG_HT; ++ot) {
-            if (ot < nto) {
-              const float* wc = W + (16 * ot + g4) * pitch + 16 * it + r;
-              acc = rg_mfma(wc[0], gi[ot].x, acc);
-              acc = rg_mfma(wc[pitch], gi[ot].y, acc);
-              acc = rg_mfma(wc[2 * pitch], gi[ot].z, acc);
-              acc = rg_mfma(wc[3 * pitch], gi[ot].w, acc);
-            }
-          }
-        }
-        go[it] = acc;
+    for (int ot = 0; ot < RG_HT; ++ot) {
+      if (ot < nto) {
+        float* pd = blk + m.sd[l] + 16 * ot + g4;
+        const f32x4 d = gi[ot] * *(const f32x4*)pd;
+        *(f32x4*)pd = d;
+        gi[ot] = d;
       }
     }
+    const float* W = lds + m.lw[l];
+#pragma unroll
+    for (int it = 0; it < RG_HT; ++it) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      if (it < nti) {
+#pragma unroll
+        for (int ot = 0; ot < RG_HT; ++ot) {
+          if (ot < nto) {
+            const float* wc = W + (16 * ot + g4) * pitch + 16 * it + r;
+            acc = rg_mfma(wc[0], gi[ot].x, acc);
+            acc = rg_mfma(wc[pitch], gi[ot].y, acc);
+            acc = rg_mfma(wc[2 * pitch], gi[ot].z, acc);
+            acc = rg_mfma(wc[3 * pitch], gi[ot].w, acc);
+          }
+        }
+      }
+      go[it] = acc;
+    }
+#pragma unroll
+    for (int t = 0; t < RG_HT; ++t) gi[t] = go[t];
   }
 #pragma unroll
-  for (int t = 0; t < RG_HT; ++t) gout[t] = gb[t];
+  for (int t = 0; t < RG_HT; ++t) gout[t] = gi[t];
 }
 
 // Parameter part of one trajectory's adjoint state, run by the whole wave for
@@ -606,73 +596,101 @@ template <int MODE>
 __device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, double dt,
                              double atol, double rtol, const float* mu, float* mu_new,
                              double* flag, int lane) {
-  constexpr int NJ = MODE == 0 ? 7 : (MODE == 1 ? 1 : 2);
-  constexpr int CB = 8;  // weight columns per batch of loads
-  int blkj[NJ];
-  float cb[NJ], ce[NJ];
+  // The stage sums are outer products over the stages j:
+  //   k_b[i][o] = sum_j cb_j delta_j[o] h_j[i],  k_e likewise with ce_j,
+  // i.e. for every 16x16 block of (layer input i, output o) a GEMM with
+  // K = 7 stages (padded to 8): two v_mfma_f32_16x16x4f32 each, A[i][j] =
+  // h_j[i], B[j][o] = c_j delta_j[o].  Lane (r, g) supplies stage j = 4q + g
+  // of MFMA step q; the accumulator gives it output o = r and inputs
+  // i = 4g + v, i.e. flat entries W[o, i] at i*out + o (coalesced over r).
+  const int r = lane & 15, g = lane >> 4;
+  int blk[2];
+  float cb[2], ce[2];
+  bool use[2];
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
+  for (int q = 0; q < 2; ++q) {
+    const int j = 4 * q + g;
     if (MODE == 0) {
-      blkj[j] = j == 0 ? p0 : (j == 6 ? 7 - p0 : j);
-      cb[j] = j < 6 ? cA[6][j] : 0.f;
-      ce[j] = cBT[j];
+      use[q] = j < 7;
+      blk[q] = j == 0 ? p0 : (j == 6 ? 7 - p0 : (j < 7 ? j : 0));
+      cb[q] = j < 6 ? cA[6][j] : 0.f;
+      ce[q] = j < 7 ? cBT[j] : 0.f;
+    } else if (MODE == 1) {
+      use[q] = j == 0;
+      blk[q] = p0;
+      cb[q] = 1.f;   // g_0
+      ce[q] = 0.f;
     } else {
-      blkj[j] = j == 0 ? p0 : 7 - p0;
-      cb[j] = j == 0 ? 1.f : 0.f;   // g_0
-      ce[j] = j == 0 ? -1.f : 1.f;  // g_1 - g_0
+      use[q] = j < 2;
+      blk[q] = j == 0 ? p0 : 7 - p0;
+      cb[q] = 0.f;
+      ce[q] = j == 0 ? -1.f : 1.f;  // g_1 - g_0
     }
   }
   const float dtf = (float)dt, atf = (float)atol, rtf = (float)rtol;
   float acc = 0.f, fl = 0.f;
   for (int l = 0; l < m.L; ++l) {
-    const int in = m.in[l], out = m.out[l];
+    const int in = m.in[l], out = m.out[l], nti = m.ti[l], nto = m.to[l];
     const int64_t w0 = m.w0[l];
+    // bias entries (layer input 1): lanes over o, stages summed on the VALU
     for (int o0 = 0; o0 < out; o0 += 64) {
       const int o = o0 + lane;
-      const bool ok = o < out;
-      float bd[NJ], ed[NJ];
-      float kb = 0.f, ke = 0.f;
+      if (o < out) {
+        float kb = 0.f, ke = 0.f;
+        if (MODE == 0) {
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const float d = ok ? sbs[blkj[j] * m.sv + m.sd[l] + o] : 0.f;
-        bd[j] = cb[j] * d;
-        ed[j] = ce[j] * d;
-        kb += bd[j];
-        ke += ed[j];
-      }
-      if (ok) {  // bias entry (layer input 1)
+          for (int j = 0; j < 7; ++j) {
+            const int bj = j == 0 ? p0 : (j == 6 ? 7 - p0 : j);
+            const float d = sbs[bj * m.sv + m.sd[l] + o];
+            kb += (j < 6 ? cA[6][j] : 0.f) * d;
+            ke += cBT[j] * d;
+          }
+        } else if (MODE == 1) {
+          kb = sbs[p0 * m.sv + m.sd[l] + o];
+        } else {
+          ke = sbs[(7 - p0) * m.sv + m.sd[l] + o] - sbs[p0 * m.sv + m.sd[l] + o];
+        }
         const int64_t idx = m.b0[l] + o;
         rg_mu_entry<MODE>(idx, kb, ke, dtf, atf, rtf, mu, mu_new, mu[idx], acc, fl);
       }
-      for (int i0 = 0; i0 < in; i0 += 64) {
-        // this block's layer inputs h_j[i0 + lane]; entry i is broadcast by readlane
-        float hl[NJ];
+    }
+    // B operands of every output block: c_j delta_j[16 ot + r] for this lane's stages
+    float db[RG_HT][2], de[RG_HT][2];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j)
-          hl[j] = (i0 + lane < in) ? sbs[blkj[j] * m.sv + m.sh[l] + i0 + lane] : 0.f;
-        const int ni = min(64, in - i0);
-        for (int ii = 0; ii < ni; ii += CB) {
-          float m0v[CB];
+    for (int ot = 0; ot < RG_HT; ++ot)
 #pragma unroll
-          for (int k = 0; k < CB; ++k) {
-            const int64_t idx = w0 + (int64_t)(i0 + ii + k) * out + o;
-            m0v[k] = (ok && ii + k < ni) ? mu[idx] : 0.f;
+      for (int q = 0; q < 2; ++q) {
+        float d = 0.f;
+        if (ot < nto && use[q]) d = sbs[blk[q] * m.sv + m.sd[l] + 16 * ot + r];
+        db[ot][q] = cb[q] * d;
+        de[ot][q] = ce[q] * d;
+      }
+    for (int it = 0; it < nti; ++it) {
+      float ha[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        ha[q] = use[q] ? sbs[blk[q] * m.sv + m.sh[l] + 16 * it + r] : 0.f;
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        if (ot < nto) {
+          const int o = 16 * ot + r;
+          float m0v[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * it + 4 * g + v;
+            m0v[v] = (o < out && i < in) ? mu[w0 + (int64_t)i * out + o] : 0.f;
           }
+          f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb;
+          kb = rg_mfma(ha[0], db[ot][0], kb);
+          kb = rg_mfma(ha[1], db[ot][1], kb);
+          ke = rg_mfma(ha[0], de[ot][0], ke);
+          ke = rg_mfma(ha[1], de[ot][1], ke);
 #pragma unroll
-          for (int k = 0; k < CB; ++k) {
-            if (ii + k < ni) {
-              float kbw = 0.f, kew = 0.f;
-#pragma unroll
-              for (int j = 0; j < NJ; ++j) {
-                const float hj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hl[j]), ii + k));
-                kbw += bd[j] * hj;
-                kew += ed[j] * hj;
-              }
-              if (ok) {
-                const int64_t idx = w0 + (int64_t)(i0 + ii + k) * out + o;
-                rg_mu_entry<MODE>(idx, kbw, kew, dtf, atf, rtf, mu, mu_new, m0v[k], acc, fl);
-              }
-            }
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * it + 4 * g + v;
+            if (o < out && i < in)
+              rg_mu_entry<MODE>(w0 + (int64_t)i * out + o, kb[v], ke[v], dtf, atf, rtf, mu, mu_new,
+                                m0v[v], acc, fl);
           }
         }
       }
