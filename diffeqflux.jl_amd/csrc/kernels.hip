@@ -842,6 +842,21 @@ __device__ __forceinline__ void epi_apply(const EpiWgrad& e, f32x16 (&acc)[MR][M
   EPI_LOOP_END
 }
 
+// split-K partial products of the skinny stage GEMMs: split s stores its FP32
+// tile at ws[s * pstride + r * ld + c]; the stage kernels below sum the splits
+// in split order and run the per-element stage work
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiPartial& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& geo, int lane, const Ctl*, int split,
+                                          unsigned char*) {
+  const int rb = geo.rb, cb = geo.cb;
+  float* w = e.ws + (int64_t)split * e.pstride;
+  EPI_LOOP_BEGIN
+#pragma unroll
+    for (int p = 0; p < 4; ++p) w[(int64_t)(r0 + p) * e.ld + c] = acc[mi][mj][q * 4 + p];
+  EPI_LOOP_END
+}
+
 // ---------------------------------------------------------------- GEMM
 // C[r][c] = sum_k A[r][k] * B[c][k]; rows = batch (or input feature for
 // weight gradients), cols = output feature.  WRxWC waves per workgroup, each
@@ -1007,8 +1022,9 @@ __global__ void __launch_bounds__(WR * WC * 64)
 }
 
 // tile configurations: {BR, BC, WR, WC}
-static const int kCfg[4][4] = {{128, 128, 2, 2}, {128, 64, 2, 2}, {64, 64, 2, 2}, {32, 64, 1, 2}};
-int gemm_cfg_count() { return 4; }
+static const int kCfg[6][4] = {{128, 128, 2, 2}, {128, 64, 2, 2}, {64, 64, 2, 2}, {32, 64, 1, 2},
+                               {128, 128, 4, 2}, {128, 128, 2, 4}};
+int gemm_cfg_count() { return 6; }
 int gemm_cfg_tile_rows(int cfg) { return kCfg[cfg][0]; }
 int gemm_cfg_tile_cols(int cfg) { return kCfg[cfg][1]; }
 
@@ -1027,8 +1043,9 @@ static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, i
   // the stage epilogues (per-element Tsit5 / adjoint stage work) are planned
   // with at most 64 output columns per tile; their 128x128 form is not built
   constexpr bool kStage = std::is_same<Epi, EpiFwdStage>::value ||
-                         std::is_same<Epi, EpiBwdLast>::value || std::is_same<Epi, EpiCnfStage>::value;
-  if (kStage && cfg == 0) cfg = 1;
+                         std::is_same<Epi, EpiBwdLast>::value || std::is_same<Epi, EpiCnfStage>::value ||
+                         std::is_same<Epi, EpiPartial>::value;
+  if (kStage && (cfg == 0 || cfg >= 4)) cfg = 1;
   switch (cfg) {
     case 0:
       if constexpr (!kStage)
@@ -1036,7 +1053,15 @@ static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, i
       break;
     case 1: hipLaunchKernelGGL((gemm_tn<PP, 128, 64, 2, 2, Depth<PP, 128, 64>::value, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
     case 2: hipLaunchKernelGGL((gemm_tn<PP, 64, 64, 2, 2, Depth<PP, 64, 64>::value, Epi>), dim3(nb), dim3(256), 0, s, ga, e, ctl, gate); break;
-    default: hipLaunchKernelGGL((gemm_tn<PP, 32, 64, 1, 2, Depth<PP, 32, 64>::value, Epi>), dim3(nb), dim3(128), 0, s, ga, e, ctl, gate); break;
+    case 3: hipLaunchKernelGGL((gemm_tn<PP, 32, 64, 1, 2, Depth<PP, 32, 64>::value, Epi>), dim3(nb), dim3(128), 0, s, ga, e, ctl, gate); break;
+    case 4:
+      if constexpr (!kStage)
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 4, 2, Depth<PP, 128, 128>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
+      break;
+    default:
+      if constexpr (!kStage)
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 4, Depth<PP, 128, 128>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
+      break;
   }
 }
 void launch_gemm_hidden(int prec, int cfg, const GemmArgs& g, const EpiHidden& e,
@@ -1049,8 +1074,261 @@ void launch_gemm_cnfstage(int prec, int cfg, const GemmArgs& g, const EpiCnfStag
                           const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_bwdlast(int prec, int cfg, const GemmArgs& g, const EpiBwdLast& e,
                          const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
+void launch_gemm_partial(int prec, int cfg, const GemmArgs& g, const EpiPartial& e,
+                         const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
                        const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
+
+// ---------------------------------------------------------------- stage kernels
+// Per-element stage work of the skinny last-layer GEMMs, run over the summed
+// split-K partials by the whole chip: a 256-thread block takes tiles of
+// 16 rows x 64 columns (thread: column tid % 64, rows 4 (tid / 64) + j).  The
+// arithmetic per element is that of the fused epilogues (EpiFwdStage,
+// EpiBwdLast); loads are issued for all four rows before any store.
+constexpr int ST_ROWS = 16;
+constexpr int ST_MAX_BLOCKS = 4096;
+int stage_grid(int rows, int cols) {
+  const int t = (rows / ST_ROWS) * (cols / 64);
+  return t < ST_MAX_BLOCKS ? t : ST_MAX_BLOCKS;
+}
+
+__device__ __forceinline__ float sum_splits(const float* ws, int ks, int64_t pstride, int64_t o) {
+  float v = ws[o];
+  for (int s = 1; s < ks; ++s) v += ws[s * pstride + o];
+  return v;
+}
+
+template <class PP>
+__global__ void __launch_bounds__(256)
+    k_fwd_stage(EpiFwdStage e, const float* __restrict__ ws, int ks, int64_t pstride, int ldw,
+                int tc, int ntile, const Ctl* __restrict__ ctl, int gate) {
+  if (!gate_ok(ctl, gate)) return;
+  using T = typename PP::T;
+  const int tid = threadIdx.x, cl = tid & 63, rq = tid >> 6;
+  const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
+  float* ent = e.tape + n * e.ent;
+  const float dt = (float)ctl->dt;
+  const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
+  const int s = e.stage;
+  T* X0 = (T*)e.X0;
+  double sum = 0.0, flag = 0.0;
+  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const int r0 = (tile / tc) * ST_ROWS + 4 * rq, c = (tile % tc) * 64 + cl;
+    if (c >= e.D) continue;
+    const float bias = e.bias[c];
+    float kv[4], uv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + j;
+      const int64_t idx = (int64_t)r * e.D + c;
+      const float k = act_f(e.act, sum_splits(ws, ks, pstride, (int64_t)r * ldw + c) + bias);
+      kv[j] = k;
+      uv[j] = 0.f;
+      const bool live = r < e.B;
+      if (e.mode == SM_STEP) {
+        if (s < 6) {
+          float sm = cA[s + 1][s] * k;
+#pragma unroll
+          for (int i = 0; i < 5; ++i)
+            if (i < s) sm += cA[s + 1][i] * ent[(1 + i) * e.sv + idx];
+          uv[j] = ent[idx] + dt * sm;
+        } else {
+          float ut = cBT[6] * k;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) ut += cBT[i] * ent[(1 + i) * e.sv + idx];
+          ut *= dt;
+          const float up = ent[idx], un = ent[e.ent + idx];
+          const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
+          if (live) {
+            const double a = (double)(ut / sk);
+            sum += a * a;
+            if (isnan(un)) flag = 1.0;
+          }
+        }
+      } else if (e.mode == SM_INIT_F0) {
+        const float sk = atol + fabsf(ent[idx]) * rtol;
+        if (live) { const double a = (double)(k / sk); sum += a * a; }
+      } else if (e.mode == SM_INIT_F1) {
+        const float f0 = ent[e.sv + idx];
+        const float sk = atol + fabsf(ent[idx]) * rtol;
+        if (live) {
+          const double a = (double)((k - f0) / sk);
+          sum += a * a;
+          if (k != f0) flag = 1.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + j;
+      const int64_t idx = (int64_t)r * e.D + c;
+      const float k = kv[j];
+      if (e.mode == SM_STEP) {
+        ent[(1 + s) * e.sv + idx] = k;
+        if (s < 6) {
+          put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, map_f(e.map, uv[j]));
+          if (s == 5) ent[e.ent + idx] = uv[j];           // candidate tape[n+1].uprev
+        } else {
+          ent[e.ent + e.sv + idx] = k;                     // FSAL: tape[n+1].k1
+        }
+      } else if (e.mode == SM_INIT_F0) {
+        ent[e.sv + idx] = k;
+      } else {
+        e.scratch[idx] = k;
+      }
+    }
+  }
+  if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
+}
+
+template <class PP>
+__global__ void __launch_bounds__(256)
+    k_bwd_stage(EpiBwdLast e, const float* __restrict__ ws, int ks, int64_t pstride, int ldw,
+                int tc, int ntile, const Ctl* __restrict__ ctl, int gate) {
+  if (!gate_ok(ctl, gate)) return;
+  using T = typename PP::T;
+  __shared__ float lt[2][64][ST_ROWS + 1];
+  const int tid = threadIdx.x, cl = tid & 63, rq = tid >> 6;
+  const float dt = (float)ctl->dt;
+  const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
+  const int s = e.stage;
+  const bool stage_out = e.mode == SM_STEP && s < 6;
+  T* L = (T*)e.L;
+  T* LT = (T*)e.LT;
+  T* X0 = (T*)e.X0;
+  T* X0T = (T*)e.X0T;
+  // interpolation data of the next stage's time
+  int tj = 0, ex = 1;
+  float tdt = 0.f, w[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (stage_out) {
+    tj = ctl->st_step[s + 1];
+    ex = ctl->st_exact[s + 1];
+    tdt = ctl->st_dt[s + 1];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) w[i] = ctl->st_w[s + 1][i];
+  }
+  const float* te = e.tape + (int64_t)tj * e.ent;
+  double sum = 0.0, flag = 0.0;
+  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const int tr0 = (tile / tc) * ST_ROWS, tcol = (tile % tc) * 64;
+    const int r0 = tr0 + 4 * rq, c = tcol + cl;
+    const bool cv = c < e.D;
+    float kv[4], lsv[4], uv[4], xv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      kv[j] = lsv[j] = uv[j] = xv[j] = 0.f;
+      const int r = r0 + j;
+      const int64_t idx = (int64_t)r * e.D + c;
+      if (!cv) continue;
+      float g = sum_splits(ws, ks, pstride, (int64_t)r * ldw + c);
+      if (e.map == 1) { const float u = e.Ucur[idx]; g *= 3.f * u * u; }
+      const float k = -g;
+      kv[j] = e.mode == SM_PLAIN ? g : k;
+      const bool live = r < e.B;
+      if (e.mode == SM_STEP) {
+        if (s < 6) {
+          float sm = cA[s + 1][s] * k;
+#pragma unroll
+          for (int i = 0; i < 5; ++i)
+            if (i < s) sm += cA[s + 1][i] * e.kl[i * e.sv + idx];
+          lsv[j] = e.lam[idx] + dt * sm;
+          float u = te[idx];
+          if (!ex) {
+            float sm2 = 0.f;
+#pragma unroll
+            for (int i = 0; i < 7; ++i) sm2 += w[i] * te[(1 + i) * e.sv + idx];
+            u += tdt * sm2;
+          }
+          uv[j] = u;
+          xv[j] = map_f(e.map, u);
+        } else {
+          float ut = cBT[6] * k;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) ut += cBT[i] * e.kl[i * e.sv + idx];
+          ut *= dt;
+          const float lp = e.lam[idx], ln = e.lam_new[idx];
+          const float sk = atol + fmaxf(fabsf(lp), fabsf(ln)) * rtol;
+          if (live) {
+            const double a = (double)(ut / sk);
+            sum += a * a;
+            if (isnan(ln)) flag = 1.0;
+          }
+        }
+      } else if (e.mode == SM_INIT_F0) {
+        const float sk = atol + fabsf(e.lam[idx]) * rtol;
+        if (live) { const double a = (double)(k / sk); sum += a * a; }
+      } else if (e.mode == SM_INIT_F1) {
+        const float f0 = e.kl[idx];
+        const float sk = atol + fabsf(e.lam[idx]) * rtol;
+        if (live) {
+          const double a = (double)((k - f0) / sk);
+          sum += a * a;
+          if (k != f0) flag = 1.0;
+        }
+      }
+    }
+    if (cv) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t idx = (int64_t)(r0 + j) * e.D + c;
+        if (e.mode == SM_PLAIN) {
+          e.plain_out[idx] = kv[j];
+        } else if (e.mode == SM_INIT_F0) {
+          e.kl[idx] = kv[j];
+        } else if (e.mode == SM_INIT_F1) {
+          e.kl[7 * e.sv + idx] = kv[j];
+        } else {
+          e.kl[s * e.sv + idx] = kv[j];
+          if (s < 6) {
+            if (s == 5) e.lam_new[idx] = lsv[j];
+            e.Ucur[idx] = uv[j];
+          }
+        }
+      }
+    }
+    if (stage_out) {
+      // next-stage inputs lambda_s and x = map(u(t_s)): row-major (padded
+      // columns get zeros) and transposed through LDS
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = r0 + j;
+        put<PP>(L + (int64_t)r * e.ldl + c, e.pl, lsv[j]);
+        put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, xv[j]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        lt[0][cl][4 * rq + j] = lsv[j];
+        lt[1][cl][4 * rq + j] = xv[j];
+      }
+      __syncthreads();
+      const int cc = tid >> 2, rr = 4 * (tid & 3);
+      const int64_t col = tcol + cc;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = tr0 + rr + j;
+        put<PP>(LT + col * e.ldlt + r, e.plt, lt[0][cc][rr + j]);
+        put<PP>(X0T + col * e.ldxt + r, e.pxt, lt[1][cc][rr + j]);
+      }
+    }
+  }
+  if (e.part) block_reduce_store(sum, flag, e.part, blockIdx.x);
+}
+
+void launch_fwd_stage_kernel(int prec, const EpiFwdStage& e, const float* ws, int ks,
+                             int64_t pstride, int ldw, int rows, int cols, const Ctl* ctl,
+                             int gate, hipStream_t s) {
+  const int tc = cols / 64, nt = (rows / ST_ROWS) * tc;
+  DISPATCH_PREC(prec, hipLaunchKernelGGL(k_fwd_stage<PP>, dim3(stage_grid(rows, cols)), dim3(256),
+                                         0, s, e, ws, ks, pstride, ldw, tc, nt, ctl, gate));
+}
+void launch_bwd_stage_kernel(int prec, const EpiBwdLast& e, const float* ws, int ks,
+                             int64_t pstride, int ldw, int rows, int cols, const Ctl* ctl,
+                             int gate, hipStream_t s) {
+  const int tc = cols / 64, nt = (rows / ST_ROWS) * tc;
+  DISPATCH_PREC(prec, hipLaunchKernelGGL(k_bwd_stage<PP>, dim3(stage_grid(rows, cols)), dim3(256),
+                                         0, s, e, ws, ks, pstride, ldw, tc, nt, ctl, gate));
+}
 
 // ---------------------------------------------------------------- elementwise
 constexpr int EW_THREADS = 256;

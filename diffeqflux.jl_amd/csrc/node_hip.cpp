@@ -205,10 +205,12 @@ void dfree(node_handle* h, void* p) {
 
 // Tile configuration with >= 240 workgroups if one exists (largest tile
 // first), else the one with the most workgroups.  min_cfg > 0 skips the
-// 128x128 tile (the stage epilogues use at most 64 columns per tile).
+// 128x128 tiles (the stage epilogues use at most 64 columns per tile).
+bool cfg_allowed(int c, int min_cfg) { return c >= min_cfg && (min_cfg == 0 || gemm_cfg_tile_cols(c) <= 64); }
 GemmPlan plan(int rows, int cols, int min_cfg = 0) {
   int best = -1, best_tiles = -1;
   for (int c = min_cfg; c < gemm_cfg_count(); ++c) {
+    if (!cfg_allowed(c, min_cfg)) continue;
     const int br = gemm_cfg_tile_rows(c), bc = gemm_cfg_tile_cols(c);
     if (rows % br || cols % bc) continue;
     const int t = (rows / br) * (cols / bc);
@@ -298,6 +300,17 @@ GemmPlan plan_role(node_handle* h, int role, int layer, int rows, int cols, int 
   return plan(rows, cols, min_cfg);
 }
 
+// The skinny stage GEMMs (last forward layer, layer-1 data gradient) run
+// their per-element stage work in a separate elementwise kernel over split-K
+// partials when the fused epilogue would have fewer than 2048 tiles of 32x64
+// (its ~14 dependent loads per element are then memory-latency bound); larger
+// batches keep the fused epilogue, which saves the partials' round trip.
+bool stage_split(const node_handle* h, int role) {
+  if (h->cnf && role == PR_FWD_STAGE) return false;
+  const int cols = role == PR_BWD_LAST ? h->Dp : h->layers[h->L - 1].outp;
+  return (int64_t)(h->Bp / 32) * (cols / 64) < 2048;
+}
+
 // split-K factor of a stage-epilogue role (1 without a fixup workspace)
 int stage_ks(node_handle* h, int role, int layer) {
   if (!h->fix_ws) return 1;
@@ -333,7 +346,9 @@ void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* par
   GemmPlan p = plan_role(h, PR_FWD_STAGE, l, h->Bp, ly.outp, 1);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   g.ksplit = stage_ks(h, PR_FWD_STAGE, l);
-  g.ws = h->fix_ws; g.cnt = h->fix_cnt;
+  const bool split = stage_split(h, PR_FWD_STAGE);
+  g.ws = split ? nullptr : h->fix_ws;
+  g.cnt = split ? nullptr : h->fix_cnt;
   EpiFwdStage e{};
   e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
   e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D; e.sv = (int64_t)h->Bp * h->D;
@@ -341,7 +356,14 @@ void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* par
   e.scratch = h->scratch; e.part = part;
   e.px = (int64_t)h->Bp * h->Dp;
   ProfScope ps(h, PR_FWD_STAGE, l);
-  launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  if (split) {
+    const EpiPartial ep{h->fix_ws, ly.outp, (int64_t)h->Bp * ly.outp};
+    launch_gemm_partial(h->prec, p.cfg, g, ep, h->ctl, gate, h->stream);
+    launch_fwd_stage_kernel(h->prec, e, h->fix_ws, g.ksplit, ep.pstride, ly.outp, h->Bp, ly.outp,
+                            h->ctl, gate, h->stream);
+  } else {
+    launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  }
 }
 
 // One batched right-hand side: X0 -> layers -> last-layer epilogue.
@@ -357,6 +379,7 @@ void rhs_eval(node_handle* h, int stage, int mode, int gate, double* part) {
 int rhs_last_tiles(node_handle* h) {
   if (h->cnf) return cnf_last_tiles(h);
   const Layer& ly = h->layers[h->L - 1];
+  if (stage_split(h, PR_FWD_STAGE)) return stage_grid(h->Bp, ly.outp);
   GemmPlan p = plan_role(h, PR_FWD_STAGE, h->L - 1, h->Bp, ly.outp, 1);
   return p.rt * p.ct;
 }
@@ -417,7 +440,9 @@ void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part
   GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 2);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   g.ksplit = stage_ks(h, PR_BWD_LAST, 0);
-  g.ws = h->fix_ws; g.cnt = h->fix_cnt;
+  const bool split = stage_split(h, PR_BWD_LAST);
+  g.ws = split ? nullptr : h->fix_ws;
+  g.cnt = split ? nullptr : h->fix_cnt;
   EpiBwdLast e{};
   e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
   e.kl = h->kl; e.sv = (int64_t)h->Bp * h->D;
@@ -428,10 +453,18 @@ void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part
   e.part = part; e.plain_out = plain_out;
   e.pl = e.plt = e.px = e.pxt = (int64_t)h->Bp * h->Dp;
   ProfScope ps(h, PR_BWD_LAST, 0);
-  launch_gemm_bwdlast(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  if (split) {
+    const EpiPartial ep{h->fix_ws, h->Dp, (int64_t)h->Bp * h->Dp};
+    launch_gemm_partial(h->prec, p.cfg, g, ep, h->ctl, gate, h->stream);
+    launch_bwd_stage_kernel(h->prec, e, h->fix_ws, g.ksplit, ep.pstride, h->Dp, h->Bp, h->Dp,
+                            h->ctl, gate, h->stream);
+  } else {
+    launch_gemm_bwdlast(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  }
 }
 
 int vjp_last_tiles(node_handle* h) {
+  if (stage_split(h, PR_BWD_LAST)) return stage_grid(h->Bp, h->Dp);
   GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, h->Dp, 2);
   return p.rt * p.ct;
 }
@@ -566,6 +599,7 @@ int autotune_impl(node_handle* h) {
     int bc = -1, bks = 1;
     const int kt = (int)((int64_t)K * h->esz / 128);
     for (int c = role == PR_BWD_LAST ? 2 : 1; c < gemm_cfg_count(); ++c) {
+      if (!cfg_allowed(c, 1)) continue;
       if (h->Bp % gemm_cfg_tile_rows(c) || cols % gemm_cfg_tile_cols(c)) continue;
       for (int ks = 1; ks <= 8; ks *= 2) {
         if (kt % ks) continue;

@@ -109,6 +109,11 @@ struct EpiBwdLast {
   int64_t pl, plt, px, pxt;                // plane strides of L / LT / X0 / X0T
 };
 
+// split-K partial products of a skinny stage GEMM (FP32 [ksplit][rows][ld])
+struct EpiPartial {
+  float* ws; int ld; int64_t pstride;
+};
+
 // weight gradient (K = batch): fold into the mu accumulators
 enum WgradFlags { WG_INIT = 1, WG_B = 2, WG_E = 4, WG_STORE_CUR = 8,
                   WG_STORE_NEXT = 16, WG_STORE_PTR = 32 };
@@ -153,6 +158,17 @@ void launch_gemm_bwdlast(int prec, int cfg, const GemmArgs& g, const EpiBwdLast&
                          const Ctl* ctl, int gate, hipStream_t s);
 void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
                        const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_partial(int prec, int cfg, const GemmArgs& g, const EpiPartial& e,
+                         const Ctl* ctl, int gate, hipStream_t s);
+// elementwise stage kernels over summed split-K partials (rows x cols, cols a
+// multiple of 64, rows a multiple of 16); one partial-sum slot per block
+int stage_grid(int rows, int cols);
+void launch_fwd_stage_kernel(int prec, const EpiFwdStage& e, const float* ws, int ks,
+                             int64_t pstride, int ldw, int rows, int cols, const Ctl* ctl,
+                             int gate, hipStream_t s);
+void launch_bwd_stage_kernel(int prec, const EpiBwdLast& e, const float* ws, int ks,
+                             int64_t pstride, int ldw, int rows, int cols, const Ctl* ctl,
+                             int gate, hipStream_t s);
 int gemm_cfg_tile_rows(int cfg);
 int gemm_cfg_tile_cols(int cfg);
 int gemm_cfg_count();
