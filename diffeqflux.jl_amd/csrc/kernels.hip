@@ -390,6 +390,30 @@ __device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR]
         }
       }
     }
+  if (e.bsum) {
+    // column sums of the tile (fixed order: registers, lane halves, waves)
+    constexpr int WR = BR / (MR * 32);
+    float* red = (float*)lds;  // [WR][BC]
+    const int wr = g.lr0 / (BR / WR);
+    __syncthreads();
+#pragma unroll
+    for (int mj = 0; mj < MC; ++mj) {
+      float cs = 0.f;
+#pragma unroll
+      for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) cs += acc[mi][mj][v];
+      cs += __shfl_xor(cs, 32);
+      if (lane < 32) red[wr * BC + g.lc0 + mj * 32 + lane] = cs;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < BC; i += NT) {
+      float sm = 0.f;
+#pragma unroll
+      for (int w = 0; w < WR; ++w) sm += red[w * BC + i];
+      e.bsum[(int64_t)(g.row0 / BR) * e.ldo + g.col0 + i] = sm;
+    }
+  }
   tile_write<T, PP::SPLIT, BR, BC, NT>(lds, acc, g, lane, (T*)e.Dl, e.ldo, e.pd, (T*)e.DlT,
                                        e.ldt, e.pdt);
 }
@@ -1082,10 +1106,11 @@ void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
 // ---------------------------------------------------------------- stage kernels
 // Per-element stage work of the skinny last-layer GEMMs, run over the summed
 // split-K partials by the whole chip: a 256-thread block takes tiles of
-// 16 rows x 64 columns (thread: column tid % 64, rows 4 (tid / 64) + j).  The
-// arithmetic per element is that of the fused epilogues (EpiFwdStage,
-// EpiBwdLast); loads are issued for all four rows before any store.
-constexpr int ST_ROWS = 16;
+// ST_ROWS rows x 64 columns (thread: column tid % 64, rows RPT (tid / 64) + j).
+// The arithmetic per element is that of the fused epilogues (EpiFwdStage,
+// EpiBwdLast); a thread issues the loads of all its rows before any store.
+constexpr int ST_ROWS = 4;
+constexpr int RPT = ST_ROWS / 4;
 constexpr int ST_MAX_BLOCKS = 4096;
 int stage_grid(int rows, int cols) {
   const int t = (rows / ST_ROWS) * (cols / 64);
@@ -1113,12 +1138,12 @@ __global__ void __launch_bounds__(256)
   T* X0 = (T*)e.X0;
   double sum = 0.0, flag = 0.0;
   for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-    const int r0 = (tile / tc) * ST_ROWS + 4 * rq, c = (tile % tc) * 64 + cl;
+    const int r0 = (tile / tc) * ST_ROWS + RPT * rq, c = (tile % tc) * 64 + cl;
     if (c >= e.D) continue;
     const float bias = e.bias[c];
-    float kv[4], uv[4];
+    float kv[RPT], uv[RPT];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < RPT; ++j) {
       const int r = r0 + j;
       const int64_t idx = (int64_t)r * e.D + c;
       const float k = act_f(e.act, sum_splits(ws, ks, pstride, (int64_t)r * ldw + c) + bias);
@@ -1159,7 +1184,7 @@ __global__ void __launch_bounds__(256)
       }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < RPT; ++j) {
       const int r = r0 + j;
       const int64_t idx = (int64_t)r * e.D + c;
       const float k = kv[j];
@@ -1211,11 +1236,11 @@ __global__ void __launch_bounds__(256)
   double sum = 0.0, flag = 0.0;
   for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const int tr0 = (tile / tc) * ST_ROWS, tcol = (tile % tc) * 64;
-    const int r0 = tr0 + 4 * rq, c = tcol + cl;
+    const int r0 = tr0 + RPT * rq, c = tcol + cl;
     const bool cv = c < e.D;
-    float kv[4], lsv[4], uv[4], xv[4];
+    float kv[RPT], lsv[RPT], uv[RPT], xv[RPT];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < RPT; ++j) {
       kv[j] = lsv[j] = uv[j] = xv[j] = 0.f;
       const int r = r0 + j;
       const int64_t idx = (int64_t)r * e.D + c;
@@ -1269,7 +1294,7 @@ __global__ void __launch_bounds__(256)
     }
     if (cv) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < RPT; ++j) {
         const int64_t idx = (int64_t)(r0 + j) * e.D + c;
         if (e.mode == SM_PLAIN) {
           e.plain_out[idx] = kv[j];
@@ -1290,22 +1315,23 @@ __global__ void __launch_bounds__(256)
       // next-stage inputs lambda_s and x = map(u(t_s)): row-major (padded
       // columns get zeros) and transposed through LDS
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < RPT; ++j) {
         const int r = r0 + j;
         put<PP>(L + (int64_t)r * e.ldl + c, e.pl, lsv[j]);
         put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, xv[j]);
       }
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        lt[0][cl][4 * rq + j] = lsv[j];
-        lt[1][cl][4 * rq + j] = xv[j];
+      for (int j = 0; j < RPT; ++j) {
+        lt[0][cl][RPT * rq + j] = lsv[j];
+        lt[1][cl][RPT * rq + j] = xv[j];
       }
       __syncthreads();
-      const int cc = tid >> 2, rr = 4 * (tid & 3);
+      // transposed: thread -> column tid / 4, rows RPT (tid % 4) + j
+      const int cc = tid >> 2, rr = RPT * (tid & 3);
       const int64_t col = tcol + cc;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < RPT; ++j) {
         const int r = tr0 + rr + j;
         put<PP>(LT + col * e.ldlt + r, e.plt, lt[0][cc][rr + j]);
         put<PP>(X0T + col * e.ldxt + r, e.pxt, lt[1][cc][rr + j]);
@@ -1872,9 +1898,14 @@ __global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_b
     if (wave >= fa.bias_rows) return;
     int l = 0, row = wave;
     while (l < fa.nl - 1 && row >= fa.outs[l]) { row -= fa.outs[l]; ++l; }
-    const T* src = (const T*)fa.dT[l] + (int64_t)row * fa.ldt;
     float s = 0.f;
-    for (int b = lane; b < fa.Bp; b += 64) s += get<PP>(src + b, fa.dplane[l]);
+    if (fa.bpart[l]) {
+      const float* bp = fa.bpart[l] + row;
+      for (int t = lane; t < fa.bpart_rt[l]; t += 64) s += bp[(int64_t)t * fa.bpart_ld[l]];
+    } else {
+      const T* src = (const T*)fa.dT[l] + (int64_t)row * fa.ldt;
+      for (int b = lane; b < fa.Bp; b += 64) s += get<PP>(src + b, fa.dplane[l]);
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) fold_one(e, gcur, gnext, fa.b0s[l] + row, -s);

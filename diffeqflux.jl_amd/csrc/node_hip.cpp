@@ -44,6 +44,10 @@ struct Layer {
   float* wpart = nullptr; // split-K partials [wg_ks][in*out] when wg_ks > 1
   int64_t wpart_cap = 0;  // floats allocated for wpart
   int wg_ks0 = 1;         // heuristic split-K (autotuning off)
+  // column sums of this layer's output gradient per row tile of the data
+  // gradient GEMM that produces it ([Bp/32][outp]; hidden layers only)
+  float* bpart = nullptr;
+  int bpart_rt = 0;
 };
 
 struct GemmPlan {
@@ -385,7 +389,7 @@ int rhs_last_tiles(node_handle* h) {
 }
 
 // data gradient of layer l >= 1: delta_{l-1} = (delta_l W_l) * act'
-void launch_bwd_data(node_handle* h, int l, int gate) {
+void launch_bwd_data(node_handle* h, int l, int gate, bool bias_sums = true) {
   const Layer& ly = h->layers[l];
   GemmArgs g{};
   g.A = l == h->L - 1 ? h->Lm : h->Dl[l];
@@ -394,7 +398,10 @@ void launch_bwd_data(node_handle* h, int l, int gate) {
   GemmPlan p = plan_role(h, PR_BWD_DATA, l, h->Bp, ly.inp, 0);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   const int64_t pd = (int64_t)h->Bp * ly.inp;
-  EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd};
+  Layer& lo = h->layers[l - 1];
+  float* bsum = bias_sums ? lo.bpart : nullptr;
+  if (bsum) lo.bpart_rt = p.rt;
+  EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd, bsum};
   ProfScope ps(h, PR_BWD_DATA, l);
   launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
 }
@@ -479,6 +486,9 @@ void launch_fold_all(node_handle* h, int gate, const EpiWgrad& wt) {
     fa.outs[l] = ly.out;
     fa.b0s[l] = ly.b0;
     fa.bias_rows += ly.out;
+    fa.bpart[l] = ly.bpart_rt > 0 ? ly.bpart : nullptr;
+    fa.bpart_rt[l] = ly.bpart_rt;
+    fa.bpart_ld[l] = ly.outp;
     if (ly.wg_ks > 1) {
       fa.sk_w0[fa.nsk] = ly.w0;
       fa.sk_n[fa.nsk] = (int64_t)ly.in * ly.out;
@@ -676,6 +686,10 @@ int ensure_adjoint(node_handle* h) {
   if (int rc = dalloc_t(h, &h->acce, h->P * 4)) return rc;
   if (int rc = dalloc_t(h, &h->gf0, h->P * 4)) return rc;
   if (int rc = dalloc_t(h, &h->gf1, h->P * 4)) return rc;
+  for (int l = 0; l < h->L - 1; ++l) {
+    Layer& ly = h->layers[l];
+    if (int rc = dalloc_t(h, &ly.bpart, (size_t)(h->Bp / 32) * ly.outp * sizeof(float))) return rc;
+  }
   if (int rc = ensure_wpart(h)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -889,7 +903,7 @@ void cnf_eval(node_handle* h, int stage, int mode, int gate, double* part) {
     ProfScope ps(h, PR_FWD_STAGE, l);
     launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
   }
-  for (int l = L - 1; l >= 1; --l) launch_bwd_data(h, l, gate);
+  for (int l = L - 1; l >= 1; --l) launch_bwd_data(h, l, gate, false);
   const Layer& l0 = h->layers[0];
   GemmArgs g{};
   g.A = h->Dl[0];

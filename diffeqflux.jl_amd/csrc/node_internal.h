@@ -90,10 +90,13 @@ struct EpiCnfStage {
   double* part;
 };
 
-// hidden data gradient: delta = acc * act'(z)  (row-major + transposed)
+// hidden data gradient: delta = acc * act'(z)  (row-major + transposed);
+// bsum (optional): FP32 column sums of delta per row tile, [row_tiles][ldo]
+// (the layer's bias gradient before the batch reduction over row tiles)
 struct EpiBwdData {
   int C; int ldo; int ldt; const void* S; void* Dl; void* DlT;
   int64_t pd, pdt;             // plane strides of Dl / DlT
+  float* bsum;
 };
 
 // layer-1 data gradient: k^lambda_s = -(acc * map'(u)), next stage inputs
@@ -133,6 +136,10 @@ struct FoldArgs {
   int outs[MAXL];
   int64_t b0s[MAXL];
   int bias_rows;
+  // per-row-tile column sums of delta from the data-gradient epilogue
+  // (nullptr: the bias gradient sums the transposed delta over the batch)
+  const float* bpart[MAXL];
+  int bpart_rt[MAXL], bpart_ld[MAXL];
   int ldt, Bp;
   int nsk;                     // narrow layers with split-K partials
   int64_t sk_w0[MAXL], sk_n[MAXL];  // flat offset / entry count
