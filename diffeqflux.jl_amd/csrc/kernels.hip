@@ -367,11 +367,11 @@ __device__ __forceinline__ void epi_apply(const EpiHidden& e, f32x16 (&acc)[MR][
     tile_write<ST, false, BR, BC, NT>(lds, sv, g, lane, (ST*)e.S, e.ldo, 0, (ST*)nullptr, 0, 0);
 }
 
-template <class PP, int BR, int BC, int NT, int MR, int MC>
-__device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC],
-                                          const TileGeo& g, int lane, const Ctl*, int,
-                                          unsigned char* lds) {
-  using T = typename PP::T;
+// act' of this thread's accumulator elements, loaded while the GEMM's last K
+// tile is computed (padded columns read the zero-filled padding)
+template <class PP, int MR, int MC>
+__device__ __forceinline__ void bwd_data_prefetch(const EpiBwdData& e, f32x16 (&pre)[MR][MC],
+                                                  const TileGeo& g, int lane) {
   using ST = typename PP::ST;
   const ST* S = (const ST*)e.S;
 #pragma unroll
@@ -379,16 +379,28 @@ __device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR]
 #pragma unroll
     for (int mj = 0; mj < MC; ++mj) {
       const int c = g.cb + mj * 32 + (lane & 31);
-      const bool cv = c < e.C;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int r0 = g.rb + mi * 32 + q * 8 + (lane >> 5) * 4;
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
-          acc[mi][mj][q * 4 + p] = cv ? acc[mi][mj][q * 4 + p] * tof<ST>(S[o]) : 0.f;
-        }
+        for (int p = 0; p < 4; ++p)
+          pre[mi][mj][q * 4 + p] = tof<ST>(S[(int64_t)(r0 + p) * e.ldo + c]);
       }
+    }
+}
+
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC],
+                                          const f32x16 (&pre)[MR][MC], const TileGeo& g,
+                                          int lane, unsigned char* lds) {
+  using T = typename PP::T;
+#pragma unroll
+  for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+    for (int mj = 0; mj < MC; ++mj) {
+      const bool cv = g.cb + mj * 32 + (lane & 31) < e.C;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[mi][mj][v] = cv ? acc[mi][mj][v] * pre[mi][mj][v] : 0.f;
     }
   if (e.bsum) {
     // column sums of the tile (fixed order: registers, lane halves, waves)
@@ -963,6 +975,16 @@ __global__ void __launch_bounds__(WR * WC * 64)
     if (j < nk) GEMM_GLOAD(j, j);
   GEMM_SSTORE(0, 0);
   __syncthreads();
+  TileGeo geo;
+  geo.slot = bid;
+  geo.row0 = row0;
+  geo.col0 = col0;
+  geo.lr0 = wr * (BR / WR);
+  geo.lc0 = wc * (BC / WC);
+  geo.rb = row0 + geo.lr0;
+  geo.cb = col0 + geo.lc0;
+  constexpr bool kPre = std::is_same<Epi, EpiBwdData>::value;
+  f32x16 pre[kPre ? MR : 1][kPre ? MC : 1];
   const int arow = wr * (BR / WR) + (lane & 31);
   const int brow = NPL * BR + wc * (BC / WC) + (lane & 31);
   const int koff = (lane >> 5) * 16;
@@ -973,6 +995,8 @@ __global__ void __launch_bounds__(WR * WC * 64)
       if (t < nk) {
         // slot j held tile t, already in LDS: refill it with tile t+S
         if (t + S < nk) GEMM_GLOAD(t + S, j);
+        if constexpr (kPre)
+          if (t == nk - 1) bwd_data_prefetch<PP>(epi, pre, geo, lane);
         const unsigned char* base = lds + (t & 1) * ROWS * PITCH;
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
@@ -1034,15 +1058,8 @@ __global__ void __launch_bounds__(WR * WC * 64)
         }
     if (tid == 0) g.cnt[bid] = 0;
   }
-  TileGeo geo;
-  geo.slot = bid;
-  geo.row0 = row0;
-  geo.col0 = col0;
-  geo.lr0 = wr * (BR / WR);
-  geo.lc0 = wc * (BC / WC);
-  geo.rb = row0 + geo.lr0;
-  geo.cb = col0 + geo.lc0;
-  epi_apply<PP, BR, BC, NT>(epi, acc, geo, lane, ctl, split, lds);
+  if constexpr (kPre) epi_apply<PP, BR, BC, NT>(epi, acc, pre, geo, lane, lds);
+  else epi_apply<PP, BR, BC, NT>(epi, acc, geo, lane, ctl, split, lds);
 }
 
 // tile configurations: {BR, BC, WR, WC}
