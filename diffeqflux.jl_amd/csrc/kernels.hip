@@ -367,13 +367,39 @@ __device__ __forceinline__ void epi_apply(const EpiHidden& e, f32x16 (&acc)[MR][
     tile_write<ST, false, BR, BC, NT>(lds, sv, g, lane, (ST*)e.S, e.ldo, 0, (ST*)nullptr, 0, 0);
 }
 
-// act' of this thread's accumulator elements, loaded while the GEMM's last K
-// tile is computed (padded columns read the zero-filled padding)
+// act' from the activation's output y (identity, softplus, relu):
+// softplus'(z) = sigmoid(z) = 1 - exp(-y) with y = softplus(z) > 0; below
+// y = 1/4 its Taylor series to y^6 (relative truncation < 5e-8, no
+// cancellation), above it the hardware exp2 (relative error < 1e-6)
+__device__ __forceinline__ float act_d_y(int a, float y) {
+  switch (a) {
+    case 2: {
+      if (y < 0.25f) {
+        float t = 1.f - y * (1.f / 6.f);
+        t = 1.f - y * (1.f / 5.f) * t;
+        t = 1.f - y * 0.25f * t;
+        t = 1.f - y * (1.f / 3.f) * t;
+        t = 1.f - y * 0.5f * t;
+        return y * t;
+      }
+      return 1.f - __builtin_amdgcn_exp2f(-y * LOG2E_F);
+    }
+    case 3: return y > 0.f ? 1.f : 0.f;
+    default: return 1.f;
+  }
+}
+
+// act' of this thread's accumulator
+// elements, loaded while the GEMM's last K tile is computed (padded columns
+// read the zero-filled padding)
 template <class PP, int MR, int MC>
 __device__ __forceinline__ void bwd_data_prefetch(const EpiBwdData& e, f32x16 (&pre)[MR][MC],
                                                   const TileGeo& g, int lane) {
   using ST = typename PP::ST;
   const ST* S = (const ST*)e.S;
+  // (the output-derived form is loaded by the epilogue itself: its two
+  // planes would not fit the main loop's registers)
+  if (e.yact >= 0) return;
 #pragma unroll
   for (int mi = 0; mi < MR; ++mi)
 #pragma unroll
@@ -389,18 +415,74 @@ __device__ __forceinline__ void bwd_data_prefetch(const EpiBwdData& e, f32x16 (&
     }
 }
 
+// Output-derived act': the tile's Y planes copied global -> LDS by the
+// direct-to-LDS loads (4 bytes per lane, a dense [plane][BR][BC] tile) into
+// the LDS buffer that the GEMM's last K tile does not use.
+template <class PP, int BR, int BC>
+constexpr int ylds_bytes() { return (PP::SPLIT ? 2 : 1) * BR * BC * (int)sizeof(typename PP::T); }
+
+template <class PP, int BR, int BC, int NT>
+__device__ __forceinline__ void bwd_data_prefetch_lds(const EpiBwdData& e, unsigned char* dst,
+                                                      const TileGeo& g) {
+  using T = typename PP::T;
+  constexpr int NPL = PP::SPLIT ? 2 : 1;
+  constexpr int EPL = 4 / (int)sizeof(T);                 // elements per lane
+  constexpr int NI = BR * BC * (int)sizeof(T) / 256;      // wave loads per plane
+  static_assert(BR * BC * sizeof(T) % 256 == 0, "tile / wave-load mismatch");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const T* Y = (const T*)e.Y;
+  for (int i = wave; i < NI * NPL; i += NT / 64) {
+    const int p = i / NI, ii = i - p * NI;
+    const int el = (ii * 64 + lane) * EPL;
+    const int r = el / BC, c = el - r * BC;
+    const T* src = Y + p * e.py + (int64_t)(g.row0 + r) * e.ldo + g.col0 + c;
+    __builtin_amdgcn_global_load_lds(
+        (const void*)src,
+        (__attribute__((address_space(3))) void*)(dst + p * (BR * BC * (int)sizeof(T)) + ii * 256),
+        4, 0, 0);
+  }
+}
+
 template <class PP, int BR, int BC, int NT, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC],
                                           const f32x16 (&pre)[MR][MC], const TileGeo& g,
-                                          int lane, unsigned char* lds) {
+                                          int lane, unsigned char* lds, const unsigned char* ylds) {
   using T = typename PP::T;
+  if (e.yact >= 0 && ylds) __syncthreads();  // direct loads done (vmcnt) and visible
 #pragma unroll
   for (int mi = 0; mi < MR; ++mi)
 #pragma unroll
     for (int mj = 0; mj < MC; ++mj) {
-      const bool cv = g.cb + mj * 32 + (lane & 31) < e.C;
+      const int c = g.cb + mj * 32 + (lane & 31);
+      const bool cv = c < e.C;
+      if (e.yact >= 0 && ylds) {
+        const T* yt = (const T*)ylds;
+        constexpr int PL = BR * BC;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) acc[mi][mj][v] = cv ? acc[mi][mj][v] * pre[mi][mj][v] : 0.f;
+        for (int q = 0; q < 4; ++q) {
+          const int r0 = g.lr0 + mi * 32 + q * 8 + (lane >> 5) * 4;
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            const T* ye = yt + (r0 + p) * BC + (c - g.col0);
+            const float y = PP::SPLIT ? tof<T>(ye[0]) + tof<T>(ye[PL]) : tof<T>(ye[0]);
+            acc[mi][mj][q * 4 + p] = cv ? acc[mi][mj][q * 4 + p] * act_d_y(e.yact, y) : 0.f;
+          }
+        }
+      } else if (e.yact >= 0) {
+        const T* Y = (const T*)e.Y;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r0 = g.rb + mi * 32 + q * 8 + (lane >> 5) * 4;
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            const float y = get<PP>(Y + (int64_t)(r0 + p) * e.ldo + c, e.py);
+            acc[mi][mj][q * 4 + p] = cv ? acc[mi][mj][q * 4 + p] * act_d_y(e.yact, y) : 0.f;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[mi][mj][v] = cv ? acc[mi][mj][v] * pre[mi][mj][v] : 0.f;
+      }
     }
   if (e.bsum) {
     // column sums of the tile (fixed order: registers, lane halves, waves)
@@ -984,6 +1066,7 @@ __global__ void __launch_bounds__(WR * WC * 64)
   geo.rb = row0 + geo.lr0;
   geo.cb = col0 + geo.lc0;
   constexpr bool kPre = std::is_same<Epi, EpiBwdData>::value;
+  constexpr bool kYlds = kPre && ylds_bytes<PP, BR, BC>() <= ROWS * PITCH;
   f32x16 pre[kPre ? MR : 1][kPre ? MC : 1];
   const int arow = wr * (BR / WR) + (lane & 31);
   const int brow = NPL * BR + wc * (BC / WC) + (lane & 31);
@@ -995,8 +1078,14 @@ __global__ void __launch_bounds__(WR * WC * 64)
       if (t < nk) {
         // slot j held tile t, already in LDS: refill it with tile t+S
         if (t + S < nk) GEMM_GLOAD(t + S, j);
-        if constexpr (kPre)
-          if (t == nk - 1) bwd_data_prefetch<PP>(epi, pre, geo, lane);
+        if constexpr (kPre) {
+          if (t == nk - 1) {
+            bwd_data_prefetch<PP>(epi, pre, geo, lane);
+            if constexpr (kYlds)
+              if (epi.yact >= 0)
+                bwd_data_prefetch_lds<PP, BR, BC, NT>(epi, lds + (nk & 1) * ROWS * PITCH, geo);
+          }
+        }
         const unsigned char* base = lds + (t & 1) * ROWS * PITCH;
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
@@ -1058,7 +1147,9 @@ __global__ void __launch_bounds__(WR * WC * 64)
         }
     if (tid == 0) g.cnt[bid] = 0;
   }
-  if constexpr (kPre) epi_apply<PP, BR, BC, NT>(epi, acc, pre, geo, lane, lds);
+  if constexpr (kPre)
+    epi_apply<PP, BR, BC, NT>(epi, acc, pre, geo, lane, lds,
+                              kYlds ? lds + (nk & 1) * ROWS * PITCH : nullptr);
   else epi_apply<PP, BR, BC, NT>(epi, acc, geo, lane, ctl, split, lds);
 }
 
@@ -1904,6 +1995,32 @@ void launch_set_stage_interp(Ctl* ctl, int slot, double tau, const double* fwd_t
 // per bias row), and the fixed-order sum of the split-K partials of the
 // narrow layers' weight gradients (one thread per entry); both folded into
 // the mu accumulators like the weight-gradient epilogue does.
+// fold_one on four consecutive entries (16-byte aligned)
+__device__ __forceinline__ void fold_four(const EpiWgrad& e, float* gcur, float* gnext,
+                                          int64_t idx, float4 g) {
+  if (e.flags & WG_STORE_CUR) *(float4*)(gcur + idx) = g;
+  if (e.flags & WG_STORE_PTR) *(float4*)(e.gptr + idx) = g;
+  if (e.flags & WG_INIT) {
+    const float4 g1 = *(const float4*)(gcur + idx);
+    *(float4*)(e.accb + idx) = make_float4(e.cb1 * g1.x + e.cb * g.x, e.cb1 * g1.y + e.cb * g.y,
+                                           e.cb1 * g1.z + e.cb * g.z, e.cb1 * g1.w + e.cb * g.w);
+    *(float4*)(e.acce + idx) = make_float4(e.ce1 * g1.x + e.ce * g.x, e.ce1 * g1.y + e.ce * g.y,
+                                           e.ce1 * g1.z + e.ce * g.z, e.ce1 * g1.w + e.ce * g.w);
+  } else {
+    if (e.flags & WG_B) {
+      float4 a = *(const float4*)(e.accb + idx);
+      a.x += e.cb * g.x; a.y += e.cb * g.y; a.z += e.cb * g.z; a.w += e.cb * g.w;
+      *(float4*)(e.accb + idx) = a;
+    }
+    if (e.flags & WG_E) {
+      float4 a = *(const float4*)(e.acce + idx);
+      a.x += e.ce * g.x; a.y += e.ce * g.y; a.z += e.ce * g.z; a.w += e.ce * g.w;
+      *(float4*)(e.acce + idx) = a;
+    }
+  }
+  if (e.flags & WG_STORE_NEXT) *(float4*)(gnext + idx) = g;
+}
+
 template <class PP>
 __global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_bias) {
   using T = typename PP::T;
@@ -1928,22 +2045,38 @@ __global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_b
     if (lane == 0) fold_one(e, gcur, gnext, fa.b0s[l] + row, -s);
     return;
   }
-  int64_t i = (int64_t)(blockIdx.x - nb_bias) * blockDim.x + threadIdx.x;
+  // split-K partial sums: four consecutive entries per thread (16-byte
+  // accesses where the layer's flat offset and size allow them)
+  int64_t i = 4 * ((int64_t)(blockIdx.x - nb_bias) * blockDim.x + threadIdx.x);
   for (int k = 0; k < fa.nsk; ++k) {
-    if (i < fa.sk_n[k]) {
-      float s = 0.f;
-      for (int j = 0; j < fa.sk_split[k]; ++j) s += fa.sk_part[k][(int64_t)j * fa.sk_n[k] + i];
-      fold_one(e, gcur, gnext, fa.sk_w0[k] + i, s);
+    const int64_t n = fa.sk_n[k], nq = (n + 3) & ~(int64_t)3;
+    if (i < nq) {
+      const float* part = fa.sk_part[k];
+      const int64_t w0 = fa.sk_w0[k];
+      if (((w0 | n) & 3) == 0) {
+        float4 sv = *(const float4*)(part + i);
+        for (int j = 1; j < fa.sk_split[k]; ++j) {
+          const float4 pv = *(const float4*)(part + (int64_t)j * n + i);
+          sv.x += pv.x; sv.y += pv.y; sv.z += pv.z; sv.w += pv.w;
+        }
+        fold_four(e, gcur, gnext, w0 + i, sv);
+      } else {
+        for (int64_t q = i; q < i + 4 && q < n; ++q) {
+          float sm = 0.f;
+          for (int j = 0; j < fa.sk_split[k]; ++j) sm += part[(int64_t)j * n + q];
+          fold_one(e, gcur, gnext, w0 + q, sm);
+        }
+      }
       return;
     }
-    i -= fa.sk_n[k];
+    i -= nq;
   }
 }
 void launch_fold(int prec, const FoldArgs& fa, const EpiWgrad& tmpl, const Ctl* ctl, int gate,
                  hipStream_t s) {
   const int nb_bias = (fa.bias_rows * 64 + 255) / 256;
   int64_t nsk = 0;
-  for (int k = 0; k < fa.nsk; ++k) nsk += fa.sk_n[k];
+  for (int k = 0; k < fa.nsk; ++k) nsk += (fa.sk_n[k] + 3) / 4;
   const int nb = nb_bias + (int)((nsk + 255) / 256);
   DISPATCH_PREC(prec, hipLaunchKernelGGL(k_fold<PP>, dim3(nb), dim3(256), 0, s, fa, tmpl, ctl,
                                          gate, nb_bias));

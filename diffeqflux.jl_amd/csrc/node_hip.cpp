@@ -116,6 +116,7 @@ struct node_handle {
   int tcfg[6][MAXL];
   int tks[6][MAXL];       // split-K of the stage epilogues (serial fixup)
   bool tuned = false, autotune = true;
+  bool tuned_data = false;  // plans of every role but the weight gradients are tuned
   int tune_role = -1, tune_layer = -1, tune_cfg = -1, tune_ks = 1;  // override while timing
   float* tune_wpart = nullptr;
   float* fix_ws = nullptr;  // split-K fixup workspace / per-tile counters
@@ -322,6 +323,12 @@ int stage_ks(node_handle* h, int role, int layer) {
   return h->tks[role][layer];
 }
 
+// act' of identity, softplus and relu is a function of the layer output: the
+// adjoint recompute then stores no act' and the data gradient derives it
+bool act_from_output(int act) {
+  return act == NODE_ACT_IDENTITY || act == NODE_ACT_SOFTPLUS || act == NODE_ACT_RELU;
+}
+
 // hidden layer l: forward (Y) or adjoint recompute (Y, Y^T, act')
 void launch_hidden(node_handle* h, int l, int gate, bool adjoint) {
   const Layer& ly = h->layers[l];
@@ -334,7 +341,7 @@ void launch_hidden(node_handle* h, int l, int gate, bool adjoint) {
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   const int64_t ph = (int64_t)h->Bp * ly.outp;
   EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], adjoint ? h->HT[l] : nullptr,
-              adjoint ? h->S[l] : nullptr, ph, ph};
+              adjoint && !act_from_output(ly.act) ? h->S[l] : nullptr, ph, ph};
   ProfScope ps(h, role, l);
   launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
 }
@@ -401,7 +408,8 @@ void launch_bwd_data(node_handle* h, int l, int gate, bool bias_sums = true) {
   Layer& lo = h->layers[l - 1];
   float* bsum = bias_sums ? lo.bpart : nullptr;
   if (bsum) lo.bpart_rt = p.rt;
-  EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd, bsum};
+  EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd, bsum,
+               h->H[l - 1], pd, act_from_output(lo.act) ? lo.act : -1};
   ProfScope ps(h, PR_BWD_DATA, l);
   launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
 }
@@ -585,7 +593,9 @@ int autotune_impl(node_handle* h) {
   if (int rc = dalloc_t(h, &tmp, (size_t)8 * maxio * sizeof(float))) return rc;
   const uint64_t saved_mask = h->prof_mask;
   h->prof_mask = 0;
+  const bool data = !h->tuned_data;
   auto tune_slot = [&](int role, int layer, int rows, int cols, int min_cfg, auto&& launch) {
+    if (!data) return;
     float best = 1e30f;
     int bc = -1;
     for (int c = min_cfg; c < gemm_cfg_count(); ++c) {
@@ -605,6 +615,7 @@ int autotune_impl(node_handle* h) {
   // (the adjoint stage epilogue keeps five value tiles live: its 128-row
   // form spills, so it starts at the 64-row tiles)
   auto tune_stage = [&](int role, int layer, int cols, int K, auto&& launch) {
+    if (!data) return;
     float best = 1e30f;
     int bc = -1, bks = 1;
     const int kt = (int)((int64_t)K * h->esz / 128);
@@ -661,6 +672,7 @@ int autotune_impl(node_handle* h) {
   dfree(h, tmp);
   if (int rc = ensure_wpart(h)) return rc;
   h->tuned = true;
+  h->tuned_data = true;
   if (h->fwd_graph) { (void)hipGraphExecDestroy(h->fwd_graph); h->fwd_graph = nullptr; }
   if (h->adj_graph) { (void)hipGraphExecDestroy(h->adj_graph); h->adj_graph = nullptr; }
   HIPCHK(h, hipGetLastError());
@@ -1684,6 +1696,7 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
     h->use_graphs = value != 0;
   } else if (key == NODE_OPT_AUTOTUNE) {
     h->autotune = value != 0;
+    h->tuned_data = false;
     if (!h->autotune) {  // back to the heuristic plans
       h->tuned = false;
       for (auto& row : h->tcfg)
@@ -1703,7 +1716,7 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
     const bool ok = value == h->prec || (h->prec == NODE_PREC_BF16X3 && value == NODE_PREC_BF16);
     if (!ok) return fail(h, NODE_ERR_INVALID, "weight-gradient precision must equal the handle's, or be BF16 on a BF16X3 handle");
     h->wg_prec = (int)value;
-    h->tuned = false;
+    h->tuned = false;  // re-tunes the weight-gradient plans only (tuned_data stays)
   } else if (key == NODE_OPT_GRAPH_ATTEMPTS) {
     if (value < 1 || value > 16) return fail(h, NODE_ERR_INVALID, "graph attempts must be 1..16");
     h->graph_attempts = (int)value;

@@ -93,10 +93,13 @@ struct EpiCnfStage {
 // hidden data gradient: delta = acc * act'(z)  (row-major + transposed);
 // bsum (optional): FP32 column sums of delta per row tile, [row_tiles][ldo]
 // (the layer's bias gradient before the batch reduction over row tiles)
+// yact >= 0: act' is derived from the layer output Y (planes py apart) instead
+// of read from S (activations whose derivative is a function of the output)
 struct EpiBwdData {
   int C; int ldo; int ldt; const void* S; void* Dl; void* DlT;
   int64_t pd, pdt;             // plane strides of Dl / DlT
   float* bsum;
+  const void* Y; int64_t py; int yact;
 };
 
 // layer-1 data gradient: k^lambda_s = -(acc * map'(u)), next stage inputs
