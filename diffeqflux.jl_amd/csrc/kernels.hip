@@ -68,14 +68,20 @@ __device__ const double dRI[6][3] = {
     {-27.896526289197286, 65.09189467479366, -34.8706578614966},
     {1.5, -4.0, 2.5}};
 
-void upload_tableau() {
+// Stream-ordered on the library stream (a non-blocking stream does not wait
+// for the null stream, and a copy from pageable memory may return before its
+// DMA lands); waits for the copies before the host arrays go out of scope.
+hipError_t upload_tableau(hipStream_t s) {
   float a[7][7], bt[7];
   for (int i = 0; i < 7; ++i) {
     for (int j = 0; j < 7; ++j) a[i][j] = (float)kA[i][j];
     bt[i] = (float)kBT[i];
   }
-  (void)hipMemcpyToSymbol(HIP_SYMBOL(cA), a, sizeof(a));
-  (void)hipMemcpyToSymbol(HIP_SYMBOL(cBT), bt, sizeof(bt));
+  hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(cA), a, sizeof(a), 0, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyToSymbolAsync(HIP_SYMBOL(cBT), bt, sizeof(bt), 0, hipMemcpyHostToDevice, s);
+  const hipError_t w = hipStreamSynchronize(s);
+  return e != hipSuccess ? e : w;
 }
 
 // PI controller constants (OrdinaryDiffEq defaults for Tsit5)

@@ -867,7 +867,8 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
   if (h->everystep) {
     // every accepted step: states are tape[j].uprev, j = 0..n
     std::vector<double> ft(r.n + 1);
-    HIPCHK(h, hipMemcpy(ft.data(), h->fwd_times, (r.n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpyAsync(ft.data(), h->fwd_times, (r.n + 1) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     ts.clear();
     int64_t j0 = o->save_start ? 0 : 1;
     for (int64_t j = j0; j <= r.n; ++j) ts.push_back(ft[j]);
@@ -1354,7 +1355,11 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
     delete h;
     return NODE_ERR_HIP;
   }
-  upload_tableau();
+  if (upload_tableau(h->stream) != hipSuccess) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return NODE_ERR_HIP;
+  }
   int64_t off = 0;
   int rc = 0;
   for (int l = 0; l < h->L && !rc; ++l) {
@@ -1463,16 +1468,20 @@ static int set_params_host(node_handle* h, const float* p, int64_t n) {
           c16[cnt + k] = tobf(wc[k] - frombf(c16[k]));
         }
       }
-      HIPCHK(h, hipMemcpy(ly.Wr, r16.data(), cnt * 2 * np, hipMemcpyHostToDevice));
-      HIPCHK(h, hipMemcpy(ly.Wc, c16.data(), cnt * 2 * np, hipMemcpyHostToDevice));
+      HIPCHK(h, hipMemcpyAsync(ly.Wr, r16.data(), cnt * 2 * np, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemcpyAsync(ly.Wc, c16.data(), cnt * 2 * np, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemcpyAsync(ly.bias, b.data(), ly.outp * 4, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));  // host staging vectors end with this layer
     } else {
-      HIPCHK(h, hipMemcpy(ly.Wr, wr.data(), cnt * 4, hipMemcpyHostToDevice));
-      HIPCHK(h, hipMemcpy(ly.Wc, wc.data(), cnt * 4, hipMemcpyHostToDevice));
+      HIPCHK(h, hipMemcpyAsync(ly.Wr, wr.data(), cnt * 4, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemcpyAsync(ly.Wc, wc.data(), cnt * 4, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemcpyAsync(ly.bias, b.data(), ly.outp * 4, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
     }
-    HIPCHK(h, hipMemcpy(ly.bias, b.data(), ly.outp * 4, hipMemcpyHostToDevice));
   }
   if (!h->pflat && dalloc_t(h, &h->pflat, h->P * sizeof(float))) return NODE_ERR_HIP;
-  HIPCHK(h, hipMemcpy(h->pflat, p, h->P * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpyAsync(h->pflat, p, h->P * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_params = true;
   h->have_forward = false;
   h->rg.have = false;
@@ -1490,7 +1499,8 @@ int32_t node_set_params_device(node_handle_t* h, const float* d_p, int64_t n) {
   std::lock_guard<std::mutex> g(h->mtx);
   if (n != h->P) return fail(h, NODE_ERR_INVALID, "parameter count mismatch");
   std::vector<float> p(n);
-  HIPCHK(h, hipMemcpy(p.data(), d_p, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpyAsync(p.data(), d_p, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return set_params_host(h, p.data(), n);
 }
 
@@ -1527,7 +1537,8 @@ int32_t node_forward(node_handle_t* h, const node_solve_opts* opts, const float*
   if (rc) return rc;
   if (out) {
     if (h->nsaved > out_cap) return fail(h, NODE_ERR_CAPACITY, "out_cap smaller than the number of saved states");
-    HIPCHK(h, hipMemcpy(out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpyAsync(out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
   }
   return 0;
 }
@@ -1564,8 +1575,10 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
   h->cnf_active = false;
   if (rc) return rc;
   std::vector<float> fin((size_t)B * h->D);
-  if (st.nsaved >= 1)
-    HIPCHK(h, hipMemcpy(fin.data(), h->out_dev, fin.size() * sizeof(float), hipMemcpyDeviceToHost));
+  if (st.nsaved >= 1) {
+    HIPCHK(h, hipMemcpyAsync(fin.data(), h->out_dev, fin.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
   const double lc = 0.5 * dm * std::log(2.0 * M_PI);
   for (int b = 0; b < B; ++b) {
     double ss = 0.0;
@@ -1597,7 +1610,8 @@ int32_t node_copy_saved(node_handle_t* h, float* out, int64_t cap) {
   std::lock_guard<std::mutex> g(h->mtx);
   if (!h->have_forward) return fail(h, NODE_ERR_STATE, "no forward solve");
   if (cap < h->nsaved) return fail(h, NODE_ERR_CAPACITY, "cap too small");
-  HIPCHK(h, hipMemcpy(out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpyAsync(out, h->out_dev, (size_t)h->nsaved * h->B * h->D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 

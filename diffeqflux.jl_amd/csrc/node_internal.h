@@ -150,8 +150,8 @@ struct FoldArgs {
   int sk_split[MAXL];
 };
 
-// tableau in constant memory
-void upload_tableau();
+// tableau in constant memory (stream-ordered copy, waited for)
+hipError_t upload_tableau(hipStream_t s);
 
 // ---- launchers (kernels.hip) ----
 template <typename T> struct TypeTag {};

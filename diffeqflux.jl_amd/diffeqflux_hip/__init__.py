@@ -10,8 +10,11 @@ from .neural_ode import (NeuralODE, Chain, Dense, Cube, Tsit5, InterpolatingAdjo
 from .handle import Handle
 from ._lib import load, NodeHipError
 from .ffjord import FFJORD
+from .multiple_shooting import (group_ranges, multiple_shoot, EnsembleProblem, EnsembleThreads,
+                                EnsembleSerial, EnsembleGPUArray)
 
 __all__ = ["NeuralODE", "Chain", "Dense", "Cube", "Tsit5", "InterpolatingAdjoint",
            "ZygoteVJP", "BacksolveAdjoint", "TrackerAdjoint", "QuadratureAdjoint",
            "GaussAdjoint", "ODESolution", "save_plan", "AugmentedNDELayer", "DimMover",
-           "Handle", "load", "NodeHipError", "FFJORD"]
+           "Handle", "load", "NodeHipError", "FFJORD", "group_ranges", "multiple_shoot",
+           "EnsembleProblem", "EnsembleThreads", "EnsembleSerial", "EnsembleGPUArray"]

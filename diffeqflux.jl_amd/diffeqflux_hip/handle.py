@@ -49,6 +49,10 @@ class Handle:
         self.h = h
         self.last_stats = None
         self.saved_times = None
+        # count of forward solves (batched / per-trajectory): a backward pass
+        # adjoints the handle's most recent forward of its kind only
+        self.fwd_gen = 0
+        self.rg_gen = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -92,6 +96,7 @@ class Handle:
 
     def forward(self, u0, opts, saveat_interior=()):
         """Returns (states (Nt, B, D), times (Nt,), stats dict)."""
+        self.fwd_gen += 1
         sa = np.ascontiguousarray(np.asarray(saveat_interior, dtype=np.float64).reshape(-1))
         sa_p = sa.ctypes.data_as(ctypes.c_void_p) if sa.size else None
         st = _lib.Stats()
@@ -176,6 +181,7 @@ class Handle:
         increasing sequences.  Returns (list of (len_i, D) saved states,
         retcodes (n,), per-trajectory stats (n, 4) [nf, naccept, nreject,
         records], totals dict)."""
+        self.rg_gen += 1
         u0 = _f32(u0).reshape(-1, self.D)
         n = u0.shape[0]
         t1 = np.ascontiguousarray(np.asarray(t1, dtype=np.float64).reshape(n))

@@ -317,6 +317,7 @@ def _NeuralODEFn_factory():
                 o, t, stats = h.forward(u0.cpu().numpy(), opts, interior)
                 out = torch.from_numpy(np.ascontiguousarray(o))
             ctx.h = h
+            ctx.gen = h.fwd_gen
             ctx.xshape = xt.shape
             ctx.pdtype, ctx.xdtype = pt.dtype, xt.dtype
             ctx.pdev, ctx.xdev = pt.device, xt.device
@@ -326,6 +327,10 @@ def _NeuralODEFn_factory():
         @staticmethod
         def backward(ctx, g, _gt=None, _gs=None):
             h = ctx.h
+            if h.fwd_gen != ctx.gen:
+                raise RuntimeError("NeuralODE backward: the layer's handle ran another forward "
+                                   "solve since this one (the handle keeps one tape); call "
+                                   "backward before the next forward of the same layer and batch")
             if g.is_cuda:
                 du0, dp, _ = h.backward(g.contiguous())
             else:
