@@ -276,6 +276,82 @@ def ragged_leg(n=10000, reps=3, cpu_sample=8):
     return res
 
 
+# ---------------------------------------------------------------- multiple shooting leg
+def spiral_ensemble(ntraj, t, seed=8, substeps=64):
+    """The multiple-shooting docs problem (u' = ((u.^3)' A)', A = [-0.1 2; -2
+    -0.1]) from ntraj initial states on radius 1.5..2.5, integrated by
+    classical RK4 with `substeps` steps per data interval (synthetic data)."""
+    rng = np.random.default_rng(seed)
+    ang = rng.uniform(0, 2 * math.pi, ntraj)
+    rad = rng.uniform(1.5, 2.5, ntraj)
+    u = np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)
+    A = np.array([[-0.1, 2.0], [-2.0, -0.1]])
+    f = lambda x: (x ** 3) @ A                                    # noqa: E731
+    out = [u.copy()]
+    for k in range(len(t) - 1):
+        h = (t[k + 1] - t[k]) / substeps
+        for _ in range(substeps):
+            k1 = f(u); k2 = f(u + 0.5 * h * k1); k3 = f(u + 0.5 * h * k2); k4 = f(u + h * k3)
+            u = u + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        out.append(u.copy())
+    return np.stack(out, 1).astype(np.float32)                    # (ntraj, T, 2)
+
+
+def multiple_shoot_leg(ntraj=1024, reps=3, cpu_traj=2):
+    """Ensemble multiple shooting (src/multiple_shooting.jl:131-195) on the
+    docs example: Chain(x -> x.^3, Dense(2,16,tanh), Dense(16,2)), 30 points
+    on (0,5), group size 3, continuity term 200, ntraj trajectories: one loss
+    + dL/dp evaluation = 15 x ntraj segment solves forward + adjoint (one
+    per-trajectory launch each way); the oracle on the segments of cpu_traj
+    trajectories as the CPU baseline."""
+    import torch
+    from diffeqflux_hip import (Chain, Cube, Dense, NeuralODE, Tsit5, multiple_shoot,
+                                EnsembleProblem, EnsembleThreads, group_ranges)
+    from oracle.mlp import MLP, MAP_CUBE
+    m = MLP([2, 16, 2], ["tanh", "identity"], input_map=MAP_CUBE)
+    p = m.glorot_init(np.random.default_rng(3)).astype(np.float32)
+    t = np.linspace(0.0, 5.0, 30)
+    data = spiral_ensemble(ntraj, t)
+    node = NeuralODE(Chain(Cube(), Dense(2, 16, "tanh"), Dense(16, 2)), (0.0, 5.0), Tsit5(),
+                     saveat=list(t))
+    loss_fn = lambda u, v: ((u - v) ** 2).sum()                   # noqa: E731
+    nseg = len(group_ranges(30, 3)) * ntraj
+    best = None
+    for _ in range(reps):
+        pt = torch.tensor(p, requires_grad=True)
+        t0 = time.perf_counter()
+        loss, _ = multiple_shoot(pt, data, t, EnsembleProblem(node), EnsembleThreads(), loss_fn,
+                                 Tsit5(), 3, continuity_term=200, trajectories=ntraj)
+        loss.backward()
+        dt = time.perf_counter() - t0
+        if best is None or dt < best[0]:
+            best = (dt, float(loss.detach()))
+    wall, lval = best
+    res = {"workload": f"ensemble multiple shooting, {ntraj} trajectories x 30 points of the docs "
+                       "spiral, group size 3 ({nseg} segments), Chain(x.^3, Dense(2,16,tanh), "
+                       "Dense(16,2)), Tsit5 reltol 1e-3 abstol 1e-6, loss + dL/dp".format(nseg=nseg),
+           "segments_per_s": round(nseg / wall, 1), "evals_per_s": round(1.0 / wall, 3),
+           "ms_per_eval": round(1e3 * wall, 2), "loss": lval,
+           "timing": "host entry point incl. PCIe copies and torch loss assembly"}
+    from oracle import adjoint as OA
+    tc = time.perf_counter()
+    nsc = 0
+    for j in range(cpu_traj):
+        for rg in group_ranges(30, 3):
+            s = [float(x) for x in t[rg.start:rg.stop]]
+            f = OA.forward(m, p, data[j, rg.start][None], s[0], s[-1], saveat=s, abstol=1e-6,
+                           reltol=1e-3)
+            c = [2.0 * (np.array(f.u)[k] - data[j, rg.start + k][None]) for k in range(len(s))]
+            OA.backward(m, p, f, c, abstol=1e-6, reltol=1e-3)
+            nsc += 1
+    cs = time.perf_counter() - tc
+    res["cpu_baseline"] = {"value": round(nsc / cs, 2), "unit": "segments/s", "cores": 1,
+                           "kind": "port",
+                           "sample": f"oracle FP32, the {nsc} segments of {cpu_traj} trajectories "
+                                     f"solved one by one, forward+adjoint, in {cs:.1f} s"}
+    return res
+
+
 # ---------------------------------------------------------------- FFJORD leg
 def gaussian_ring(n, D=16, seed=6):
     """8-component Gaussian mixture, means on a radius-4 ring in the first two
@@ -391,6 +467,8 @@ def main():
     ap.add_argument("--no-ragged", action="store_true",
                     help="skip the per-trajectory (ragged) workload leg")
     ap.add_argument("--no-ffjord", action="store_true", help="skip the FFJORD workload leg")
+    ap.add_argument("--no-multiple-shoot", action="store_true",
+                    help="skip the multiple-shooting workload leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); "
@@ -478,6 +556,9 @@ def main():
     if not args.no_ffjord:
         extra["ffjord"] = ffjord_leg()
         log(f"[bench] ffjord: {extra['ffjord']}")
+    if not args.no_multiple_shoot:
+        extra["multiple_shoot"] = multiple_shoot_leg()
+        log(f"[bench] multiple_shoot: {extra['multiple_shoot']}")
 
     rec = {
         "metric": METRIC, "value": round(value, 2), "unit": "RHS evals/s",
