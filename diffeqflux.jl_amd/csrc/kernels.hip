@@ -1352,10 +1352,11 @@ __global__ void __launch_bounds__(256)
     const int tr0 = (tile / tc) * ST_ROWS, tcol = (tile % tc) * 64;
     const int r0 = tr0 + RPT * rq, c = tcol + cl;
     const bool cv = c < e.D;
-    float kv[RPT], lsv[RPT], uv[RPT], xv[RPT];
+    float kv[RPT], lsv[RPT], uv[RPT], xv[RPT], fuv[RPT];
+    const bool bs = e.ubs != nullptr;  // BacksolveAdjoint: u integrated with lambda
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
-      kv[j] = lsv[j] = uv[j] = xv[j] = 0.f;
+      kv[j] = lsv[j] = uv[j] = xv[j] = fuv[j] = 0.f;
       const int r = r0 + j;
       const int64_t idx = (int64_t)r * e.D + c;
       if (!cv) continue;
@@ -1364,6 +1365,33 @@ __global__ void __launch_bounds__(256)
       const float k = -g;
       kv[j] = e.mode == SM_PLAIN ? g : k;
       const bool live = r < e.B;
+      if (bs && e.mode != SM_PLAIN) {
+        // f(u_s) of this stage: u's own stage value and error terms
+        const float fu = e.fu[idx];
+        fuv[j] = fu;
+        const float ub = e.ubs[idx];
+        if (e.mode == SM_STEP && s == 6) {
+          float ut = cBT[6] * fu;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) ut += cBT[i] * e.ku[i * e.sv + idx];
+          ut *= dt;
+          const float un = e.ubs_new[idx];
+          const float sk = atol + fmaxf(fabsf(ub), fabsf(un)) * rtol;
+          if (live) {
+            const double a = (double)(ut / sk);
+            sum += a * a;
+            if (isnan(un)) flag = 1.0;
+          }
+        } else if (e.mode == SM_INIT_F0 || e.mode == SM_INIT_F1) {
+          const float d = e.mode == SM_INIT_F0 ? fu : fu - e.ku[idx];
+          const float sk = atol + fabsf(ub) * rtol;
+          if (live) {
+            const double a = (double)(d / sk);
+            sum += a * a;
+            if (e.mode == SM_INIT_F1 && d != 0.f) flag = 1.0;
+          }
+        }
+      }
       if (e.mode == SM_STEP) {
         if (s < 6) {
           float sm = cA[s + 1][s] * k;
@@ -1371,12 +1399,21 @@ __global__ void __launch_bounds__(256)
           for (int i = 0; i < 5; ++i)
             if (i < s) sm += cA[s + 1][i] * e.kl[i * e.sv + idx];
           lsv[j] = e.lam[idx] + dt * sm;
-          float u = te[idx];
-          if (!ex) {
-            float sm2 = 0.f;
+          float u;
+          if (bs) {
+            float su = cA[s + 1][s] * fuv[j];
 #pragma unroll
-            for (int i = 0; i < 7; ++i) sm2 += w[i] * te[(1 + i) * e.sv + idx];
-            u += tdt * sm2;
+            for (int i = 0; i < 5; ++i)
+              if (i < s) su += cA[s + 1][i] * e.ku[i * e.sv + idx];
+            u = e.ubs[idx] + dt * su;
+          } else {
+            u = te[idx];
+            if (!ex) {
+              float sm2 = 0.f;
+#pragma unroll
+              for (int i = 0; i < 7; ++i) sm2 += w[i] * te[(1 + i) * e.sv + idx];
+              u += tdt * sm2;
+            }
           }
           uv[j] = u;
           xv[j] = map_f(e.map, u);
@@ -1414,12 +1451,17 @@ __global__ void __launch_bounds__(256)
           e.plain_out[idx] = kv[j];
         } else if (e.mode == SM_INIT_F0) {
           e.kl[idx] = kv[j];
+          if (bs) e.ku[idx] = fuv[j];
         } else if (e.mode == SM_INIT_F1) {
           e.kl[7 * e.sv + idx] = kv[j];
         } else {
           e.kl[s * e.sv + idx] = kv[j];
+          if (bs) e.ku[s * e.sv + idx] = fuv[j];
           if (s < 6) {
-            if (s == 5) e.lam_new[idx] = lsv[j];
+            if (s == 5) {
+              e.lam_new[idx] = lsv[j];
+              if (bs) e.ubs_new[idx] = uv[j];
+            }
             e.Ucur[idx] = uv[j];
           }
         }
@@ -1802,12 +1844,11 @@ void launch_fwd_controller(Ctl* ctl, const double* part, int n_part, const doubl
 }
 // ---------------------------------------------------------------- adjoint
 template <class PP>
+__device__ __forceinline__ void adj_prep_stage_u(const StateGeom& sg, const AdjBufs& ab,
+                                                 int64_t i, float lam_s, float u);
+template <class PP>
 __device__ __forceinline__ void adj_prep_stage(const StateGeom& sg, const AdjBufs& ab,
                                                const Ctl* c, int slot, int64_t i, float lam_s) {
-  using T = typename PP::T;
-  const int r = (int)(i / sg.D), col = (int)(i % sg.D);
-  put<PP>((T*)ab.L + (int64_t)r * ab.ldl + col, ab.pl, lam_s);
-  put<PP>((T*)ab.LT + (int64_t)col * ab.ldlt + r, ab.plt, lam_s);
   const float* te = ab.tape + (int64_t)c->st_step[slot] * sg.ent;
   float u = te[i];
   if (!c->st_exact[slot]) {
@@ -1816,6 +1857,16 @@ __device__ __forceinline__ void adj_prep_stage(const StateGeom& sg, const AdjBuf
     for (int j = 0; j < 7; ++j) sm += c->st_w[slot][j] * te[(1 + j) * sg.sv + i];
     u += c->st_dt[slot] * sm;
   }
+  adj_prep_stage_u<PP>(sg, ab, i, lam_s, u);
+}
+// stage inputs (lambda_s row-major + transposed, x = map(u_s) likewise, u_s)
+template <class PP>
+__device__ __forceinline__ void adj_prep_stage_u(const StateGeom& sg, const AdjBufs& ab,
+                                                 int64_t i, float lam_s, float u) {
+  using T = typename PP::T;
+  const int r = (int)(i / sg.D), col = (int)(i % sg.D);
+  put<PP>((T*)ab.L + (int64_t)r * ab.ldl + col, ab.pl, lam_s);
+  put<PP>((T*)ab.LT + (int64_t)col * ab.ldlt + r, ab.plt, lam_s);
   const float x = map_f(sg.map, u);
   put<PP>((T*)ab.X0 + (int64_t)r * ab.ldx + col, ab.px, x);
   put<PP>((T*)ab.X0T + (int64_t)col * ab.ldxt + r, ab.pxt, x);
@@ -1836,6 +1887,11 @@ __global__ void k_adj_init(StateGeom sg, AdjBufs ab, const Ctl* c, double* part)
     ab.lam[i] = l;
     adj_prep_stage<PP>(sg, ab, c, 0, i, l);
     if (r < sg.B) { const double a = (double)l / (atol + fabs((double)l) * rtol); sum += a * a; }
+    if (ab.ubs) {  // backsolve: u starts from the forward's u(T)
+      const float u = ab.U[i];
+      ab.ubs[i] = u;
+      if (r < sg.B) { const double a = (double)u / (atol + fabs((double)u) * rtol); sum += a * a; }
+    }
   }
   block_reduce_store(sum, 0.0, part, blockIdx.x);
 }
@@ -1851,8 +1907,10 @@ __global__ void k_adj_initdt_prep(StateGeom sg, AdjBufs ab, const Ctl* c) {
   const float h = (float)(c->tdir * c->dt0);
   const int64_t n = (int64_t)sg.Bp * sg.D;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    adj_prep_stage<PP>(sg, ab, c, 7, i, ab.lam[i] + h * ab.kl[i]);
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (ab.ubs) adj_prep_stage_u<PP>(sg, ab, i, ab.lam[i] + h * ab.kl[i], ab.ubs[i] + h * ab.ku[i]);
+    else adj_prep_stage<PP>(sg, ab, c, 7, i, ab.lam[i] + h * ab.kl[i]);
+  }
 }
 void launch_adj_initdt_prep(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl,
                             hipStream_t s) {
@@ -1886,6 +1944,15 @@ __global__ void k_adj_step_begin(StateGeom sg, AdjBufs ab, const Ctl* c, int nbs
     float l = acc ? ab.lam_new[i] : ab.lam[i];
     if (acc && jidx >= 0 && r < sg.B) l += ab.dLdout[(int64_t)jidx * sg.B * sg.D + i];
     if (acc) { ab.lam[i] = l; ab.kl[i] = ab.kl[6 * sg.sv + i]; }
+    if (ab.ubs) {  // backsolve: commit u; checkpoint = the forward's saved state
+      float ub = acc ? ab.ubs_new[i] : ab.ubs[i];
+      if (acc && jidx >= 0 && ab.ckpt && r < sg.B) ub = ab.fsaved[(int64_t)jidx * sg.B * sg.D + i];
+      if (acc) { ab.ubs[i] = ub; ab.ku[i] = ab.ku[6 * sg.sv + i]; }
+      if (done) continue;
+      if (re) adj_prep_stage_u<PP>(sg, ab, i, l, ub);
+      else adj_prep_stage_u<PP>(sg, ab, i, l + h * ab.kl[i], ub + h * ab.ku[i]);
+      continue;
+    }
     if (done) continue;
     if (re) adj_prep_stage<PP>(sg, ab, c, 0, i, l);
     else adj_prep_stage<PP>(sg, ab, c, 1, i, l + h * (acc ? ab.kl[6 * sg.sv + i] : ab.kl[i]));

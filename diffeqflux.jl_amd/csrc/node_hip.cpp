@@ -117,6 +117,13 @@ struct node_handle {
   int tks[6][MAXL];       // split-K of the stage epilogues (serial fixup)
   bool tuned = false, autotune = true;
   bool tuned_data = false;  // plans of every role but the weight gradients are tuned
+  int sense = NODE_SENSE_INTERPOLATING;  // adjoint of node_backward (NODE_SENSE_*)
+  // BacksolveAdjoint state: u at the step start / candidate, 8 stage slots of
+  // f(u), f(u_s) of the current stage
+  float* bs_u = nullptr;
+  float* bs_unew = nullptr;
+  float* bs_ku = nullptr;
+  float* bs_fu = nullptr;
   int tune_role = -1, tune_layer = -1, tune_cfg = -1, tune_ks = 1;  // override while timing
   float* tune_wpart = nullptr;
   float* fix_ws = nullptr;  // split-K fixup workspace / per-tile counters
@@ -312,6 +319,7 @@ GemmPlan plan_role(node_handle* h, int role, int layer, int rows, int cols, int 
 // batches keep the fused epilogue, which saves the partials' round trip.
 bool stage_split(const node_handle* h, int role) {
   if (h->cnf && role == PR_FWD_STAGE) return false;
+  if (role == PR_BWD_LAST && h->sense != NODE_SENSE_INTERPOLATING) return true;  // backsolve stage
   const int cols = role == PR_BWD_LAST ? h->Dp : h->layers[h->L - 1].outp;
   return (int64_t)(h->Bp / 32) * (cols / 64) < 2048;
 }
@@ -347,7 +355,8 @@ void launch_hidden(node_handle* h, int l, int gate, bool adjoint) {
 }
 
 // last layer with the Tsit5 stage epilogue
-void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* part) {
+void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* part,
+                      float* plain_out = nullptr) {
   const int l = h->L - 1;
   const Layer& ly = h->layers[l];
   GemmArgs g{};
@@ -364,7 +373,7 @@ void launch_fwd_stage(node_handle* h, int stage, int mode, int gate, double* par
   e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
   e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D; e.sv = (int64_t)h->Bp * h->D;
   e.X0 = h->X0; e.ldx = h->Dp; e.map = h->desc.input_map;
-  e.scratch = h->scratch; e.part = part;
+  e.scratch = plain_out ? plain_out : h->scratch; e.part = part;
   e.px = (int64_t)h->Bp * h->Dp;
   ProfScope ps(h, PR_FWD_STAGE, l);
   if (split) {
@@ -467,6 +476,9 @@ void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part
   e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D;
   e.part = part; e.plain_out = plain_out;
   e.pl = e.plt = e.px = e.pxt = (int64_t)h->Bp * h->Dp;
+  if (h->sense != NODE_SENSE_INTERPOLATING) {
+    e.fu = h->bs_fu; e.ku = h->bs_ku; e.ubs = h->bs_u; e.ubs_new = h->bs_unew;
+  }
   ProfScope ps(h, PR_BWD_LAST, 0);
   if (split) {
     const EpiPartial ep{h->fix_ws, h->Dp, (int64_t)h->Bp * h->Dp};
@@ -516,6 +528,10 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
               float cb1, float ce1, double* part, float* plain_out, float* gptr) {
   const int L = h->L;
   for (int l = 0; l < L - 1; ++l) launch_hidden(h, l, gate, true);
+  // BacksolveAdjoint: f(u_s) of this stage (the last layer on the recomputed
+  // hidden state) for u's own Runge-Kutta stages
+  if (h->sense != NODE_SENSE_INTERPOLATING && mode != SM_PLAIN)
+    launch_fwd_stage(h, 0, SM_PLAIN, gate, nullptr, h->bs_fu);
   EpiWgrad wt{};
   wt.accb = h->accb; wt.acce = h->acce; wt.gf0 = h->gf0; wt.gf1 = h->gf1; wt.gptr = gptr;
   wt.cb = cb; wt.ce = ce; wt.cb1 = cb1; wt.ce1 = ce1; wt.flags = wflags;
@@ -717,6 +733,10 @@ AdjBufs adj_bufs(node_handle* h) {
   a.dLdout = h->dLdout_dev;
   a.mu = h->mu; a.accb = h->accb; a.acce = h->acce; a.gf0 = h->gf0; a.gf1 = h->gf1;
   a.P = h->P;
+  if (h->sense != NODE_SENSE_INTERPOLATING) {
+    a.ubs = h->bs_u; a.ubs_new = h->bs_unew; a.ku = h->bs_ku; a.fsaved = h->out_dev;
+    a.ckpt = h->sense == NODE_SENSE_BACKSOLVE;
+  }
   return a;
 }
 
@@ -950,6 +970,13 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   if (!(h->fwd_opts.t1 > h->fwd_opts.t0))
     return fail(h, NODE_ERR_UNSUPPORTED, "adjoint needs a forward solve with t1 > t0");
   if (int rc = ensure_adjoint(h)) return rc;
+  if (h->sense != NODE_SENSE_INTERPOLATING && !h->bs_u) {
+    const int64_t sv = (int64_t)h->Bp * h->D;
+    if (int rc = dalloc_t(h, &h->bs_u, sv * 4)) return rc;
+    if (int rc = dalloc_t(h, &h->bs_unew, sv * 4)) return rc;
+    if (int rc = dalloc_t(h, &h->bs_ku, 8 * sv * 4)) return rc;
+    if (int rc = dalloc_t(h, &h->bs_fu, sv * 4)) return rc;
+  }
   if (int rc = maybe_autotune(h)) return rc;
   const node_solve_opts& o = h->fwd_opts;
   const StateGeom sg = geom(h);
@@ -971,7 +998,8 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   const int nb = state_blocks(sg);
   const int nmb = mu_blocks(h->P);
   const int nx = vjp_last_tiles(h);
-  const double nel = (double)h->B * h->D + (double)h->P;
+  // error norm over [lambda; mu], or [u; lambda; mu] when u is integrated too
+  const double nel = (h->sense != NODE_SENSE_INTERPOLATING ? 2.0 : 1.0) * h->B * h->D + (double)h->P;
 
   launch_set_stage_interp(h->ctl, 0, o.t1, h->fwd_times, h->stream);
   launch_adj_init(sg, ab, h->ctl, p0, h->stream);
@@ -1731,6 +1759,11 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
     if (!ok) return fail(h, NODE_ERR_INVALID, "weight-gradient precision must equal the handle's, or be BF16 on a BF16X3 handle");
     h->wg_prec = (int)value;
     h->tuned = false;  // re-tunes the weight-gradient plans only (tuned_data stays)
+  } else if (key == NODE_OPT_SENSEALG) {
+    if (value < NODE_SENSE_INTERPOLATING || value > NODE_SENSE_BACKSOLVE_NOCKPT)
+      return fail(h, NODE_ERR_INVALID, "unknown sensealg (NODE_SENSE_*)");
+    if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "sensealg: a CNF handle has no adjoint");
+    h->sense = (int)value;
   } else if (key == NODE_OPT_GRAPH_ATTEMPTS) {
     if (value < 1 || value > 16) return fail(h, NODE_ERR_INVALID, "graph attempts must be 1..16");
     h->graph_attempts = (int)value;

@@ -113,6 +113,9 @@ struct EpiBwdLast {
   double* part;
   float* plain_out;                        // node_vjp: write -k (= lambda^T df/du)
   int64_t pl, plt, px, pxt;                // plane strides of L / LT / X0 / X0T
+  // BacksolveAdjoint (ubs != nullptr; elementwise stage kernel only): fu =
+  // f(u_s) of this stage, ku its stage slots, ubs / ubs_new as in AdjBufs
+  const float* fu; float* ku; float* ubs; float* ubs_new;
 };
 
 // split-K partial products of a skinny stage GEMM (FP32 [ksplit][rows][ld])
@@ -221,6 +224,10 @@ struct AdjBufs {
   const double* cb_times; const float* dLdout;
   double* mu; float* accb; float* acce; float* gf0; float* gf1;
   int64_t P;
+  // BacksolveAdjoint (ubs != nullptr): u integrated with the adjoint; ubs /
+  // ubs_new the state at the step start / its candidate, ku 8 stage slots of
+  // f(u), fsaved the forward's saved states (checkpoints when ckpt)
+  float* ubs; float* ubs_new; float* ku; const float* fsaved; int ckpt;
 };
 void launch_adj_init(const StateGeom& sg, const AdjBufs& ab, const Ctl* ctl,
                      double* part, hipStream_t s);

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libnode_hip.so")
 MAX_LAYERS = 8
 ACT = {"identity": 0, "tanh": 1, "softplus": 2, "relu": 3, "sigmoid": 4}
 MAP_IDENTITY, MAP_CUBE, MAP_CNF, MAP_CNF_TIME = 0, 1, 2, 3
-OPT_GRAPHS, OPT_GRAPH_ATTEMPTS, OPT_AUTOTUNE, OPT_WGRAD_PRECISION = 1, 2, 3, 4
+OPT_GRAPHS, OPT_GRAPH_ATTEMPTS, OPT_AUTOTUNE, OPT_WGRAD_PRECISION, OPT_SENSEALG = 1, 2, 3, 4, 5
+SENSE = {"interpolating": 0, "backsolve": 1, "backsolve_nockpt": 2}
 PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 RET_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN",
              4: "MaxIters", 5: "DtLessThanMin", 6: "Unstable"}

@@ -242,13 +242,21 @@ class Handle:
         return du, dp
 
     def set_option(self, key, value):
-        """key: 'graphs' (bool), 'graph_attempts' (1..16), 'autotune' (bool) or
-        'wgrad_precision' ('fp32' / 'bf16' / 'bf16x3')."""
+        """key: 'graphs' (bool), 'graph_attempts' (1..16), 'autotune' (bool),
+        'wgrad_precision' ('fp32' / 'bf16' / 'bf16x3') or 'sensealg'
+        ('interpolating' / 'backsolve' / 'backsolve_nockpt')."""
         k = {"graphs": _lib.OPT_GRAPHS, "graph_attempts": _lib.OPT_GRAPH_ATTEMPTS,
-             "autotune": _lib.OPT_AUTOTUNE, "wgrad_precision": _lib.OPT_WGRAD_PRECISION}[key]
+             "autotune": _lib.OPT_AUTOTUNE, "wgrad_precision": _lib.OPT_WGRAD_PRECISION,
+             "sensealg": _lib.OPT_SENSEALG}[key]
         if key == "wgrad_precision" and isinstance(value, str):
             value = _lib.PREC[value]
+        if key == "sensealg" and isinstance(value, str):
+            value = _lib.SENSE[value]
+        if key == "sensealg" and getattr(self, "_sense", None) == int(value):
+            return  # unchanged: keep the captured adjoint graph
         self._check(self.lib.node_set_option(self.h, k, int(value)), "node_set_option")
+        if key == "sensealg":
+            self._sense = int(value)
 
     def plans(self):
         """{(role name, layer): (tile rows, tile cols, ksplit)} currently in use."""

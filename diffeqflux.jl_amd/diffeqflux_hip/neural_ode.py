@@ -61,8 +61,19 @@ class _Unsupported:
         return f"{type(self).__name__}()"
 
 
-class BacksolveAdjoint(_Unsupported):
-    pass
+class BacksolveAdjoint:
+    """``BacksolveAdjoint(; autojacvec, checkpointing = true)``: the adjoint
+    integrates ``u`` back from ``u(T)`` together with ``[lambda; mu]``
+    (checkpointing: ``u`` reset to the forward's saved state at each saved
+    time).  On the HIP path with ``autojacvec`` ``ZygoteVJP()`` or unset."""
+
+    def __init__(self, autojacvec=None, checkpointing=True, noisemixing=False):
+        self.autojacvec = autojacvec
+        self.checkpointing = bool(checkpointing)
+        self.noisemixing = noisemixing
+
+    def __repr__(self):
+        return f"BacksolveAdjoint(checkpointing={self.checkpointing})"
 
 
 class TrackerAdjoint(_Unsupported):
@@ -272,16 +283,25 @@ class NeuralODE:
                              save_start=ss, save_end=se, save_everystep=sev)
         return o, interior
 
-    def __call__(self, x, p, st=None):
+    def _sense_mode(self):
         sens = self.sensealg
-        if not isinstance(sens, InterpolatingAdjoint) or not isinstance(sens.autojacvec, ZygoteVJP):
-            raise NotImplementedError(f"sensealg {sens!r} is not on the HIP path")
+        vjp_ok = sens.autojacvec is None or isinstance(sens.autojacvec, ZygoteVJP) \
+            if isinstance(sens, (InterpolatingAdjoint, BacksolveAdjoint)) else False
+        if isinstance(sens, InterpolatingAdjoint) and isinstance(sens.autojacvec, ZygoteVJP):
+            return "interpolating"
+        if isinstance(sens, BacksolveAdjoint) and vjp_ok and not sens.noisemixing:
+            return "backsolve" if sens.checkpointing else "backsolve_nockpt"
+        raise NotImplementedError(f"sensealg {sens!r} is not on the HIP path")
+
+    def __call__(self, x, p, st=None):
+        mode = self._sense_mode()
         vec = np.ndim(x) == 1
         D = self.model.widths[0]
         B = 1 if vec else int(x.shape[0])
         if (x.shape[-1] if not vec else x.shape[0]) != D:
             raise ValueError(f"u0 must have D={D} entries per trajectory")
         h = self._handle(B)
+        h.set_option("sensealg", mode)
         opts, interior = self._opts()
         try:
             import torch

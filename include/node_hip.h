@@ -232,8 +232,17 @@ int32_t node_ragged_backward(node_handle_t* h, const float* dLdout, float* dLdu0
  *                            accepts NODE_PREC_BF16: one MFMA pass instead of three
  *                            on those products; the data path (state, adjoint state,
  *                            and so the step sizes) keeps BF16X3. */
+/*   NODE_OPT_SENSEALG        adjoint of node_backward / node_backward_device
+ *                            (the reference's sensealg keyword, src/neural_de.jl:57):
+ *                            NODE_SENSE_INTERPOLATING (default) InterpolatingAdjoint,
+ *                            u(t) from the forward tape; NODE_SENSE_BACKSOLVE
+ *                            BacksolveAdjoint(checkpointing = true): u integrated
+ *                            back from u(T) with the adjoint, reset to the forward's
+ *                            saved state at each saved time; NODE_SENSE_BACKSOLVE_NOCKPT
+ *                            BacksolveAdjoint(checkpointing = false). */
 enum { NODE_OPT_GRAPHS = 1, NODE_OPT_GRAPH_ATTEMPTS = 2, NODE_OPT_AUTOTUNE = 3,
-       NODE_OPT_WGRAD_PRECISION = 4 };
+       NODE_OPT_WGRAD_PRECISION = 4, NODE_OPT_SENSEALG = 5 };
+enum { NODE_SENSE_INTERPOLATING = 0, NODE_SENSE_BACKSOLVE = 1, NODE_SENSE_BACKSOLVE_NOCKPT = 2 };
 int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value);
 /* Tile plan chosen for a GEMM role (roles as for node_prof_enable below) and
  * layer: tile rows x cols and split-K factor. */

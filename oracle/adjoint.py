@@ -83,6 +83,59 @@ def backward(mlp, p, fwd, dL_du, **solver_kw):
     return yT[:nlam].reshape(u_shape), yT[nlam:], adj
 
 
+def backward_backsolve(mlp, p, fwd, dL_du, checkpointing=True, **solver_kw):
+    """SciMLSensitivity ``BacksolveAdjoint(; checkpointing)``, restated
+    (third-party: SciMLSensitivity >= 7.112, ``ODEBacksolveSensitivityFunction``
+    and ``backsolve_checkpoint_callbacks``); selected by ``sensealg`` at the
+    call in ``src/neural_de.jl:55-58`` (the reference's own test matrix runs it
+    in ``test/BasicNeuralDE/neural_de_tests.jl:34-37``).
+
+    Instead of interpolating the forward solution, ``u`` is integrated back
+    from ``u(T)`` together with the adjoint: ``y = [u; lambda; mu]`` solves
+
+        u' = f(u),  lambda' = -(df/du)^T lambda,  mu' = -(df/dp)^T lambda
+
+    from ``T`` to ``t0`` with the same Tsit5 (error norm = RMS over all
+    ``2 D B + P`` entries), the same cotangent jumps at the saved times, and,
+    with ``checkpointing``, ``u`` reset to the forward solution at every saved
+    time before ``T`` (a callback at the same times as the jumps).
+    Returns (dL/du0 (B, D), dL/dp (P,), adjoint Solution)."""
+    u_shape = np.shape(fwd.u[0]) if fwd.u else np.shape(fwd.steps[0].y0)
+    nlam = int(np.prod(u_shape))
+    P = mlp.nparams
+    jumps, saved = {}, {}
+    for t, g, u in zip(fwd.t, dL_du, fwd.u):
+        g = np.asarray(g).reshape(-1)
+        jumps[t] = jumps.get(t, 0) + g
+        saved[t] = np.asarray(u).reshape(-1)
+    dtype = np.result_type(fwd.steps[0].y0 if fwd.steps else fwd.u[0], p)
+
+    def f(y, t):
+        u = y[:nlam].reshape(u_shape)
+        lam = y[nlam:2 * nlam].reshape(u_shape)
+        fu = mlp.forward(u, p)
+        gx, gp = mlp.vjp(u, p, lam)
+        return np.concatenate([fu.reshape(-1), -gx.reshape(-1), -gp]).astype(dtype, copy=False)
+
+    def affect(y, t):
+        y = y.copy()
+        y[nlam:2 * nlam] += jumps[t].astype(dtype, copy=False)
+        if checkpointing and t != fwd.t1:
+            y[:nlam] = saved[t].astype(dtype, copy=False)
+        return y
+
+    uT = np.asarray(fwd.interp(fwd.t1)).reshape(-1)
+    y0 = np.concatenate([uT, np.zeros(nlam + P)]).astype(dtype)
+    kw = dict(solver_kw)
+    kw.pop("saveat", None)
+    adj = tsit5.solve(f, y0, fwd.t1, fwd.t0, saveat_interior=(),
+                      save_start=False, save_end=True, save_everystep=False,
+                      callback_times=sorted(jumps), affect=affect,
+                      record_dense=False, **kw)
+    yT = adj.u[-1]
+    return yT[nlam:2 * nlam].reshape(u_shape), yT[2 * nlam:], adj
+
+
 def loss_mse_cotangent(pred, target):
     """d/dpred of mean((pred - target)^2) over all saved entries."""
     pred = np.asarray(pred)

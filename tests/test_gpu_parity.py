@@ -211,7 +211,8 @@ REF_KWARGS = [  # test/BasicNeuralDE/neural_de_tests.jl:27-42
 def test_reference_kwargs_matrix(ki, vec):
     """The reference's NeuralODE test matrix through the layer API and torch
     autograd: grads non-zero (the reference's own assertion) and equal to the
-    oracle's; sensealgs off the HIP path raise NotImplementedError."""
+    oracle's (BacksolveAdjoint against the oracle's BacksolveAdjoint);
+    TrackerAdjoint, off the HIP path, raises NotImplementedError."""
     import torch
     import diffeqflux_hip as dfh
     kw = dict(REF_KWARGS[ki])
@@ -229,28 +230,29 @@ def test_reference_kwargs_matrix(ki, vec):
                     "runs to MaxIters")
     xt = torch.tensor(x, requires_grad=True)
     pt = torch.tensor(ps, requires_grad=True)
-    if "sensealg" in kw:
+    if isinstance(kw.get("sensealg"), dfh.TrackerAdjoint):
         with pytest.raises(NotImplementedError):
             node(xt, pt, st)
         return
+    backsolve = isinstance(kw.get("sensealg"), dfh.BacksolveAdjoint)
     sol, _ = node(xt, pt, st)
     loss = sol.array.sum()                      # sum o first o node
     loss.backward()
     assert xt.grad.abs().sum() > 0 and pt.grad.abs().sum() > 0
     m = MLP([2, 50, 2], ["tanh", "identity"])
-    okw = {k: v for k, v in kw.items()}
+    okw = {k: v for k, v in kw.items() if k != "sensealg"}
     B = 1 if vec else 3
     fwd = OA.forward(m, ps.astype(np.float64), x.reshape(B, 2).astype(np.float64), 0.0, 1.0,
                      **okw)
-    if kw:
+    if okw:
         assert np.array_equal(np.array(fwd.t), sol.t)
     elif len(fwd.t) != len(sol.t) or not np.allclose(fwd.t, sol.t, rtol=0, atol=1e-5):
         print(f"  every-step sequences differ (gpu {len(sol.t)} vs oracle {len(fwd.t)} "
               f"states): loss differs, values not compared", flush=True)
         return
     tol = {k: okw.pop(k) for k in ("abstol", "reltol") if k in okw}
-    gu0, gp, _ = OA.backward(m, ps.astype(np.float64), fwd, [np.ones((B, 2))] * len(fwd.t),
-                             **tol)
+    back = OA.backward_backsolve if backsolve else OA.backward
+    gu0, gp, _ = back(m, ps.astype(np.float64), fwd, [np.ones((B, 2))] * len(fwd.t), **tol)
     eu = rel(xt.grad.numpy().reshape(B, 2), gu0)
     ep = rel(pt.grad.numpy(), gp)
     print(f"  retcode {sol.retcode} Nt {len(sol.t)} grad rel err du0 {eu:.2e} dp {ep:.2e}",

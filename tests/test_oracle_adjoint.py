@@ -132,3 +132,68 @@ def test_dropped_sign_would_be_caught():
     finally:
         m.vjp = orig
     assert np.abs(gbad - gp).max() > 1e-2
+
+
+# ---- BacksolveAdjoint restatement: same closed forms / finite differences
+@pytest.mark.parametrize("ckpt", [True, False])
+def test_backsolve_linear_closed_forms(ckpt):
+    m, p, W, u0 = _linear_problem()
+    T, D = 1.3, 3
+    tol = dict(abstol=1e-12, reltol=1e-12)
+    fwd = adj.forward(m, p, u0, 0.0, T, save_everystep=False, save_start=False, **tol)
+    gu0, gp, _ = adj.backward_backsolve(m, p, fwd, [fwd.u[-1]], checkpointing=ckpt, **tol)
+    E = sla.expm(W * T)
+    assert np.abs(gu0 - u0 @ (E.T @ E).T).max() < 1e-10
+    G = np.zeros((D, D))
+    for b in range(u0.shape[0]):
+        uT = E @ u0[b]
+        M = np.zeros((2 * D, 2 * D))
+        M[:D, :D], M[:D, D:], M[D:, D:] = W.T, np.outer(uT, u0[b]), W.T
+        G += sla.expm(M * T)[:D, D:]
+    assert np.abs(gp[:D * D].reshape(D, D).T - G).max() < 1e-9
+
+
+@pytest.mark.parametrize("ckpt", [True, False])
+def test_backsolve_saveat_jumps_closed_form(ckpt):
+    m, p, W, u0 = _linear_problem(seed=1)
+    ts = [0.0, 0.4, 0.9, 1.2]
+    rng = np.random.default_rng(2)
+    cs = [rng.standard_normal(u0.shape) for _ in ts]
+    tol = dict(abstol=1e-12, reltol=1e-12)
+    fwd = adj.forward(m, p, u0, 0.0, 1.2, saveat=ts, **tol)
+    gu0, _, _ = adj.backward_backsolve(m, p, fwd, cs, checkpointing=ckpt, **tol)
+    expect = sum(c @ sla.expm(W * t) for c, t in zip(cs, ts))
+    assert np.abs(gu0 - expect).max() < 1e-9
+
+
+def test_backsolve_spiral_vs_central_fd_and_interpolating():
+    pr = spiral()
+    m, p, u0 = pr["model"], pr["p"], pr["u0"]
+    tol = dict(abstol=1e-10, reltol=1e-10)
+    fwd = adj.forward(m, p, u0, *pr["tspan"], saveat=pr["saveat"], **tol)
+    target = np.array(fwd.u) * 0.9
+    cot = [2 * (u - t) for u, t in zip(fwd.u, target)]
+    gu0, gp, _ = adj.backward_backsolve(m, p, fwd, cot, **tol)
+    gi0, gi, _ = adj.backward(m, p, fwd, cot, **tol)
+    assert np.abs(gp - gi).max() < 1e-6 * max(1.0, np.abs(gi).max())
+    assert np.abs(gu0 - gi0).max() < 1e-6 * max(1.0, np.abs(gi0).max())
+    idx = np.random.default_rng(0).choice(m.nparams, 10, replace=False)
+    fd = _fd_grad(m, p, u0, pr["tspan"], pr["saveat"], target, tol, idx)
+    assert np.abs(fd - gp[idx]).max() < 1e-5 * max(1.0, np.abs(fd).max())
+
+
+def test_backsolve_without_u_integration_would_be_caught():
+    """A restatement that forgot to integrate u (kept u(T) frozen) fails the
+    closed form: plant it by zeroing f's u-part."""
+    m, p, W, u0 = _linear_problem()
+    T = 1.3
+    tol = dict(abstol=1e-12, reltol=1e-12)
+    fwd = adj.forward(m, p, u0, 0.0, T, save_everystep=False, save_start=False, **tol)
+    orig = m.forward
+    try:
+        m.forward = lambda u, pp: np.zeros_like(u)
+        _, gp, _ = adj.backward_backsolve(m, p, fwd, [fwd.u[-1]], checkpointing=False, **tol)
+    finally:
+        m.forward = orig
+    _, gref, _ = adj.backward_backsolve(m, p, fwd, [fwd.u[-1]], checkpointing=False, **tol)
+    assert np.abs(gp - gref).max() > 1e-3
