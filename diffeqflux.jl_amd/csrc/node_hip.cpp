@@ -1763,6 +1763,7 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
     if (value < NODE_SENSE_INTERPOLATING || value > NODE_SENSE_BACKSOLVE_NOCKPT)
       return fail(h, NODE_ERR_INVALID, "unknown sensealg (NODE_SENSE_*)");
     if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "sensealg: a CNF handle has no adjoint");
+    if (value == h->sense) return 0;  // unchanged: the captured graphs stay valid
     h->sense = (int)value;
   } else if (key == NODE_OPT_GRAPH_ATTEMPTS) {
     if (value < 1 || value > 16) return fail(h, NODE_ERR_INVALID, "graph attempts must be 1..16");

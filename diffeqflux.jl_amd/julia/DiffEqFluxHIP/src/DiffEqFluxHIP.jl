@@ -8,14 +8,15 @@ supported case, and falls back to the reference method otherwise:
   * model: a Lux `Chain` of `Dense` layers (identity / tanh / softplus / relu /
     sigmoid), optionally preceded by `x -> x.^3`, with equal input/output width;
   * alg: `Tsit5()`;
-  * sensealg: the default `InterpolatingAdjoint(autojacvec = ZygoteVJP())`;
+  * sensealg: the default `InterpolatingAdjoint(autojacvec = ZygoteVJP())`, or
+    `BacksolveAdjoint(; checkpointing)` (autojacvec unset or `ZygoteVJP()`);
   * kwargs: saveat (range, vector or number), save_everystep, save_start,
     save_end, abstol, reltol, dt, adaptive, dtmax, dtmin, maxiters;
   * u0: `Vector{Float32}` (D) or `Matrix{Float32}` (D x B), t1 > t0.
 
 `ChainRulesCore.rrule(::typeof(hip_neuralode), ...)` is the pullback: it
-calls `node_backward` (the HIP InterpolatingAdjoint) on the forward tape kept
-in the handle.  Julia was not available where this package was written; the
+calls `node_backward` (the HIP InterpolatingAdjoint, or BacksolveAdjoint as
+selected by `NODE_OPT_SENSEALG`) on the forward tape kept in the handle.  Julia was not available where this package was written; the
 call contract it binds is exercised from Python through the same C ABI.
 """
 module DiffEqFluxHIP
@@ -24,7 +25,7 @@ using ChainRulesCore: ChainRulesCore, NoTangent, ZeroTangent, unthunk
 using DiffEqFlux: DiffEqFlux, NeuralODE
 using Lux: Lux, Chain, Dense, WrappedFunction
 using OrdinaryDiffEq: Tsit5
-using SciMLSensitivity: InterpolatingAdjoint, ZygoteVJP
+using SciMLSensitivity: InterpolatingAdjoint, BacksolveAdjoint, ZygoteVJP
 
 const CRC = ChainRulesCore
 
@@ -102,11 +103,21 @@ function chain_spec(model)
     return widths, acts, imap
 end
 
+# NODE_OPT_SENSEALG value of the layer's sensealg (nothing: not on the HIP path)
+const NODE_OPT_SENSEALG = Int32(5)
+function sense_mode(n::NeuralODE)
+    haskey(n.kwargs, :sensealg) || return Int64(0)
+    s = n.kwargs[:sensealg]
+    s isa InterpolatingAdjoint && s.autojacvec isa ZygoteVJP && return Int64(0)
+    if s isa BacksolveAdjoint && (s.autojacvec === nothing || s.autojacvec isa ZygoteVJP)
+        return s.checkpointing ? Int64(1) : Int64(2)
+    end
+    return nothing
+end
+
 supported(n::NeuralODE, x) = chain_spec(n.model) !== nothing && length(n.args) >= 1 &&
     n.args[1] isa Tsit5 && eltype(x) === Float32 && ndims(x) in (1, 2) &&
-    n.tspan[2] > n.tspan[1] &&
-    (!haskey(n.kwargs, :sensealg) || (n.kwargs[:sensealg] isa InterpolatingAdjoint &&
-                                      n.kwargs[:sensealg].autojacvec isa ZygoteVJP))
+    n.tspan[2] > n.tspan[1] && sense_mode(n) !== nothing
 
 # OrdinaryDiffEq's saving keywords -> interior save times + flags
 function save_plan(t0, t1, kw)
@@ -218,6 +229,8 @@ function (n::NeuralODE{<:Chain})(x::AbstractArray{Float32}, p, st)
     supported(n, x) || return invoke(n, Tuple{Any, Any, Any}, x, p, st)  # reference path
     B = ndims(x) == 1 ? 1 : size(x, 2)
     h = handle_for(n, B)
+    check(h, ccall((:node_set_option, libnode()), Int32, (Ptr{Cvoid}, Int32, Int64),
+                   h.ptr, NODE_OPT_SENSEALG, sense_mode(n)), "node_set_option")
     o, interior = make_opts(n)
     # flat Lux/ComponentArray order (test/BasicNeuralDE/neural_de_tests.jl:45); for a
     # ComponentArray `Vector(p)` is already that vector and Zygote restructures dp

@@ -27,6 +27,12 @@ own forward solve; and, for fixed dt, torch-FP64 autograd through the discrete
 forward solver (the continuous adjoint agrees to O(dt^5)).  SciMLSensitivity's
 exact adaptive adjoint step sequence needs Julia (absent): parity unpinned for
 that sequence, pinned to solver tolerance.
+
+``backward_backsolve`` restates ``BacksolveAdjoint(; checkpointing)`` (u
+integrated back with the adjoint); pinned by the same linear closed forms and
+saveat-jump closed form (with and without checkpointing), by central finite
+differences and by agreement with ``backward`` on the spiral problem, and by a
+planted mistake (u left un-integrated) that the closed form catches.
 """
 import numpy as np
 
