@@ -1219,12 +1219,16 @@ void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
 
 // ---------------------------------------------------------------- stage kernels
 // Per-element stage work of the skinny last-layer GEMMs, run over the summed
-// split-K partials by the whole chip: a 256-thread block takes tiles of
-// ST_ROWS rows x 64 columns (thread: column tid % 64, rows RPT (tid / 64) + j).
-// The arithmetic per element is that of the fused epilogues (EpiFwdStage,
-// EpiBwdLast); a thread issues the loads of all its rows before any store.
-constexpr int ST_ROWS = 4;
-constexpr int RPT = ST_ROWS / 4;
+// split-K partials by the whole chip: a ST_THREADS block takes tiles of
+// ST_ROWS rows x 64 columns (thread: column tid % 64, rows RPT (tid / 64) + j;
+// transposed outputs: column tid / ST_G, rows RPT (tid % ST_G) + j, so each
+// column's rows of a tile are one contiguous run).  The arithmetic per element
+// is that of the fused epilogues (EpiFwdStage, EpiBwdLast); a thread issues
+// the loads of all its rows before any store.
+constexpr int ST_THREADS = 1024;
+constexpr int ST_ROWS = 16;
+constexpr int RPT = ST_ROWS * 64 / ST_THREADS;
+constexpr int ST_G = ST_THREADS / 64;
 constexpr int ST_MAX_BLOCKS = 4096;
 int stage_grid(int rows, int cols) {
   const int t = (rows / ST_ROWS) * (cols / 64);
@@ -1238,7 +1242,7 @@ __device__ __forceinline__ float sum_splits(const float* ws, int ks, int64_t pst
 }
 
 template <class PP>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(ST_THREADS)
     k_fwd_stage(EpiFwdStage e, const float* __restrict__ ws, int ks, int64_t pstride, int ldw,
                 int tc, int ntile, const Ctl* __restrict__ ctl, int gate) {
   if (!gate_ok(ctl, gate)) return;
@@ -1321,7 +1325,7 @@ __global__ void __launch_bounds__(256)
 }
 
 template <class PP>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(ST_THREADS)
     k_bwd_stage(EpiBwdLast e, const float* __restrict__ ws, int ks, int64_t pstride, int ldw,
                 int tc, int ntile, const Ctl* __restrict__ ctl, int gate) {
   if (!gate_ok(ctl, gate)) return;
@@ -1483,8 +1487,7 @@ __global__ void __launch_bounds__(256)
         lt[1][cl][RPT * rq + j] = xv[j];
       }
       __syncthreads();
-      // transposed: thread -> column tid / 4, rows RPT (tid % 4) + j
-      const int cc = tid >> 2, rr = RPT * (tid & 3);
+      const int cc = tid / ST_G, rr = RPT * (tid % ST_G);
       const int64_t col = tcol + cc;
 #pragma unroll
       for (int j = 0; j < RPT; ++j) {
@@ -1501,14 +1504,14 @@ void launch_fwd_stage_kernel(int prec, const EpiFwdStage& e, const float* ws, in
                              int64_t pstride, int ldw, int rows, int cols, const Ctl* ctl,
                              int gate, hipStream_t s) {
   const int tc = cols / 64, nt = (rows / ST_ROWS) * tc;
-  DISPATCH_PREC(prec, hipLaunchKernelGGL(k_fwd_stage<PP>, dim3(stage_grid(rows, cols)), dim3(256),
+  DISPATCH_PREC(prec, hipLaunchKernelGGL(k_fwd_stage<PP>, dim3(stage_grid(rows, cols)), dim3(ST_THREADS),
                                          0, s, e, ws, ks, pstride, ldw, tc, nt, ctl, gate));
 }
 void launch_bwd_stage_kernel(int prec, const EpiBwdLast& e, const float* ws, int ks,
                              int64_t pstride, int ldw, int rows, int cols, const Ctl* ctl,
                              int gate, hipStream_t s) {
   const int tc = cols / 64, nt = (rows / ST_ROWS) * tc;
-  DISPATCH_PREC(prec, hipLaunchKernelGGL(k_bwd_stage<PP>, dim3(stage_grid(rows, cols)), dim3(256),
+  DISPATCH_PREC(prec, hipLaunchKernelGGL(k_bwd_stage<PP>, dim3(stage_grid(rows, cols)), dim3(ST_THREADS),
                                          0, s, e, ws, ks, pstride, ldw, tc, nt, ctl, gate));
 }
 

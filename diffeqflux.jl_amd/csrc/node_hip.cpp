@@ -764,6 +764,20 @@ void init_ctl(node_handle* h, Ctl& c, const node_solve_opts& o, double t0, doubl
   (void)h;
 }
 
+// Step attempts to enqueue before the next poll: the span left over the
+// current step size, at least the save (tstop) times still ahead, plus one for
+// a rejection, in whole units of `unit` attempts (one captured graph), at most
+// 16.  Attempts enqueued past the end of the solve return at once on the
+// device but still cost their launches, so the chunk follows the estimate
+// rather than growing blindly.
+int64_t next_chunk(const Ctl& c, int64_t saves_left, int unit) {
+  const double span = std::fabs(c.t_end - c.t), step = std::fabs(c.dt);
+  const double est = step > 0.0 ? std::ceil(std::min(span / step, 64.0)) : 64.0;
+  int64_t n = std::max<int64_t>((int64_t)est, saves_left) + 1;
+  n = (n + unit - 1) / unit * unit;
+  return std::min<int64_t>(std::max<int64_t>(n, unit), std::max(16, unit));
+}
+
 int poll_ctl(node_handle* h) {
   HIPCHK(h, hipMemcpyAsync(h->hctl, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -855,6 +869,9 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
     if (h->hctl->done) break;
     if (enqueued > c.maxiters + 64)
       return fail(h, NODE_ERR_STATE, "step controller did not terminate (internal error)");
+    if (enqueued > 0)
+      chunk = next_chunk(*h->hctl, std::max<int64_t>(0, h->hctl->n_save - h->hctl->save_next),
+                         graphs ? ga : 1);
     enqueued += chunk;
     if (int rc = ensure_tape(h, h->hctl->n + chunk + 2)) return rc;
     if (graphs) {
@@ -877,7 +894,6 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
       for (int64_t k = 0; k < chunk; ++k) attempt();
     }
     HIPCHK(h, hipGetLastError());
-    chunk = std::min<int64_t>(chunk * 2, 16);
   }
   // pending saves of the final step
   launch_fwd_step_begin(sg, h->tape, h->X0, xpl, h->save_times, h->out_dev, h->ctl, h->stream);
@@ -1042,6 +1058,8 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
     if (h->hctl->done) break;
     if (enqueued > c.maxiters + 64)
       return fail(h, NODE_ERR_STATE, "adjoint step controller did not terminate (internal error)");
+    if (enqueued > 0)
+      chunk = next_chunk(*h->hctl, h->hctl->save_next + 1, graphs ? ga : 1);
     enqueued += chunk;
     if (graphs) {
       if (!h->adj_graph) {
@@ -1058,7 +1076,6 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
       for (int64_t k = 0; k < chunk; ++k) attempt();
     }
     HIPCHK(h, hipGetLastError());
-    chunk = std::min<int64_t>(chunk * 2, 16);
   }
   launch_adj_step_begin(sg, ab, h->ctl, GATE_ALWAYS, h->stream);
   if (int rc = poll_ctl(h)) return rc;
