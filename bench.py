@@ -152,9 +152,10 @@ def gemm_shape(role, layer, B):
 # HBM bytes per launch of a kernel, from two rocprofv3 --pmc runs of this
 # bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the same
 # counters read for a plain 256 MiB device copy (FETCH_SIZE 0.500x of the bytes
-# read, WRITE_SIZE 1.000x of the bytes written): bwd_data at K = 1024 fetches
-# 49,210 KiB x 2.0 and writes 32,768 KiB per launch.
-TRAFFIC = {("bf16x3", 3, (4096, 1024, 1024)): (49210 * 2.0 + 32768) * 1024}
+# read, WRITE_SIZE 1.000x of the bytes written): bwd_data at K = 1024 (8-wave
+# 128x128 tiles, act' derived from the layer output) fetches 32,921 KiB x 2.0
+# and writes 32,896 KiB per launch.
+TRAFFIC = {("bf16x3", 3, (4096, 1024, 1024)): (32921 * 2.0 + 32896) * 1024}
 
 
 def profile_dominant(run):
