@@ -16,6 +16,11 @@ Follows ``src/ffjord.jl``:
   * ``__forward_ffjord`` (:138-178): solve from ``[x; 0]`` saving only the end
     state, ``logpz = -|z|²/2 - D/2 log 2π`` (standard-normal base), and
     ``logpx = logpz - Δlogp``.
+  * ``backward``: the InterpolatingAdjoint of that solve (``:143``) with the
+    second-order VJP of the Hutchinson term (``trace_vjp``).  Pinned by torch
+    FP64 double-backward of the same augmented right-hand side, central finite
+    differences of ``forward``'s loss, and a planted mistake (the
+    second-order source dropped) that the finite differences catch.
 """
 import math
 
@@ -71,3 +76,107 @@ def forward(mlp, p, x, t0, t1, probe=None, time_concat=False, **solver_kw):
     z, dl = yT[:, :D], yT[:, D]
     logpz = -0.5 * np.sum(z * z, axis=1) - 0.5 * D * math.log(2 * math.pi)
     return logpz - dl, z, dl, sol
+
+
+# ---------------------------------------------------------------- adjoint
+def _act_d2(code, a):
+    """Second derivative of the activation at the pre-activation ``a``."""
+    from .mlp import ACT_TANH, ACT_SOFTPLUS, ACT_SIGMOID
+    if code == ACT_TANH:
+        t = np.tanh(a)
+        return -2.0 * t * (1.0 - t * t)
+    if code == ACT_SOFTPLUS:
+        s = 1.0 / (1.0 + np.exp(-a))
+        return s * (1.0 - s)
+    if code == ACT_SIGMOID:
+        s = 1.0 / (1.0 + np.exp(-a))
+        return s * (1.0 - s) * (1.0 - 2.0 * s)
+    return np.zeros_like(a)
+
+
+def trace_vjp(mlp, p, x, probe, lam_z, lam_d):
+    """VJP of the augmented right-hand side ``g = [f(x); -eᵀ (∂f/∂z) e]`` at
+    the model input ``x`` (B, in; ``z`` its first D columns) with cotangents
+    ``lam_z`` (B, D) and ``lam_d`` (B,): returns (∂/∂x, ∂/∂p) of
+    ``Σ_b lam_z·f - lam_d·s`` with ``s = eᵀ J e``, ``J = ∂f/∂z``.
+
+    What Zygote's pullback through the Hutchinson term computes
+    (``src/ffjord.jl:95-115`` differentiated by the InterpolatingAdjoint the
+    layer selects at ``:143``): ``s`` is the tangent chain
+    ``ṫ_{-1} = [e; 0]``, ``ȧ_l = ṫ_{l-1} W_l``, ``ṫ_l = σ'(a_l) ȧ_l``,
+    ``s = e·ṫ_{L-1}``; its reverse gives per layer ``ǡ_l = ṫ̄_l σ'(a_l)``,
+    ``W̄_l += ṫ_{l-1}ᵀ ǡ_l`` and the second-order source
+    ``ṫ̄_l ȧ_l σ''(a_l)`` added to the primal backward chain's ``ā_l``."""
+    from .mlp import act_fwd, act_deriv
+    layers = mlp.unflatten(p)
+    L = len(layers)
+    D = probe.shape[1]
+    h, hs, As, ts, adots = x, [x], [], [], []
+    t = np.zeros_like(x)
+    t[:, :D] = probe
+    ts.append(t)
+    for li, ((Wt, b), act) in enumerate(zip(layers, mlp.acts)):
+        a = h @ Wt + b
+        h = act_fwd(act, a)
+        ad = ts[-1] @ Wt
+        As.append(a)
+        hs.append(h)
+        adots.append(ad)
+        ts.append(act_deriv(act, a, h) * ad)
+    gp = np.zeros(mlp.nparams, dtype=np.result_type(x, p, lam_z))
+    c = (-lam_d)[:, None]
+    # tangent chain reverse, seeded by d(c s)/d t_{L-1} = c e
+    tbar = c * probe
+    src = [None] * L
+    for li in range(L - 1, -1, -1):
+        Wt, _ = layers[li]
+        act = mlp.acts[li]
+        abar_t = tbar * act_deriv(act, As[li], hs[li + 1])
+        src[li] = tbar * adots[li] * _act_d2(act, As[li])
+        w0, b0, _ = mlp.offsets[li]
+        gp[w0:b0] += (ts[li].T @ abar_t).ravel()
+        tbar = abar_t @ Wt.T
+    # primal chain with the second-order sources
+    g = lam_z
+    for li in range(L - 1, -1, -1):
+        Wt, _ = layers[li]
+        act = mlp.acts[li]
+        abar = g * act_deriv(act, As[li], hs[li + 1]) + src[li]
+        w0, b0, e = mlp.offsets[li]
+        gp[w0:b0] += (hs[li].T @ abar).ravel()
+        gp[b0:e] += abar.sum(axis=0)
+        g = abar @ Wt.T
+    return g, gp
+
+
+def backward(mlp, p, fwd, probe, dL_dz, dL_ddlogp, time_concat=False, **solver_kw):
+    """InterpolatingAdjoint of the FFJORD solve (the layer's default
+    ``sensealg``, ``src/ffjord.jl:143``): the adjoint state
+    ``[λ_z; λ_Δ; μ]`` solves, from T back to t0,
+    ``λ_z' = -(∂g/∂z)ᵀ λ``, ``λ_Δ' = 0``, ``μ' = -(∂g/∂p)ᵀ λ`` with
+    ``λ(T) = [dL/dz(T); dL/dΔlogp(T)]`` and ``z(t)`` from the forward's dense
+    interpolant.  Returns (dL/dx (B, D), dL/dp (P,), adjoint Solution)."""
+    B, D = probe.shape
+    P = mlp.nparams
+    dtype = np.result_type(fwd.steps[0].y0, p)
+
+    def f(y, t):
+        lz = y[:B * D].reshape(B, D)
+        ld = y[B * D:B * D + B]
+        z = np.asarray(fwd.interp(t))[:, :D]
+        gx, gp = trace_vjp(mlp, p, model_input(z, t, time_concat), probe.astype(dtype), lz, ld)
+        return np.concatenate([-gx[:, :D].reshape(-1), np.zeros(B, dtype), -gp]).astype(dtype)
+
+    y0 = np.concatenate([np.asarray(dL_dz, dtype).reshape(-1),
+                         np.asarray(dL_ddlogp, dtype).reshape(-1), np.zeros(P, dtype)])
+    kw = dict(solver_kw)
+    adj = tsit5.solve(f, y0, fwd.t1, fwd.t0, saveat_interior=(), save_start=False,
+                      save_end=True, save_everystep=False, record_dense=False, **kw)
+    yT = adj.u[-1]
+    return yT[:B * D].reshape(B, D), yT[B * D + B:], adj
+
+
+def nll_cotangents(z, B):
+    """Cotangents of the mean negative log-likelihood ``-mean(logpx)`` at T:
+    ``dL/dz = z / B``, ``dL/dΔlogp = 1 / B`` (``logpx = logpz - Δlogp``)."""
+    return z / B, np.full(z.shape[0], 1.0 / B)

@@ -85,3 +85,95 @@ def test_time_concatenation_feeds_t():
     assert not np.allclose(f(y, t)[:, :D], f(y, 0.0)[:, :D])
     planted = m.forward(np.concatenate([y[:, :D], np.zeros((4, 1))], 1), p)
     assert not np.allclose(f(y, t)[:, :D], planted)
+
+
+# ---- FFJORD adjoint (second-order VJP of the Hutchinson term)
+def _small_cnf(acts, tcat, seed=0):
+    rng = np.random.default_rng(seed)
+    D, H = 3, 8
+    m = MLP([D + (1 if tcat else 0), H, H, D], acts)
+    p = m.glorot_init(rng)
+    for (w0, b0, e) in m.offsets:
+        p[b0:e] = 0.1 * rng.standard_normal(e - b0)
+    return m, p, D, rng
+
+
+@pytest.mark.parametrize("acts", [["tanh", "tanh", "identity"], ["softplus", "sigmoid", "identity"]])
+@pytest.mark.parametrize("tcat", [False, True])
+def test_trace_vjp_matches_torch_double_backward(acts, tcat):
+    import torch
+    m, p, D, rng = _small_cnf(acts, tcat)
+    B = 4
+    x = rng.standard_normal((B, m.widths[0]))
+    e = rng.choice([-1.0, 1.0], size=(B, D))
+    lz = rng.standard_normal((B, D))
+    ld = rng.standard_normal(B)
+    gx, gp = FF.trace_vjp(m, p, x, e, lz, ld)
+    xt = torch.tensor(x, requires_grad=True)
+    pt = torch.tensor(p, requires_grad=True)
+    acts_t = {"tanh": torch.tanh, "softplus": torch.nn.functional.softplus,
+              "sigmoid": torch.sigmoid, "identity": lambda v: v}
+
+    def f(xx):
+        h = xx
+        for (i, j), (w0, b0, e_), a in zip(zip(m.widths[:-1], m.widths[1:]), m.offsets, acts):
+            h = acts_t[a](h @ pt[w0:b0].reshape(i, j) + pt[b0:e_])
+        return h
+    fx = f(xt)
+    et = torch.tensor(e)
+    vj = torch.autograd.grad(fx, xt, grad_outputs=et, create_graph=True)[0][:, :D]
+    s = (vj * et).sum(1)
+    obj = (torch.tensor(lz) * fx).sum() - (torch.tensor(ld) * s).sum()
+    tgx, tgp = torch.autograd.grad(obj, [xt, pt])
+    assert np.abs(gx - tgx.numpy()).max() < 1e-12 * max(1.0, np.abs(gx).max())
+    assert np.abs(gp - tgp.numpy()).max() < 1e-12 * max(1.0, np.abs(gp).max())
+
+
+def _nll(m, p, x, e, tcat, tol):
+    lp, _, _, _ = FF.forward(m, p, x, 0.0, 0.7, probe=e, time_concat=tcat, **tol)
+    return -float(np.mean(lp))
+
+
+@pytest.mark.parametrize("tcat", [False, True])
+def test_ffjord_adjoint_vs_central_fd(tcat):
+    m, p, D, rng = _small_cnf(["tanh", "tanh", "identity"], tcat, seed=1)
+    B = 3
+    x = rng.standard_normal((B, D))
+    e = rng.choice([-1.0, 1.0], size=(B, D))
+    tol = dict(abstol=1e-11, reltol=1e-11)
+    lp, z, dl, fwd = FF.forward(m, p, x, 0.0, 0.7, probe=e, time_concat=tcat, **tol)
+    cz, cd = FF.nll_cotangents(z, B)
+    gx, gp, _ = FF.backward(m, p, fwd, e, cz, cd, time_concat=tcat, **tol)
+    h = 1e-6
+    idx = np.random.default_rng(5).choice(m.nparams, 12, replace=False)
+    fd = []
+    for j in idx:
+        d = np.zeros_like(p); d[j] = h
+        fd.append((_nll(m, p + d, x, e, tcat, tol) - _nll(m, p - d, x, e, tcat, tol)) / (2 * h))
+    fd = np.array(fd)
+    assert np.abs(fd - gp[idx]).max() < 1e-6 * max(1.0, np.abs(fd).max())
+    fx = []
+    for b in range(B):
+        for c in range(D):
+            d = np.zeros_like(x); d[b, c] = h
+            fx.append((_nll(m, p, x + d, e, tcat, tol) - _nll(m, p, x - d, e, tcat, tol)) / (2 * h))
+    assert np.abs(np.array(fx).reshape(B, D) - gx).max() < 1e-6 * max(1.0, np.abs(gx).max())
+
+
+def test_ffjord_adjoint_without_second_order_source_is_caught():
+    """Dropping the σ'' source (a plausible mistake) fails the FD pin."""
+    m, p, D, rng = _small_cnf(["tanh", "tanh", "identity"], False, seed=2)
+    B = 3
+    x = rng.standard_normal((B, D))
+    e = rng.choice([-1.0, 1.0], size=(B, D))
+    tol = dict(abstol=1e-11, reltol=1e-11)
+    lp, z, dl, fwd = FF.forward(m, p, x, 0.0, 0.7, probe=e, **tol)
+    cz, cd = FF.nll_cotangents(z, B)
+    orig = FF._act_d2
+    try:
+        FF._act_d2 = lambda code, a: np.zeros_like(a)
+        _, gbad, _ = FF.backward(m, p, fwd, e, cz, cd, **tol)
+    finally:
+        FF._act_d2 = orig
+    _, gp, _ = FF.backward(m, p, fwd, e, cz, cd, **tol)
+    assert np.abs(gbad - gp).max() > 1e-4 * np.abs(gp).max()
