@@ -395,6 +395,9 @@ __device__ __forceinline__ float act_d_y(int a, float y) {
   }
 }
 
+// activations whose derivative the epilogues derive from the output
+__device__ __forceinline__ bool act_from_output_d(int a) { return a == 0 || a == 2 || a == 3; }
+
 // act' of this thread's accumulator
 // elements, loaded while the GEMM's last K tile is computed (padded columns
 // read the zero-filled padding)
@@ -449,6 +452,39 @@ __device__ __forceinline__ void bwd_data_prefetch_lds(const EpiBwdData& e, unsig
   }
 }
 
+// optional bias column sums, then the row-major + transposed delta tiles
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void bwd_data_finish(const EpiBwdData& e, f32x16 (&acc)[MR][MC],
+                                                const TileGeo& g, int lane, unsigned char* lds) {
+  using T = typename PP::T;
+  if (e.bsum) {
+    // column sums of the tile (fixed order: registers, lane halves, waves)
+    constexpr int WR = BR / (MR * 32);
+    float* red = (float*)lds;  // [WR][BC]
+    const int wr = g.lr0 / (BR / WR);
+    __syncthreads();
+#pragma unroll
+    for (int mj = 0; mj < MC; ++mj) {
+      float cs = 0.f;
+#pragma unroll
+      for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) cs += acc[mi][mj][v];
+      cs += __shfl_xor(cs, 32);
+      if (lane < 32) red[wr * BC + g.lc0 + mj * 32 + lane] = cs;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < BC; i += NT) {
+      float sm = 0.f;
+#pragma unroll
+      for (int w = 0; w < WR; ++w) sm += red[w * BC + i];
+      e.bsum[(int64_t)(g.row0 / BR) * e.ldo + g.col0 + i] = sm;
+    }
+  }
+  tile_write<T, PP::SPLIT, BR, BC, NT>(lds, acc, g, lane, (T*)e.Dl, e.ldo, e.pd, (T*)e.DlT,
+                                       e.ldt, e.pdt);
+}
+
 template <class PP, int BR, int BC, int NT, int MR, int MC>
 __device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR][MC],
                                           const f32x16 (&pre)[MR][MC], const TileGeo& g,
@@ -490,32 +526,76 @@ __device__ __forceinline__ void epi_apply(const EpiBwdData& e, f32x16 (&acc)[MR]
         for (int v = 0; v < 16; ++v) acc[mi][mj][v] = cv ? acc[mi][mj][v] * pre[mi][mj][v] : 0.f;
       }
     }
-  if (e.bsum) {
-    // column sums of the tile (fixed order: registers, lane halves, waves)
-    constexpr int WR = BR / (MR * 32);
-    float* red = (float*)lds;  // [WR][BC]
-    const int wr = g.lr0 / (BR / WR);
-    __syncthreads();
+  bwd_data_finish<PP, BR, BC, NT>(e, acc, g, lane, lds);
+}
+
+// FFJORD adjoint: primal data gradient plus the second-order source (act'
+// loaded in the epilogue; the source read per element)
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiBwdSrc& es, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& g, int lane, const Ctl*, int,
+                                          unsigned char* lds) {
+  using T = typename PP::T;
+  using ST = typename PP::ST;
+  const EpiBwdData& e = es.d;
+#pragma unroll
+  for (int mi = 0; mi < MR; ++mi)
 #pragma unroll
     for (int mj = 0; mj < MC; ++mj) {
-      float cs = 0.f;
+      const int c = g.cb + mj * 32 + (lane & 31);
+      const bool cv = c < e.C;
 #pragma unroll
-      for (int mi = 0; mi < MR; ++mi)
+      for (int q = 0; q < 4; ++q) {
+        const int r0 = g.rb + mi * 32 + q * 8 + (lane >> 5) * 4;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) cs += acc[mi][mj][v];
-      cs += __shfl_xor(cs, 32);
-      if (lane < 32) red[wr * BC + g.lc0 + mj * 32 + lane] = cs;
+        for (int p = 0; p < 4; ++p) {
+          const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
+          const float d = e.yact >= 0 ? act_d_y(e.yact, get<PP>((const T*)e.Y + o, e.py))
+                                      : tof<ST>(((const ST*)e.S)[o]);
+          const int v = q * 4 + p;
+          acc[mi][mj][v] = cv ? acc[mi][mj][v] * d + es.src[o] : 0.f;
+        }
+      }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < BC; i += NT) {
-      float sm = 0.f;
+  bwd_data_finish<PP, BR, BC, NT>(e, acc, g, lane, lds);
+}
+
+// FFJORD adjoint: tangent-backward chain (see EpiTanBwd)
+template <class PP, int BR, int BC, int NT, int MR, int MC>
+__device__ __forceinline__ void epi_apply(const EpiTanBwd& e, f32x16 (&acc)[MR][MC],
+                                          const TileGeo& g, int lane, const Ctl*, int,
+                                          unsigned char* lds) {
+  using T = typename PP::T;
+  using ST = typename PP::ST;
+  const int yact = act_from_output_d(e.act) ? e.act : -1;
 #pragma unroll
-      for (int w = 0; w < WR; ++w) sm += red[w * BC + i];
-      e.bsum[(int64_t)(g.row0 / BR) * e.ldo + g.col0 + i] = sm;
+  for (int mi = 0; mi < MR; ++mi)
+#pragma unroll
+    for (int mj = 0; mj < MC; ++mj) {
+      const int c = g.cb + mj * 32 + (lane & 31);
+      const bool cv = c < e.C;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r0 = g.rb + mi * 32 + q * 8 + (lane >> 5) * 4;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int64_t o = (int64_t)(r0 + p) * e.ldo + c;
+          const float y = get<PP>((const T*)e.Y + o, e.py);
+          const float d = yact >= 0 ? act_d_y(yact, y) : tof<ST>(((const ST*)e.S)[o]);
+          const float t = get<PP>((const T*)e.Tf + o, e.ptf);
+          float sq = 0.f;  // adot * act''
+          if (e.act == 1) sq = -2.f * y * t;
+          else if (e.act == 2) sq = t * (1.f - d);
+          else if (e.act == 4) sq = t * (1.f - 2.f * y);
+          const int v = q * 4 + p;
+          const float tb = acc[mi][mj][v];
+          e.src[o] = cv ? tb * sq : 0.f;
+          acc[mi][mj][v] = cv ? tb * d : 0.f;
+        }
+      }
     }
-  }
-  tile_write<T, PP::SPLIT, BR, BC, NT>(lds, acc, g, lane, (T*)e.Dl, e.ldo, e.pd, (T*)e.DlT,
-                                       e.ldt, e.pdt);
+  tile_write<T, PP::SPLIT, BR, BC, NT>(lds, acc, g, lane, (T*)e.Ab, e.ldo, e.pab, (T*)e.AbT,
+                                       e.ldt, e.pabt);
 }
 
 template <class PP, int BR, int BC, int NT, int MR, int MC>
@@ -1216,6 +1296,10 @@ void launch_gemm_partial(int prec, int cfg, const GemmArgs& g, const EpiPartial&
                          const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
                        const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
+void launch_gemm_tanbwd(int prec, int cfg, const GemmArgs& g, const EpiTanBwd& e,
+                        const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
+void launch_gemm_bwdsrc(int prec, int cfg, const GemmArgs& g, const EpiBwdSrc& e,
+                        const Ctl* ctl, int gate, hipStream_t s) { DISPATCH_PREC(prec, launch_t<PP>(cfg, g, e, ctl, gate, s)); }
 
 // ---------------------------------------------------------------- stage kernels
 // Per-element stage work of the skinny last-layer GEMMs, run over the summed
@@ -1351,6 +1435,7 @@ __global__ void __launch_bounds__(ST_THREADS)
     for (int i = 0; i < 7; ++i) w[i] = ctl->st_w[s + 1][i];
   }
   const float* te = e.tape + (int64_t)tj * e.ent;
+  const float tnext = stage_out ? (float)(ctl->t + dC[s + 1] * ctl->dt) : 0.f;
   double sum = 0.0, flag = 0.0;
   for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const int tr0 = (tile / tc) * ST_ROWS, tcol = (tile % tc) * 64;
@@ -1364,9 +1449,11 @@ __global__ void __launch_bounds__(ST_THREADS)
       const int r = r0 + j;
       const int64_t idx = (int64_t)r * e.D + c;
       if (!cv) continue;
-      float g = sum_splits(ws, ks, pstride, (int64_t)r * ldw + c);
+      // (a CNF's log p column: k = 0, its lambda stays constant)
+      const bool zc = !e.cnf || c < e.dm;
+      float g = zc ? sum_splits(ws, ks, pstride, (int64_t)r * ldw + c) : 0.f;
       if (e.map == 1) { const float u = e.Ucur[idx]; g *= 3.f * u * u; }
-      const float k = -g;
+      const float k = zc ? -g : 0.f;
       kv[j] = e.mode == SM_PLAIN ? g : k;
       const bool live = r < e.B;
       if (bs && e.mode != SM_PLAIN) {
@@ -1421,6 +1508,7 @@ __global__ void __launch_bounds__(ST_THREADS)
           }
           uv[j] = u;
           xv[j] = map_f(e.map, u);
+          if (e.cnf && c >= e.dm) xv[j] = e.tcat && c == e.dm && r < e.B ? tnext : 0.f;
         } else {
           float ut = cBT[6] * k;
 #pragma unroll
@@ -1474,16 +1562,17 @@ __global__ void __launch_bounds__(ST_THREADS)
     if (stage_out) {
       // next-stage inputs lambda_s and x = map(u(t_s)): row-major (padded
       // columns get zeros) and transposed through LDS
+      const bool lz = !e.cnf || c < e.dm;   // lambda columns the model sees
 #pragma unroll
       for (int j = 0; j < RPT; ++j) {
         const int r = r0 + j;
-        put<PP>(L + (int64_t)r * e.ldl + c, e.pl, lsv[j]);
+        put<PP>(L + (int64_t)r * e.ldl + c, e.pl, lz ? lsv[j] : 0.f);
         put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, xv[j]);
       }
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < RPT; ++j) {
-        lt[0][cl][RPT * rq + j] = lsv[j];
+        lt[0][cl][RPT * rq + j] = lz ? lsv[j] : 0.f;
         lt[1][cl][RPT * rq + j] = xv[j];
       }
       __syncthreads();
@@ -1848,7 +1937,11 @@ void launch_fwd_controller(Ctl* ctl, const double* part, int n_part, const doubl
 // ---------------------------------------------------------------- adjoint
 template <class PP>
 __device__ __forceinline__ void adj_prep_stage_u(const StateGeom& sg, const AdjBufs& ab,
-                                                 int64_t i, float lam_s, float u);
+                                                 int64_t i, float lam_s, float u, float t);
+// time of interpolation slot `slot` of the next attempt (7: initial-dt probe)
+__device__ __forceinline__ float slot_time(const Ctl* c, int slot) {
+  return (float)(slot == 7 ? c->t + c->tdir * c->dt0 : c->t + dC[slot] * c->dt);
+}
 template <class PP>
 __device__ __forceinline__ void adj_prep_stage(const StateGeom& sg, const AdjBufs& ab,
                                                const Ctl* c, int slot, int64_t i, float lam_s) {
@@ -1860,17 +1953,21 @@ __device__ __forceinline__ void adj_prep_stage(const StateGeom& sg, const AdjBuf
     for (int j = 0; j < 7; ++j) sm += c->st_w[slot][j] * te[(1 + j) * sg.sv + i];
     u += c->st_dt[slot] * sm;
   }
-  adj_prep_stage_u<PP>(sg, ab, i, lam_s, u);
+  adj_prep_stage_u<PP>(sg, ab, i, lam_s, u, sg.tcat ? slot_time(c, slot) : 0.f);
 }
-// stage inputs (lambda_s row-major + transposed, x = map(u_s) likewise, u_s)
+// stage inputs (lambda_s row-major + transposed, x = map(u_s) likewise, u_s);
+// a CNF's log p column (col == dm) feeds no lambda and, time-concatenated,
+// the stage time t to the model
 template <class PP>
 __device__ __forceinline__ void adj_prep_stage_u(const StateGeom& sg, const AdjBufs& ab,
-                                                 int64_t i, float lam_s, float u) {
+                                                 int64_t i, float lam_s, float u, float t) {
   using T = typename PP::T;
   const int r = (int)(i / sg.D), col = (int)(i % sg.D);
-  put<PP>((T*)ab.L + (int64_t)r * ab.ldl + col, ab.pl, lam_s);
-  put<PP>((T*)ab.LT + (int64_t)col * ab.ldlt + r, ab.plt, lam_s);
-  const float x = map_f(sg.map, u);
+  const bool zc = col < sg.dm;
+  const float lv = zc ? lam_s : 0.f;
+  put<PP>((T*)ab.L + (int64_t)r * ab.ldl + col, ab.pl, lv);
+  put<PP>((T*)ab.LT + (int64_t)col * ab.ldlt + r, ab.plt, lv);
+  const float x = zc ? map_f(sg.map, u) : (sg.tcat && r < sg.B ? t : 0.f);
   put<PP>((T*)ab.X0 + (int64_t)r * ab.ldx + col, ab.px, x);
   put<PP>((T*)ab.X0T + (int64_t)col * ab.ldxt + r, ab.pxt, x);
   ab.U[i] = u;
@@ -1911,7 +2008,7 @@ __global__ void k_adj_initdt_prep(StateGeom sg, AdjBufs ab, const Ctl* c) {
   const int64_t n = (int64_t)sg.Bp * sg.D;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    if (ab.ubs) adj_prep_stage_u<PP>(sg, ab, i, ab.lam[i] + h * ab.kl[i], ab.ubs[i] + h * ab.ku[i]);
+    if (ab.ubs) adj_prep_stage_u<PP>(sg, ab, i, ab.lam[i] + h * ab.kl[i], ab.ubs[i] + h * ab.ku[i], 0.f);
     else adj_prep_stage<PP>(sg, ab, c, 7, i, ab.lam[i] + h * ab.kl[i]);
   }
 }
@@ -1952,8 +2049,8 @@ __global__ void k_adj_step_begin(StateGeom sg, AdjBufs ab, const Ctl* c, int nbs
       if (acc && jidx >= 0 && ab.ckpt && r < sg.B) ub = ab.fsaved[(int64_t)jidx * sg.B * sg.D + i];
       if (acc) { ab.ubs[i] = ub; ab.ku[i] = ab.ku[6 * sg.sv + i]; }
       if (done) continue;
-      if (re) adj_prep_stage_u<PP>(sg, ab, i, l, ub);
-      else adj_prep_stage_u<PP>(sg, ab, i, l + h * ab.kl[i], ub + h * ab.ku[i]);
+      if (re) adj_prep_stage_u<PP>(sg, ab, i, l, ub, 0.f);
+      else adj_prep_stage_u<PP>(sg, ab, i, l + h * ab.kl[i], ub + h * ab.ku[i], 0.f);
       continue;
     }
     if (done) continue;

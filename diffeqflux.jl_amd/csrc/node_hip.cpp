@@ -62,8 +62,20 @@ struct node_handle {
   int wg_prec = 0;  // precision of the weight-gradient products (<= prec)
   bool cnf = false;        // continuous normalizing flow: state [z; log p]
   bool cnf_active = false; // inside node_cnf_forward (probe set)
+  bool cnf_adj = false;    // inside node_cnf_backward (tangent chain on)
+  bool cnf_fwd = false;    // the last solve was a CNF forward (adjoint available)
   float* probe = nullptr;  // CNF probe e, FP32 [Bp][dm]
   float* fbuf = nullptr;   // CNF model output f, FP32 [Bp][dm]
+  std::vector<float> probe_h;  // host copy of the last CNF forward's probe [B][dm]
+  // CNF adjoint (second-order VJP of the Hutchinson term): tangent t_l and
+  // its cotangent row-major per layer (operand format), the FP32 second-order
+  // sources, and [e; 0] at layer 0's input width
+  std::vector<void*> Tn, Ab;
+  std::vector<float*> src2;
+  void* E0 = nullptr;
+  // row stride of the transposed operands (the weight gradients' K): Bp, and
+  // 2 Bp on a CNF handle, whose second half holds the tangent chain
+  int ldT = 0;
   int64_t P = 0;
   std::vector<Layer> layers;
   hipStream_t stream = nullptr;
@@ -319,7 +331,8 @@ GemmPlan plan_role(node_handle* h, int role, int layer, int rows, int cols, int 
 // batches keep the fused epilogue, which saves the partials' round trip.
 bool stage_split(const node_handle* h, int role) {
   if (h->cnf && role == PR_FWD_STAGE) return false;
-  if (role == PR_BWD_LAST && h->sense != NODE_SENSE_INTERPOLATING) return true;  // backsolve stage
+  if (role == PR_BWD_LAST && (h->sense != NODE_SENSE_INTERPOLATING || h->cnf))
+    return true;  // backsolve / CNF adjoint stage: elementwise kernel only
   const int cols = role == PR_BWD_LAST ? h->Dp : h->layers[h->L - 1].outp;
   return (int64_t)(h->Bp / 32) * (cols / 64) < 2048;
 }
@@ -348,8 +361,9 @@ void launch_hidden(node_handle* h, int l, int gate, bool adjoint) {
   GemmPlan p = plan_role(h, role, l, h->Bp, ly.outp, 0);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   const int64_t ph = (int64_t)h->Bp * ly.outp;
-  EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->Bp, h->H[l], adjoint ? h->HT[l] : nullptr,
-              adjoint && !act_from_output(ly.act) ? h->S[l] : nullptr, ph, ph};
+  EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, h->ldT, h->H[l], adjoint ? h->HT[l] : nullptr,
+              adjoint && !act_from_output(ly.act) ? h->S[l] : nullptr, ph,
+              (int64_t)h->ldT * ly.outp};
   ProfScope ps(h, role, l);
   launch_gemm_hidden(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
 }
@@ -417,10 +431,59 @@ void launch_bwd_data(node_handle* h, int l, int gate, bool bias_sums = true) {
   Layer& lo = h->layers[l - 1];
   float* bsum = bias_sums ? lo.bpart : nullptr;
   if (bsum) lo.bpart_rt = p.rt;
-  EpiBwdData e{ly.in, ly.inp, h->Bp, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd, pd, bsum,
-               h->H[l - 1], pd, act_from_output(lo.act) ? lo.act : -1};
+  EpiBwdData e{ly.in, ly.inp, h->ldT, h->S[l - 1], h->Dl[l - 1], h->DlT[l - 1], pd,
+               (int64_t)h->ldT * ly.inp, bsum, h->H[l - 1], pd,
+               act_from_output(lo.act) ? lo.act : -1};
   ProfScope ps(h, PR_BWD_DATA, l);
+  if (h->cnf_adj) {  // plus the Hutchinson term's second-order source
+    const EpiBwdSrc es{e, h->src2[l - 1]};
+    launch_gemm_bwdsrc(h->prec, p.cfg, g, es, h->ctl, gate, h->stream);
+  } else {
+    launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  }
+}
+
+// pointer to the second (tangent) half of a CNF handle's transposed operand
+void* tan_half(const node_handle* h, void* T) { return (char*)T + (size_t)h->Bp * h->esz; }
+
+// CNF adjoint, tangent forward of hidden layer l: t_l = act'_l * (t_{l-1} W_l^T)
+// with t_{-1} = [e; 0], row-major and transposed (second half of H_l^T)
+void launch_tan_fwd(node_handle* h, int l, int gate) {
+  const Layer& ly = h->layers[l];
+  GemmArgs g{};
+  g.A = l == 0 ? h->E0 : h->Tn[l - 1];
+  g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
+  g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
+  GemmPlan p = plan_role(h, PR_ADJ_HIDDEN, l, h->Bp, ly.outp, 0);
+  g.row_tiles = p.rt; g.col_tiles = p.ct;
+  const int64_t ph = (int64_t)h->Bp * ly.outp;
+  EpiBwdData e{ly.out, ly.outp, h->ldT, h->S[l], h->Tn[l], tan_half(h, h->HT[l]), ph,
+               (int64_t)h->ldT * ly.outp, nullptr, h->H[l], ph,
+               act_from_output(ly.act) ? ly.act : -1};
+  ProfScope ps(h, PR_ADJ_HIDDEN, l);
   launch_gemm_bwddata(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+}
+
+// CNF adjoint, tangent backward through layer l >= 1 into layer l - 1:
+// abar_{l-1} = (abar_l W_l) act'_{l-1} (+ its second-order source)
+void launch_tan_bwd(node_handle* h, int l, int gate) {
+  const Layer& ly = h->layers[l];
+  const Layer& lo = h->layers[l - 1];
+  GemmArgs g{};
+  g.A = h->Ab[l];
+  g.lda = ly.outp; g.B = ly.Wc; g.ldb = ly.outp; g.K = ly.outp;
+  g.pa = (int64_t)h->Bp * ly.outp; g.pb = (int64_t)ly.inp * ly.outp; g.ksplit = 1;
+  GemmPlan p = plan_role(h, PR_BWD_DATA, l, h->Bp, ly.inp, 0);
+  g.row_tiles = p.rt; g.col_tiles = p.ct;
+  const int64_t pd = (int64_t)h->Bp * ly.inp;
+  EpiTanBwd e{};
+  e.C = ly.in; e.ldo = ly.inp; e.ldt = h->ldT; e.act = lo.act;
+  e.Y = h->H[l - 1]; e.S = h->S[l - 1]; e.Tf = h->Tn[l - 1]; e.py = e.ptf = pd;
+  e.Ab = h->Ab[l - 1]; e.AbT = tan_half(h, h->DlT[l - 1]);
+  e.pab = pd; e.pabt = (int64_t)h->ldT * ly.inp;
+  e.src = h->src2[l - 1];
+  ProfScope ps(h, PR_BWD_DATA, l);
+  launch_gemm_tanbwd(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
 }
 
 // weight gradient of layer l (K = batch), folded into the mu accumulators or
@@ -428,13 +491,14 @@ void launch_bwd_data(node_handle* h, int l, int gate, bool bias_sums = true) {
 void launch_wgrad(node_handle* h, int l, int gate, const EpiWgrad& wt) {
   const Layer& ly = h->layers[l];
   GemmArgs g{};
+  // (K = ldT: on a CNF handle the second half adds the tangent chain's t^T abar)
   g.A = l == 0 ? h->X0T : h->HT[l - 1];
-  g.lda = h->Bp;
+  g.lda = h->ldT;
   g.B = l == h->L - 1 ? h->LmT : h->DlT[l];
-  g.ldb = h->Bp;
-  g.K = h->Bp;
-  g.pa = (int64_t)ly.inp * h->Bp;
-  g.pb = (int64_t)ly.outp * h->Bp;
+  g.ldb = h->ldT;
+  g.K = h->ldT;
+  g.pa = (int64_t)ly.inp * h->ldT;
+  g.pb = (int64_t)ly.outp * h->ldT;
   int cfg = ly.wg_cfg, ks = ly.wg_ks;
   float* wpart = ly.wpart;
   if (h->tune_role == PR_WGRAD && h->tune_layer == l && h->tune_cfg >= 0) {
@@ -471,11 +535,16 @@ void launch_bwd_last(node_handle* h, int stage, int mode, int gate, double* part
   e.D = h->D; e.B = h->B; e.stage = stage; e.mode = mode;
   e.kl = h->kl; e.sv = (int64_t)h->Bp * h->D;
   e.lam = h->lam; e.lam_new = h->lam_new; e.Ucur = h->ucur;
-  e.L = h->Lm; e.LT = h->LmT; e.ldl = h->Dp; e.ldlt = h->Bp;
-  e.X0 = h->X0; e.X0T = h->X0T; e.ldx = h->Dp; e.ldxt = h->Bp; e.map = h->desc.input_map;
+  e.L = h->Lm; e.LT = h->LmT; e.ldl = h->Dp; e.ldlt = h->ldT;
+  e.X0 = h->X0; e.X0T = h->X0T; e.ldx = h->Dp; e.ldxt = h->ldT; e.map = h->desc.input_map;
   e.tape = h->tape; e.ent = 8 * (int64_t)h->Bp * h->D;
   e.part = part; e.plain_out = plain_out;
-  e.pl = e.plt = e.px = e.pxt = (int64_t)h->Bp * h->Dp;
+  e.pl = e.px = (int64_t)h->Bp * h->Dp;
+  e.plt = e.pxt = (int64_t)h->ldT * h->Dp;
+  if (h->cnf) {
+    const StateGeom sg = geom(h);
+    e.cnf = 1; e.dm = sg.dm; e.tcat = sg.tcat;
+  }
   if (h->sense != NODE_SENSE_INTERPOLATING) {
     e.fu = h->bs_fu; e.ku = h->bs_ku; e.ubs = h->bs_u; e.ubs_new = h->bs_unew;
   }
@@ -502,7 +571,7 @@ void launch_fold_all(node_handle* h, int gate, const EpiWgrad& wt) {
   for (int l = 0; l < h->L; ++l) {
     const Layer& ly = h->layers[l];
     fa.dT[l] = l == h->L - 1 ? h->LmT : h->DlT[l];
-    fa.dplane[l] = (int64_t)ly.outp * h->Bp;
+    fa.dplane[l] = (int64_t)ly.outp * h->ldT;
     fa.outs[l] = ly.out;
     fa.b0s[l] = ly.b0;
     fa.bias_rows += ly.out;
@@ -517,7 +586,7 @@ void launch_fold_all(node_handle* h, int gate, const EpiWgrad& wt) {
       fa.nsk++;
     }
   }
-  fa.ldt = h->Bp;
+  fa.ldt = h->ldT;
   fa.Bp = h->Bp;
   launch_fold(h->prec, fa, wt, h->ctl, gate, h->stream);
 }
@@ -528,6 +597,10 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
               float cb1, float ce1, double* part, float* plain_out, float* gptr) {
   const int L = h->L;
   for (int l = 0; l < L - 1; ++l) launch_hidden(h, l, gate, true);
+  if (h->cnf_adj) {  // tangent chain of the Hutchinson term, forward then reverse
+    for (int l = 0; l < L - 1; ++l) launch_tan_fwd(h, l, gate);
+    for (int l = L - 1; l >= 1; --l) launch_tan_bwd(h, l, gate);
+  }
   // BacksolveAdjoint: f(u_s) of this stage (the last layer on the recomputed
   // hidden state) for u's own Runge-Kutta stages
   if (h->sense != NODE_SENSE_INTERPOLATING && mode != SM_PLAIN)
@@ -708,7 +781,22 @@ int ensure_adjoint(node_handle* h) {
   if (int rc = dalloc_t(h, &h->lam_new, sv * 4)) return rc;
   if (int rc = dalloc_t(h, &h->kl, 8 * sv * 4)) return rc;
   if (int rc = dalloc(h, &h->Lm, act)) return rc;
-  if (int rc = dalloc(h, &h->LmT, act)) return rc;
+  if (int rc = dalloc(h, &h->LmT, act / h->Bp * h->ldT)) return rc;
+  if (h->cnf) {
+    const int L = h->L;
+    const size_t unit = (size_t)h->Bp * h->esz * h->planes;
+    if (int rc = dalloc(h, &h->E0, unit * h->layers[0].inp)) return rc;
+    h->Tn.assign(L - 1, nullptr);
+    h->Ab.assign(L, nullptr);
+    h->src2.assign(L - 1, nullptr);
+    for (int l = 0; l < L; ++l) {
+      const int op = h->layers[l].outp;
+      if (int rc = dalloc(h, &h->Ab[l], unit * op)) return rc;
+      if (l == L - 1) break;
+      if (int rc = dalloc(h, &h->Tn[l], unit * op)) return rc;
+      if (int rc = dalloc_t(h, &h->src2[l], (size_t)h->Bp * op * sizeof(float))) return rc;
+    }
+  }
   if (int rc = dalloc_t(h, &h->mu, 2 * h->P * 8)) return rc;
   if (int rc = dalloc_t(h, &h->accb, h->P * 4)) return rc;
   if (int rc = dalloc_t(h, &h->acce, h->P * 4)) return rc;
@@ -727,8 +815,9 @@ AdjBufs adj_bufs(node_handle* h) {
   AdjBufs a{};
   a.lam = h->lam; a.lam_new = h->lam_new; a.kl = h->kl; a.U = h->ucur;
   a.L = h->Lm; a.LT = h->LmT; a.X0 = h->X0; a.X0T = h->X0T;
-  a.ldl = h->Dp; a.ldlt = h->Bp; a.ldx = h->Dp; a.ldxt = h->Bp;
-  a.pl = a.plt = a.px = a.pxt = (int64_t)h->Bp * h->Dp;
+  a.ldl = h->Dp; a.ldlt = h->ldT; a.ldx = h->Dp; a.ldxt = h->ldT;
+  a.pl = a.px = (int64_t)h->Bp * h->Dp;
+  a.plt = a.pxt = (int64_t)h->ldT * h->Dp;
   a.tape = h->tape; a.fwd_times = h->fwd_times; a.cb_times = h->save_times;
   a.dLdout = h->dLdout_dev;
   a.mu = h->mu; a.accb = h->accb; a.acce = h->acce; a.gf0 = h->gf0; a.gf1 = h->gf1;
@@ -978,7 +1067,7 @@ int cnf_last_tiles(node_handle* h) {
 // ---------------------------------------------------------------- backward
 int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   if (!h->have_forward) return fail(h, NODE_ERR_STATE, "no forward solve to differentiate");
-  if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "the adjoint of a CNF solve is not on the HIP path");
+  if (h->cnf && !h->cnf_adj) return fail(h, NODE_ERR_UNSUPPORTED, "CNF handle: use node_cnf_backward");
   if (h->layers[h->L - 1].act != NODE_ACT_IDENTITY)
     return fail(h, NODE_ERR_UNSUPPORTED, "adjoint: the last Dense layer must be linear");
   if (h->fwd_stats.retcode != NODE_RET_SUCCESS)
@@ -1391,6 +1480,7 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
   h->Dp = (int)rup(h->D, KPAD);
   h->B = (int)desc->batch;
   h->Bp = (int)rup(h->B, BPAD);
+  h->ldT = h->cnf ? 2 * h->Bp : h->Bp;
   h->prec = desc->precision;
   h->wg_prec = h->prec;
   h->esz = h->prec == NODE_PREC_FP32 ? 4 : 2;
@@ -1433,15 +1523,16 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
   }
   const size_t x0b = (size_t)h->Bp * h->Dp * h->esz * h->planes;
   if (!rc) rc = dalloc(h, &h->X0, x0b);
-  if (!rc) rc = dalloc(h, &h->X0T, x0b);
+  if (!rc) rc = dalloc(h, &h->X0T, x0b / h->Bp * h->ldT);
   for (int l = 0; l < h->L - 1 && !rc; ++l) {
     const size_t hb = (size_t)h->Bp * h->layers[l].outp * h->esz * h->planes;
+    const size_t hbt = hb / h->Bp * h->ldT;
     void *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
     rc = dalloc(h, &a, hb);
-    if (!rc) rc = dalloc(h, &b, hb);
+    if (!rc) rc = dalloc(h, &b, hbt);
     if (!rc) rc = dalloc(h, &c, hb);
     if (!rc) rc = dalloc(h, &d, hb);
-    if (!rc) rc = dalloc(h, &e, hb);
+    if (!rc) rc = dalloc(h, &e, hbt);
     h->H.push_back(a); h->HT.push_back(b); h->S.push_back(c); h->Dl.push_back(d); h->DlT.push_back(e);
   }
   if (!rc) rc = dalloc_t(h, &h->ctl, sizeof(Ctl));
@@ -1596,6 +1687,7 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
   if (!h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "not a CNF handle (input_map NODE_MAP_CNF*)");
   if (!opts || !x || !probe) return fail(h, NODE_ERR_INVALID, "null argument");
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  h->cnf_fwd = false;
   if (int rc = ensure_adjoint(h)) return rc;  // Lm holds the probe for the data-gradient GEMMs
   if (int rc = maybe_autotune(h)) return rc;
   const int dm = h->D - 1, B = h->B;
@@ -1606,7 +1698,8 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
   sgm.D = dm;
   const Layer& ll = h->layers[h->L - 1];
   const int64_t pl = (int64_t)h->Bp * ll.outp;
-  launch_prep_lambda(sgm, h->probe, h->Lm, h->LmT, ll.outp, h->Bp, pl, pl, h->stream);
+  launch_prep_lambda(sgm, h->probe, h->Lm, h->LmT, ll.outp, h->ldT, pl, (int64_t)h->ldT * ll.outp,
+                     h->stream);
   // u0 = [x; 0] staged through the scratch state buffer
   std::vector<float> u0((size_t)B * h->D, 0.f);
   for (int b = 0; b < B; ++b)
@@ -1636,8 +1729,60 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
     if (delta_logp) delta_logp[b] = dl;
     if (logpx) logpx[b] = (float)(-0.5 * ss - lc - (double)dl);
   }
-  h->have_forward = false;  // no adjoint of this solve
+  h->probe_h.assign(probe, probe + (size_t)B * dm);
+  h->cnf_fwd = st.retcode == NODE_RET_SUCCESS;
   if (stats) *stats = st;
+  return 0;
+}
+
+int32_t node_cnf_backward(node_handle_t* h, const float* dL_dz, const float* dL_ddlogp,
+                          float* dL_dx, double* dL_dp, node_stats* stats) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "not a CNF handle (input_map NODE_MAP_CNF*)");
+  if (!dL_dz || !dL_ddlogp) return fail(h, NODE_ERR_INVALID, "null argument");
+  if (!h->cnf_fwd || !h->have_forward)
+    return fail(h, NODE_ERR_STATE, "no successful CNF forward solve to differentiate");
+  const int dm = h->D - 1, B = h->B, L = h->L;
+  const int64_t n = (int64_t)B * h->D;
+  if (n > h->dLdout_cap) {
+    dfree(h, h->dLdout_dev);
+    h->dLdout_dev = nullptr;
+    if (int rc = dalloc_t(h, &h->dLdout_dev, n * sizeof(float))) return rc;
+    h->dLdout_cap = n;
+  }
+  // cotangent of the saved end state [z; Δlogp], and the tangent chain's seed
+  // abar_{L-1} = -λ_Δ e (λ_Δ is constant over the adjoint solve: dλ_Δ/dt = 0)
+  std::vector<float> cot((size_t)n), seed((size_t)B * dm);
+  for (int b = 0; b < B; ++b) {
+    for (int c = 0; c < dm; ++c) {
+      cot[(size_t)b * h->D + c] = dL_dz[(size_t)b * dm + c];
+      seed[(size_t)b * dm + c] = -dL_ddlogp[b] * h->probe_h[(size_t)b * dm + c];
+    }
+    cot[(size_t)b * h->D + dm] = dL_ddlogp[b];
+  }
+  HIPCHK(h, hipMemcpyAsync(h->dLdout_dev, cot.data(), cot.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->fbuf, seed.data(), seed.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  StateGeom sgm = geom(h);
+  sgm.D = dm;
+  const Layer& l0 = h->layers[0];
+  const Layer& ll = h->layers[L - 1];
+  // t_{-1} = [e; 0]: row-major (layer 0's tangent input) and transposed
+  // (second half of X0^T, layer 0's weight gradient)
+  launch_prep_lambda(sgm, h->probe, h->E0, tan_half(h, h->X0T), l0.inp, h->ldT,
+                     (int64_t)h->Bp * l0.inp, (int64_t)h->ldT * h->Dp, h->stream);
+  launch_prep_lambda(sgm, h->fbuf, h->Ab[L - 1], tan_half(h, h->LmT), ll.outp, h->ldT,
+                     (int64_t)h->Bp * ll.outp, (int64_t)h->ldT * h->Dp, h->stream);
+  HIPCHK(h, hipGetLastError());
+  h->cnf_adj = true;
+  int rc = backward_impl(h, h->dLdout_dev, stats);
+  h->cnf_adj = false;
+  if (rc) return rc;
+  std::vector<float> lam0((size_t)n);
+  if (int rc2 = copy_grads(h, lam0.data(), dL_dp, hipMemcpyDeviceToHost)) return rc2;
+  if (dL_dx)
+    for (int b = 0; b < B; ++b)
+      for (int c = 0; c < dm; ++c) dL_dx[(size_t)b * dm + c] = lam0[(size_t)b * h->D + c];
   return 0;
 }
 
@@ -1711,12 +1856,13 @@ int32_t node_ragged_backward(node_handle_t* h, const float* dLdout, float* dLdu0
 int32_t node_rhs(node_handle_t* h, const float* u, float* du) {
   if (!h || !u || !du) return NODE_ERR_INVALID;
   std::lock_guard<std::mutex> g(h->mtx);
+  if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "CNF handle: use node_cnf_forward");
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
   if (int rc = maybe_autotune(h)) return rc;
   const StateGeom sg = geom(h);
   HIPCHK(h, hipMemcpyAsync(h->ucur, u, (size_t)h->B * h->D * sizeof(float), hipMemcpyHostToDevice, h->stream));
   const int64_t px = (int64_t)h->Bp * h->Dp;
-  launch_prep_plain(sg, h->ucur, h->X0, nullptr, nullptr, h->Dp, h->Bp, px, px, h->stream);
+  launch_prep_plain(sg, h->ucur, h->X0, nullptr, nullptr, h->Dp, h->ldT, px, px, h->stream);
   rhs_eval(h, 0, SM_PLAIN, GATE_ALWAYS, nullptr);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(du, h->scratch, (size_t)h->B * h->D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
@@ -1727,6 +1873,7 @@ int32_t node_rhs(node_handle_t* h, const float* u, float* du) {
 int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda, float* du, double* dp) {
   if (!h || !u || !lambda || !du || !dp) return NODE_ERR_INVALID;
   std::lock_guard<std::mutex> g(h->mtx);
+  if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "CNF handle: use node_cnf_backward");
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
   if (int rc = ensure_adjoint(h)) return rc;
   if (int rc = maybe_autotune(h)) return rc;
@@ -1735,8 +1882,9 @@ int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda, float* d
   HIPCHK(h, hipMemcpyAsync(h->ucur, u, sb, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->lam, lambda, sb, hipMemcpyHostToDevice, h->stream));
   const int64_t px = (int64_t)h->Bp * h->Dp;
-  launch_prep_plain(sg, h->ucur, h->X0, h->X0T, h->ucur, h->Dp, h->Bp, px, px, h->stream);
-  launch_prep_lambda(sg, h->lam, h->Lm, h->LmT, h->Dp, h->Bp, px, px, h->stream);
+  const int64_t pxt = (int64_t)h->ldT * h->Dp;
+  launch_prep_plain(sg, h->ucur, h->X0, h->X0T, h->ucur, h->Dp, h->ldT, px, pxt, h->stream);
+  launch_prep_lambda(sg, h->lam, h->Lm, h->LmT, h->Dp, h->ldT, px, pxt, h->stream);
   if (!h->dp32 && dalloc_t(h, &h->dp32, h->P * sizeof(float))) return NODE_ERR_HIP;
   vjp_eval(h, 0, SM_PLAIN, GATE_ALWAYS, WG_STORE_PTR, 0, 0, 0, 0, nullptr, h->scratch, h->dp32);
   HIPCHK(h, hipGetLastError());
@@ -1779,7 +1927,8 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
   } else if (key == NODE_OPT_SENSEALG) {
     if (value < NODE_SENSE_INTERPOLATING || value > NODE_SENSE_BACKSOLVE_NOCKPT)
       return fail(h, NODE_ERR_INVALID, "unknown sensealg (NODE_SENSE_*)");
-    if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "sensealg: a CNF handle has no adjoint");
+    if (h->cnf && value != NODE_SENSE_INTERPOLATING)
+      return fail(h, NODE_ERR_UNSUPPORTED, "sensealg: a CNF handle supports InterpolatingAdjoint only");
     if (value == h->sense) return 0;  // unchanged: the captured graphs stay valid
     h->sense = (int)value;
   } else if (key == NODE_OPT_GRAPH_ATTEMPTS) {

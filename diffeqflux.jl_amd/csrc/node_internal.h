@@ -102,6 +102,26 @@ struct EpiBwdData {
   const void* Y; int64_t py; int yact;
 };
 
+// FFJORD adjoint, primal data gradient with the Hutchinson term's
+// second-order source: delta = acc * act'(z) + src (src FP32 [rows][ldo])
+struct EpiBwdSrc {
+  EpiBwdData d;
+  const float* src;
+};
+
+// FFJORD adjoint, tangent-backward chain into hidden layer l (rows = batch):
+// tb = acc, ab = tb * act'_l written row-major + transposed (the transposed
+// copy is the second half of the K = 2B weight-gradient operand), and
+// src = tb * (adot_l act''_l) in FP32 from the stored tangent t_l = act'_l adot_l
+// and output y_l (tanh: -2 y t; softplus: t (1 - act'); sigmoid: t (1 - 2 y))
+struct EpiTanBwd {
+  int C; int ldo; int ldt; int act;
+  const void* Y; const void* S; const void* Tf;  // layer output, act' (S type), tangent
+  int64_t py, ptf;                               // plane strides of Y / Tf
+  void* Ab; void* AbT; int64_t pab, pabt;
+  float* src;
+};
+
 // layer-1 data gradient: k^lambda_s = -(acc * map'(u)), next stage inputs
 struct EpiBwdLast {
   int D; int B; int stage; int mode;
@@ -116,6 +136,10 @@ struct EpiBwdLast {
   // BacksolveAdjoint (ubs != nullptr; elementwise stage kernel only): fu =
   // f(u_s) of this stage, ku its stage slots, ubs / ubs_new as in AdjBufs
   const float* fu; float* ku; float* ubs; float* ubs_new;
+  // FFJORD adjoint (cnf; elementwise stage kernel only): state [z; Δlogp]
+  // (D = dm + 1), model input [z; t] when tcat; k of the Δlogp column is 0
+  // (its lambda is constant) and the lambda input of the model is z's part
+  int cnf; int dm; int tcat;
 };
 
 // split-K partial products of a skinny stage GEMM (FP32 [ksplit][rows][ld])
@@ -173,6 +197,10 @@ void launch_gemm_wgrad(int prec, int cfg, const GemmArgs& g, const EpiWgrad& e,
                        const Ctl* ctl, int gate, hipStream_t s);
 void launch_gemm_partial(int prec, int cfg, const GemmArgs& g, const EpiPartial& e,
                          const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_tanbwd(int prec, int cfg, const GemmArgs& g, const EpiTanBwd& e,
+                        const Ctl* ctl, int gate, hipStream_t s);
+void launch_gemm_bwdsrc(int prec, int cfg, const GemmArgs& g, const EpiBwdSrc& e,
+                        const Ctl* ctl, int gate, hipStream_t s);
 // elementwise stage kernels over summed split-K partials (rows x cols, cols a
 // multiple of 64, rows a multiple of 16); one partial-sum slot per block
 int stage_grid(int rows, int cols);

@@ -73,6 +73,8 @@ SIGNATURES = [
     ("node_cnf_forward", c_i32, [ctypes.c_void_p, ctypes.POINTER(SolveOpts), ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    ("node_cnf_backward", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("node_saved_times", c_i32, [ctypes.c_void_p, ctypes.c_void_p, c_i64]),
     ("node_copy_saved", c_i32, [ctypes.c_void_p, ctypes.c_void_p, c_i64]),
     ("node_backward", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

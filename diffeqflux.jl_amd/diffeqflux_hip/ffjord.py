@@ -14,6 +14,12 @@ whole solve; the reference redraws ``randn!`` on every right-hand-side call
 (``:96``).  A model whose input width is ``D + 1`` gets ``[z; t]``
 (time-concatenated network); a ``D``-wide input gets ``z`` as in the
 reference (``:88``).
+
+Differentiation: with ``x`` or ``ps`` a ``torch.Tensor`` the solve is a
+``torch.autograd.Function`` whose backward is the HIP InterpolatingAdjoint of
+the augmented solve (the reference's ``sensealg``, ``:143``) with the same
+probe, including the second-order VJP of the Hutchinson term; ``logpx`` is
+then assembled from ``z(T)`` and ``Δlogp`` in torch.
 """
 import numpy as np
 
@@ -66,16 +72,39 @@ class FFJORD:
             raise NotImplementedError("regularize = true is not on the HIP path")
         if not st.get("monte_carlo", True):
             raise NotImplementedError("the exact-trace (monte_carlo = false) form is not on the HIP path")
+        x_in = x
+        x = x.detach().cpu().numpy() if hasattr(x, "detach") else x
         x = np.asarray(x, np.float32)
         if x.ndim == 1:
             x = x[None, :]
+            if hasattr(x_in, "detach"):
+                x_in = x_in.reshape(1, -1)
         B = x.shape[0]
         probe = st.get("probe")
         if probe is None:
             rng = st.get("rng") or np.random.default_rng()
             probe = rng.choice(np.array([-1.0, 1.0], np.float32), size=(B, self.D))
         h = self._handle(B)
+        try:
+            import torch
+            diff = isinstance(x_in, torch.Tensor) or isinstance(ps, torch.Tensor)
+        except ImportError:
+            diff = False
+        if diff:
+            z, dl, stats = _FFJORDFn.apply(x_in, ps, self, h, probe)
+            D = self.D
+            logpz = -0.5 * (z * z).sum(dim=1) - 0.5 * D * float(np.log(2 * np.pi))
+            lp = (logpz - dl).reshape(1, B)
+            dl = dl.reshape(1, B)
+            st["stats"] = stats
+            return (lp, dl, dl), st
         h.set_params(np.asarray(ps, np.float32))
+        lp, z, dl, stats = self._solve(h, x, probe)
+        st["stats"] = stats
+        dl = dl.reshape(1, B)
+        return (lp.reshape(1, B), dl, dl), st
+
+    def _solve(self, h, x, probe):
         kw = self.kwargs
         lp, z, dl, stats = h.cnf_forward(x, probe, self.tspan[0], self.tspan[1],
                                          abstol=kw.get("abstol", 1e-6), reltol=kw.get("reltol", 1e-3),
@@ -84,6 +113,54 @@ class FFJORD:
                                          maxiters=kw.get("maxiters", 0))
         if stats["retcode"] != 1:
             raise RuntimeError(f"FFJORD solve failed: {_lib.RET_NAMES.get(stats['retcode'])}")
-        st["stats"] = stats
-        dl = dl.reshape(1, B)
-        return (lp.reshape(1, B), dl, dl), st
+        return lp, z, dl, stats
+
+
+def _FFJORDFn_factory():
+    import torch
+
+    class _Fn(torch.autograd.Function):
+        """rrule of the FFJORD solve: forward = HIP Tsit5 solve of [z; Δlogp]
+        (tape kept in the handle), backward = HIP InterpolatingAdjoint with
+        the second-order VJP of the Hutchinson term."""
+
+        @staticmethod
+        def forward(ctx, x, p, ffj, h, probe):
+            xt = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, np.float32))
+            pt = p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, np.float32))
+            h.set_params(pt.detach().cpu().numpy().astype(np.float32))
+            xn = xt.detach().cpu().numpy().astype(np.float32)
+            _, z, dl, stats = ffj._solve(h, xn, probe)
+            ctx.h, ctx.gen = h, h.fwd_gen
+            ctx.xshape, ctx.xdtype, ctx.xdev = xt.shape, xt.dtype, xt.device
+            ctx.pdtype, ctx.pdev = pt.dtype, pt.device
+            ctx.mark_non_differentiable  # noqa: B018
+            return (torch.from_numpy(z).to(xt.device), torch.from_numpy(dl).to(xt.device), stats)
+
+        @staticmethod
+        def backward(ctx, gz, gdl, _gs=None):
+            h = ctx.h
+            if h.fwd_gen != ctx.gen:
+                raise RuntimeError("FFJORD backward: the layer's handle ran another solve since "
+                                   "this one (the handle keeps one tape); call backward before "
+                                   "the next forward of the same layer and batch")
+            gz = torch.zeros(ctx.xshape) if gz is None else gz
+            gdl = torch.zeros(ctx.xshape[0]) if gdl is None else gdl
+            dx, dp, _ = h.cnf_backward(gz.detach().cpu().numpy(), gdl.detach().cpu().numpy())
+            dx = torch.from_numpy(dx).to(ctx.xdev, ctx.xdtype).reshape(ctx.xshape)
+            dp = torch.from_numpy(dp).to(ctx.pdev, ctx.pdtype)
+            return dx, dp, None, None, None
+
+    return _Fn
+
+
+class _LazyFn:
+    _fn = None
+
+    def apply(self, *a):
+        if self._fn is None:
+            self._fn = _FFJORDFn_factory()
+        return self._fn.apply(*a)
+
+
+_FFJORDFn = _LazyFn()

@@ -170,10 +170,25 @@ class Handle:
         o = self.make_opts(t0, t1, abstol=abstol, reltol=reltol, adaptive=adaptive, dt=dt,
                            dtmax=dtmax, dtmin=dtmin, maxiters=maxiters)
         st = _lib.Stats()
+        self.fwd_gen += 1
         self._check(self.lib.node_cnf_forward(self.h, ctypes.byref(o), _ptr(x), _ptr(e), _ptr(z),
                                               _ptr(dl), _ptr(lp), ctypes.byref(st)),
                     "node_cnf_forward")
         return lp, z, dl, st.as_dict()
+
+    def cnf_backward(self, dL_dz, dL_ddlogp):
+        """InterpolatingAdjoint of the last ``cnf_forward`` (same probe).
+        dL_dz: (B, D), dL_ddlogp: (B,).  Returns (dL/dx (B, D) fp32,
+        dL/dp (P,) fp64, stats dict)."""
+        D = self.widths[-1]
+        gz = _f32(dL_dz).reshape(self.B, D)
+        gd = _f32(dL_ddlogp).reshape(self.B)
+        dx = np.empty((self.B, D), np.float32)
+        dp = np.empty((self.P,), np.float64)
+        st = _lib.Stats()
+        self._check(self.lib.node_cnf_backward(self.h, _ptr(gz), _ptr(gd), _ptr(dx), _ptr(dp),
+                                               ctypes.byref(st)), "node_cnf_backward")
+        return dx, dp, st.as_dict()
 
     def ragged_forward(self, u0, t1, save_times, t0=0.0, abstol=1e-6, reltol=1e-3, adaptive=True,
                        dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=0):

@@ -150,6 +150,22 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
                          const float* probe, float* z, float* delta_logp, float* logpx,
                          node_stats* stats);
 
+/* FFJORD backward: the InterpolatingAdjoint (the layer's default sensealg,
+ * src/ffjord.jl:143, differentiated through the Zygote pullback of the
+ * Hutchinson term, :95-115) of the last node_cnf_forward solve on this
+ * handle, with the same probe.  Cotangents dL_dz (D x B) and dL_ddlogp (B)
+ * of z(T) and delta_logp; for a loss on logpx, dL/dz = -dL/dlogpx * z and
+ * dL/ddelta_logp = -dL/dlogpx.  The adjoint state [lambda_z; lambda_logp; mu]
+ * is solved back from T with Tsit5 at the forward's tolerances (error norm
+ * over all (D + 1) x B + P entries), the model's VJP including the
+ * second-order VJP of e^T (df/dz) e (tangent chain t_l = act'(a_l) (t_{l-1}
+ * W_l), its reverse, and the source abar_t . adot . act''(a) on the primal
+ * chain); the weight gradients run as one product of K = 2B over the primal
+ * and tangent halves.  Writes dL_dx (D x B) and dL_dp (P, FP64); either may
+ * be NULL.  Host pointers. */
+int32_t node_cnf_backward(node_handle_t* h, const float* dL_dz, const float* dL_ddlogp,
+                          float* dL_dx, double* dL_dp, node_stats* stats);
+
 /* Times of the states saved by the last forward (n = stats->nsaved). */
 int32_t node_saved_times(node_handle_t* h, double* t_out, int64_t cap);
 /* Copy the saved states of the last forward (D x B x n) to host memory. */

@@ -1,0 +1,131 @@
+"""GPU parity of the FFJORD adjoint (node_cnf_backward): the HIP
+InterpolatingAdjoint of the augmented solve, with the second-order VJP of the
+Hutchinson term, against the oracle's restatement (oracle.ffjord.backward,
+FP64; pinned there by torch double-backward and finite differences) on the
+same FP32 parameters and the same fixed probe.
+
+Tolerances: fixed dt (identical step sequences) at the precision's gradient
+tolerance of the NeuralODE adjoint tests (FP32 1e-4, split BF16 2e-4,
+relative, max norm); adaptive solves within 50 * tol of the FP64 oracle plus
+that tolerance.  Hidden activations cover every closed form of the source
+term: tanh, softplus, sigmoid, relu (act'' = 0)."""
+import numpy as np
+import pytest
+
+from oracle import ffjord as FF
+from oracle.mlp import MLP
+
+from test_gpu_parity import TOL, rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: widths, acts, B
+    "time_tanh": ([3, 16, 16, 2], ["tanh", "tanh", "identity"], 64),
+    "z_only": ([2, 24, 2], ["tanh", "identity"], 37),
+    "acts": ([4, 32, 32, 32, 3], ["softplus", "sigmoid", "relu", "identity"], 50),
+    "smooth_acts": ([4, 32, 32, 3], ["softplus", "sigmoid", "identity"], 50),
+    "workload_shape": ([17, 512, 512, 16], ["tanh", "tanh", "identity"], 128),
+}
+
+
+def setup(name, prec, seed=0):
+    from diffeqflux_hip import Handle, _lib
+    widths, acts, B = CASES[name]
+    D = widths[-1]
+    m = MLP(widths, acts)
+    rng = np.random.default_rng(seed)
+    p32 = m.glorot_init(rng).astype(np.float32)
+    for li in range(len(acts) - 1):   # nonzero hidden biases
+        w0, b0, e = m.offsets[li]
+        p32[b0:e] = rng.uniform(-0.3, 0.3, e - b0)
+    x = rng.standard_normal((B, D)).astype(np.float32)
+    e = rng.choice(np.array([-1.0, 1.0], np.float32), size=(B, D))
+    tc = widths[0] == D + 1
+    h = Handle(widths, acts, _lib.MAP_CNF_TIME if tc else _lib.MAP_CNF, prec, B)
+    h.set_params(p32)
+    print(f"\n[ffjord adjoint {name} prec={prec} B={B} widths={widths} acts={acts}]", flush=True)
+    return m, p32, x, e, tc, h
+
+
+def cotangents(z, B, kind, seed=3):
+    if kind == "nll":                       # loss = -mean(logpx)
+        gz, gd = FF.nll_cotangents(z.astype(np.float64), B)
+        return gz.astype(np.float32), gd.astype(np.float32)
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(z.shape).astype(np.float32),
+            rng.standard_normal(B).astype(np.float32))
+
+
+def run_pair(name, prec, kw, kind="random"):
+    m, p32, x, e, tc, h = setup(name, prec)
+    B = x.shape[0]
+    lp, z, dl, st = h.cnf_forward(x, e, 0.0, 1.0, **kw)
+    assert st["retcode"] == 1
+    gz, gd = cotangents(z, B, kind)
+    dx, dp, sb = h.cnf_backward(gz, gd)
+    assert sb["retcode"] == 1
+    p64 = p32.astype(np.float64)
+    _, rz, _, sol = FF.forward(m, p64, x.astype(np.float64), 0.0, 1.0, probe=e.astype(np.float64),
+                               time_concat=tc, **kw)
+    rdx, rdp, adj = FF.backward(m, p64, sol, e.astype(np.float64), gz.astype(np.float64),
+                                gd.astype(np.float64), time_concat=tc, **kw)
+    ex, ep = rel(dx, rdx), rel(dp, rdp)
+    print(f"  {kind} cotangents, {kw}: rel err dL/dx {ex:.3e} dL/dp {ep:.3e}; adjoint steps gpu "
+          f"{sb['naccept']}+{sb['nreject']} oracle {adj.naccept}+{adj.nreject}", flush=True)
+    return ex, ep, sb, adj
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_cnf_adjoint_fixed_dt_matches_oracle(name, prec):
+    if prec == "bf16x3" and name not in ("workload_shape", "acts"):
+        pytest.skip("split BF16 checked on the workload shape and the activation mix")
+    ex, ep, sb, adj = run_pair(name, prec, dict(adaptive=False, dt=1.0 / 16))
+    assert sb["naccept"] == adj.naccept
+    assert ex < TOL[prec]["grad"] and ep < TOL[prec]["grad"]
+
+
+@pytest.mark.parametrize("name", ["time_tanh", "smooth_acts", "workload_shape"])
+def test_cnf_adjoint_adaptive_matches_oracle(name):
+    # (smooth models only: with relu the adjoint's right-hand side jumps where
+    # a unit switches, and the FP64 oracle's own gradient at tol 1e-6 is 1.3e-3
+    # away from its tol 1e-10 gradient on the `acts` case; fixed dt covers relu)
+    tol = 1e-6
+    ex, ep, sb, adj = run_pair(name, "fp32", dict(abstol=tol, reltol=tol), kind="nll")
+    assert ex < 50 * tol + TOL["fp32"]["grad"] and ep < 50 * tol + TOL["fp32"]["grad"]
+
+
+def test_ffjord_layer_torch_gradient():
+    """FFJORD layer (src/ffjord.jl:138-178) under torch autograd: the gradient
+    of -mean(logpx) equals the oracle's adjoint with the NLL cotangents."""
+    import torch
+    from diffeqflux_hip import FFJORD, Chain, Dense, Tsit5
+    m, p32, x, e, tc, _ = setup("time_tanh", "fp32")
+    B = x.shape[0]
+    ff = FFJORD(Chain(Dense(3, 16, "tanh"), Dense(16, 16, "tanh"), Dense(16, 2)), (0.0, 1.0), (2,),
+                Tsit5(), abstol=1e-6, reltol=1e-6)
+    pt = torch.tensor(p32, requires_grad=True)
+    xt = torch.tensor(x, requires_grad=True)
+    (lp, _, _), st = ff(xt, pt, {"regularize": False, "monte_carlo": True, "probe": e})
+    loss = -lp.mean()
+    loss.backward()
+    p64 = p32.astype(np.float64)
+    rlp, rz, _, sol = FF.forward(m, p64, x.astype(np.float64), 0.0, 1.0, probe=e.astype(np.float64),
+                                 time_concat=tc, abstol=1e-6, reltol=1e-6)
+    gz, gd = FF.nll_cotangents(rz, B)
+    rdx, rdp, _ = FF.backward(m, p64, sol, e.astype(np.float64), gz, gd, time_concat=tc,
+                              abstol=1e-6, reltol=1e-6)
+    ex, ep = rel(xt.grad.numpy(), rdx), rel(pt.grad.numpy(), rdp)
+    print(f"  torch FFJORD: loss {loss.item():.6f} (oracle {-rlp.mean():.6f}); rel err dx {ex:.2e} "
+          f"dp {ep:.2e}", flush=True)
+    assert abs(loss.item() + rlp.mean()) < 1e-4 * abs(rlp.mean())
+    assert ex < 5e-5 + 1e-4 and ep < 5e-5 + 1e-4
+
+
+def test_cnf_backward_without_forward_raises():
+    from diffeqflux_hip import Handle, _lib
+    h = Handle([2, 8, 2], ["tanh", "identity"], _lib.MAP_CNF, "fp32", 4)
+    h.set_params(np.zeros(h.P, np.float32))
+    with pytest.raises(RuntimeError):
+        h.cnf_backward(np.zeros((4, 2), np.float32), np.zeros(4, np.float32))
