@@ -366,11 +366,13 @@ def gaussian_ring(n, D=16, seed=6):
     return x
 
 
-def ffjord_leg(B=8192, reps=3, precision="bf16x3", cpu_batch=64):
+def ffjord_leg(B=8192, reps=3, precision="bf16x3", cpu_batch=8192):
     """FFJORD log-likelihood forward (D=16, time-concatenated MLP 17-512-512-16
     tanh, Hutchinson trace with one Rademacher probe per sample, tspan (0,1),
-    Tsit5 rtol 1e-5 atol 1e-5) on the Gaussian-ring data; the oracle (FP32) on
-    a slice of the batch as the CPU baseline."""
+    Tsit5 rtol 1e-5 atol 1e-5) on the Gaussian-ring data, and the training
+    step (forward + InterpolatingAdjoint of -mean(logpx) with the second-order
+    VJP of the Hutchinson term); the oracle (FP32) on a slice of the batch as
+    the CPU baselines."""
     from diffeqflux_hip import Handle, _lib
     from oracle.mlp import MLP
     widths, acts = [17, 512, 512, 16], ["tanh", "tanh", "identity"]
@@ -388,6 +390,16 @@ def ffjord_leg(B=8192, reps=3, precision="bf16x3", cpu_batch=64):
         if best is None or dt < best[0]:
             best = (dt, st, lp)
     dt, st, lp = best
+    # training step: forward + InterpolatingAdjoint of -mean(logpx) (second-order
+    # VJP of the Hutchinson term), same probe
+    tbest = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, z, _, st2 = h.cnf_forward(x, e, 0.0, 1.0, abstol=1e-5, reltol=1e-5)
+        dx, dp, sb = h.cnf_backward(z / B, np.full(B, 1.0 / B, np.float32))
+        tt = time.perf_counter() - t0
+        if tbest is None or tt < tbest[0]:
+            tbest = (tt, st2, sb)
     fl = 2.0 * B * sum(i * o for i, o in zip(widths[:-1], widths[1:])) * 2  # f + probe VJP
     res = {"workload": "FFJORD forward, B=8192, D=16, MLP 17-512-512-16 tanh (time-concat), "
                        "Hutchinson 1 Rademacher probe, tspan (0,1), rtol 1e-5 atol 1e-5",
@@ -396,20 +408,45 @@ def ffjord_leg(B=8192, reps=3, precision="bf16x3", cpu_batch=64):
            "rhs_evals_per_s": round(st["nf"] / dt, 1), "samples_per_s": round(B / dt, 1),
            "tflops_issued": round(st["nf"] * fl * (3 if precision == "bf16x3" else 1) / dt / 1e12, 1),
            "mean_logpx": float(np.mean(lp)),
-           "timing": "host entry point incl. PCIe copies of x / probe / results"}
+           "train_step_ms": round(1e3 * tbest[0], 2),
+           "train_evals": f"fwd {tbest[1]['nf']} adj {tbest[2]['nf']} "
+                          f"(adjoint steps {tbest[2]['naccept']}+{tbest[2]['nreject']})",
+           "train_samples_per_s": round(B / tbest[0], 1),
+           "timing": "host entry points incl. PCIe copies of x / probe / cotangents / results"}
     from oracle import ffjord as FF
     tc = time.perf_counter()
     rlp, _, _, sol = FF.forward(m, p, x[:cpu_batch], 0.0, 1.0, probe=e[:cpu_batch],
                                 time_concat=True, abstol=1e-5, reltol=1e-5)
     cs = time.perf_counter() - tc
-    res["cpu_baseline"] = {"value": round(cpu_batch / cs, 2), "unit": "samples/s", "cores": 1,
+    res["cpu_baseline"] = {"value": round(cpu_batch / cs, 2), "unit": "samples/s",
+                           "cores": blas_cores(),
                            "kind": "port",
                            "sample": f"oracle FP32 FFJORD forward on {cpu_batch} of the samples "
                                      f"(same probe), {sol.nf} evals in {cs:.1f} s"}
+    tc = time.perf_counter()
+    _, rz, _, sol = FF.forward(m, p, x[:cpu_batch], 0.0, 1.0, probe=e[:cpu_batch],
+                               time_concat=True, abstol=1e-5, reltol=1e-5)
+    gz, gd = FF.nll_cotangents(rz, cpu_batch)
+    _, _, adj = FF.backward(m, p, sol, e[:cpu_batch], gz.astype(np.float32),
+                            gd.astype(np.float32), time_concat=True, abstol=1e-5, reltol=1e-5)
+    cs = time.perf_counter() - tc
+    res["cpu_baseline_train"] = {"value": round(cpu_batch / cs, 2), "unit": "samples/s",
+                                 "cores": blas_cores(), "kind": "port",
+                                 "sample": f"oracle FP32 FFJORD forward + adjoint on {cpu_batch} of "
+                                           f"the samples, {sol.nf}+{adj.nf} evals in {cs:.1f} s"}
     return res
 
 
 # ---------------------------------------------------------------- CPU leg
+def blas_cores():
+    """Threads the oracle's numpy BLAS runs on (the CPU baselines' cores)."""
+    try:
+        from threadpoolctl import threadpool_info
+        return max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:
+        return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline_leg(p, Bs, gpu_grads):
     """The oracle (numpy FP32 restatement of the reference algorithm) timed on
     a bounded sample: one forward + adjoint solve at Bs trajectories (same D,
@@ -417,11 +454,7 @@ def cpu_baseline_leg(p, Bs, gpu_grads):
     sample, against which the GPU gradients of that sample are checked."""
     from oracle import adjoint as OA
     from oracle.mlp import MLP
-    try:
-        from threadpoolctl import threadpool_info
-        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:
-        cores = len(os.sched_getaffinity(0))
+    cores = blas_cores()
     m = MLP(WIDTHS, ACTS)
     u0, ts, tgt = make_inputs(Bs)
 
