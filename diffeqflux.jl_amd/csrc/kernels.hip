@@ -2201,7 +2201,11 @@ __global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_b
   float* gcur = c->gsel ? e.gf1 : e.gf0;
   float* gnext = c->gsel ? e.gf0 : e.gf1;
   if ((int)blockIdx.x < nb_bias) {
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    // the row (and with it the layer) is wave-uniform: readfirstlane keeps the
+    // per-layer argument lookups on scalar loads instead of per-lane loads of
+    // the kernel-argument segment, one dependent round trip each
+    const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int lane = threadIdx.x & 63;
     if (wave >= fa.bias_rows) return;
     int l = 0, row = wave;
     while (l < fa.nl - 1 && row >= fa.outs[l]) { row -= fa.outs[l]; ++l; }
@@ -2210,47 +2214,92 @@ __global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_b
       const float* bp = fa.bpart[l] + row;
       for (int t = lane; t < fa.bpart_rt[l]; t += 64) s += bp[(int64_t)t * fa.bpart_ld[l]];
     } else {
+      // eight independent partial sums keep eight loads per lane in flight
+      // (one dependent chain of Bp / 64 loads is latency bound); fixed order
       const T* src = (const T*)fa.dT[l] + (int64_t)row * fa.ldt;
-      for (int b = lane; b < fa.Bp; b += 64) s += get<PP>(src + b, fa.dplane[l]);
+      float q[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      int b = lane;
+      for (; b + 7 * 64 < fa.Bp; b += 8 * 64) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = get<PP>(src + b + j * 64, fa.dplane[l]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[j] += v[j];
+      }
+      for (int j = 0; b < fa.Bp; b += 64, ++j) q[j] += get<PP>(src + b, fa.dplane[l]);
+      s = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) fold_one(e, gcur, gnext, fa.b0s[l] + row, -s);
     return;
   }
-  // split-K partial sums: four consecutive entries per thread (16-byte
-  // accesses where the layer's flat offset and size allow them)
-  int64_t i = 4 * ((int64_t)(blockIdx.x - nb_bias) * blockDim.x + threadIdx.x);
+  // split-K partial sums (at most 16 splits): a block takes 64 quads of four
+  // consecutive entries of one layer; its four thread groups sum the splits
+  // g, g + 4, g + 8, g + 12 of each quad (one round of independent loads) and
+  // the groups are added in group order through LDS (deterministic)
+  __shared__ float4 red[4][64];
+  int bq = (int)blockIdx.x - nb_bias;
+  const int ql = threadIdx.x & 63, grp = threadIdx.x >> 6;
   for (int k = 0; k < fa.nsk; ++k) {
-    const int64_t n = fa.sk_n[k], nq = (n + 3) & ~(int64_t)3;
-    if (i < nq) {
+    const int64_t n = fa.sk_n[k];
+    const int nbk = (int)(((n + 3) / 4 + 63) / 64);
+    if (bq < nbk) {
       const float* part = fa.sk_part[k];
       const int64_t w0 = fa.sk_w0[k];
-      if (((w0 | n) & 3) == 0) {
-        float4 sv = *(const float4*)(part + i);
-        for (int j = 1; j < fa.sk_split[k]; ++j) {
-          const float4 pv = *(const float4*)(part + (int64_t)j * n + i);
-          sv.x += pv.x; sv.y += pv.y; sv.z += pv.z; sv.w += pv.w;
+      const int ks = fa.sk_split[k];
+      const bool vec = ((w0 | n) & 3) == 0;   // 16-byte accesses
+      const int64_t i = 4 * ((int64_t)bq * 64 + ql);
+      float pv[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = grp + 4 * u;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pv[u][c] = 0.f;
+        if (j < ks && i < n) {
+          const float* src = part + (int64_t)j * n + i;
+          if (vec) {
+            const float4 v = *(const float4*)src;
+            pv[u][0] = v.x; pv[u][1] = v.y; pv[u][2] = v.z; pv[u][3] = v.w;
+          } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+              if (i + c < n) pv[u][c] = src[c];
+          }
         }
-        fold_four(e, gcur, gnext, w0 + i, sv);
-      } else {
-        for (int64_t q = i; q < i + 4 && q < n; ++q) {
-          float sm = 0.f;
-          for (int j = 0; j < fa.sk_split[k]; ++j) sm += part[(int64_t)j * n + q];
-          fold_one(e, gcur, gnext, w0 + q, sm);
+      }
+      float4 sv;
+      sv.x = ((pv[0][0] + pv[1][0]) + pv[2][0]) + pv[3][0];
+      sv.y = ((pv[0][1] + pv[1][1]) + pv[2][1]) + pv[3][1];
+      sv.z = ((pv[0][2] + pv[1][2]) + pv[2][2]) + pv[3][2];
+      sv.w = ((pv[0][3] + pv[1][3]) + pv[2][3]) + pv[3][3];
+      red[grp][ql] = sv;
+      __syncthreads();
+      if (grp == 0 && i < n) {
+        const float4 a = red[0][ql], b = red[1][ql], c2 = red[2][ql], d = red[3][ql];
+        float4 t;
+        t.x = (a.x + b.x) + (c2.x + d.x);
+        t.y = (a.y + b.y) + (c2.y + d.y);
+        t.z = (a.z + b.z) + (c2.z + d.z);
+        t.w = (a.w + b.w) + (c2.w + d.w);
+        if (vec) {
+          fold_four(e, gcur, gnext, w0 + i, t);
+        } else {
+          const float tv[4] = {t.x, t.y, t.z, t.w};
+          for (int c = 0; c < 4 && i + c < n; ++c) fold_one(e, gcur, gnext, w0 + i + c, tv[c]);
         }
       }
       return;
     }
-    i -= nq;
+    bq -= nbk;
   }
 }
 void launch_fold(int prec, const FoldArgs& fa, const EpiWgrad& tmpl, const Ctl* ctl, int gate,
                  hipStream_t s) {
   const int nb_bias = (fa.bias_rows * 64 + 255) / 256;
-  int64_t nsk = 0;
-  for (int k = 0; k < fa.nsk; ++k) nsk += (fa.sk_n[k] + 3) / 4;
-  const int nb = nb_bias + (int)((nsk + 255) / 256);
+  int64_t nsk = 0;  // blocks of 64 quads per split-K layer
+  for (int k = 0; k < fa.nsk; ++k) nsk += ((fa.sk_n[k] + 3) / 4 + 63) / 64;
+  const int nb = nb_bias + (int)nsk;
   DISPATCH_PREC(prec, hipLaunchKernelGGL(k_fold<PP>, dim3(nb), dim3(256), 0, s, fa, tmpl, ctl,
                                          gate, nb_bias));
 }

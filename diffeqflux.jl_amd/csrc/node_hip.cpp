@@ -679,7 +679,7 @@ int autotune_impl(node_handle* h) {
   int64_t maxio = 0;
   for (const Layer& ly : h->layers) maxio = std::max<int64_t>(maxio, (int64_t)ly.in * ly.out);
   float* tmp = nullptr;
-  if (int rc = dalloc_t(h, &tmp, (size_t)8 * maxio * sizeof(float))) return rc;
+  if (int rc = dalloc_t(h, &tmp, (size_t)16 * maxio * sizeof(float))) return rc;  // split-K <= 16
   const uint64_t saved_mask = h->prof_mask;
   h->prof_mask = 0;
   const bool data = !h->tuned_data;
@@ -740,7 +740,7 @@ int autotune_impl(node_handle* h) {
     for (int c = 0; c < gemm_cfg_count(); ++c) {
       if (ly.inp % gemm_cfg_tile_rows(c) || ly.outp % gemm_cfg_tile_cols(c)) continue;
       const int tiles = (ly.inp / gemm_cfg_tile_rows(c)) * (ly.outp / gemm_cfg_tile_cols(c));
-      for (int ks = 1; ks <= 8; ks *= 2) {
+      for (int ks = 1; ks <= 16; ks *= 2) {
         if (ktiles % ks || tiles * ks > 4096) continue;
         h->tune_role = PR_WGRAD; h->tune_layer = l; h->tune_cfg = c; h->tune_ks = ks;
         h->tune_wpart = tmp;
