@@ -1138,11 +1138,6 @@ __global__ void __launch_bounds__(WR * WC * 64)
     }                                                                              \
   } while (0)
 
-#pragma unroll
-  for (int j = 0; j < S; ++j)
-    if (j < nk) GEMM_GLOAD(j, j);
-  GEMM_SSTORE(0, 0);
-  __syncthreads();
   TileGeo geo;
   geo.slot = bid;
   geo.row0 = row0;
@@ -1157,50 +1152,89 @@ __global__ void __launch_bounds__(WR * WC * 64)
   const int arow = wr * (BR / WR) + (lane & 31);
   const int brow = NPL * BR + wc * (BC / WC) + (lane & 31);
   const int koff = (lane >> 5) * 16;
-  for (int t0 = 0; t0 < nk; t0 += S) {
+  // one K tile from LDS buffer `buf` through the MFMAs
+#define GEMM_COMPUTE(buf)                                                          \
+  do {                                                                             \
+    const unsigned char* base_ = lds + (buf) * ROWS * PITCH;                       \
+    _Pragma("unroll") for (int kk = 0; kk < 4; ++kk) {                             \
+      u32x4 fa[NPL][MR], fb[NPL][MC];                                              \
+      _Pragma("unroll") for (int p = 0; p < NPL; ++p) {                            \
+        _Pragma("unroll") for (int mi = 0; mi < MR; ++mi)                          \
+          fa[p][mi] = *(const u32x4*)(base_ + (arow + p * BR + mi * 32) * PITCH + kk * 32 + koff); \
+        _Pragma("unroll") for (int mj = 0; mj < MC; ++mj)                          \
+          fb[p][mj] = *(const u32x4*)(base_ + (brow + p * BC + mj * 32) * PITCH + kk * 32 + koff); \
+      }                                                                            \
+      _Pragma("unroll") for (int mi = 0; mi < MR; ++mi)                            \
+        _Pragma("unroll") for (int mj = 0; mj < MC; ++mj) {                        \
+          Mma<T>::run(acc[mi][mj], fa[0][mi], fb[0][mj]);                          \
+          if constexpr (PP::SPLIT) {                                               \
+            Mma<T>::run(acc[mi][mj], fa[NPL - 1][mi], fb[0][mj]);                  \
+            Mma<T>::run(acc[mi][mj], fa[0][mi], fb[NPL - 1][mj]);                  \
+          }                                                                        \
+        }                                                                          \
+    }                                                                              \
+  } while (0)
+  // the data-gradient epilogue's operands, loaded during the last K tile
+#define GEMM_EPI_PREFETCH()                                                        \
+  do {                                                                             \
+    if constexpr (kPre) {                                                          \
+      bwd_data_prefetch<PP>(epi, pre, geo, lane);                                  \
+      if constexpr (kYlds)                                                         \
+        if (epi.yact >= 0)                                                         \
+          bwd_data_prefetch_lds<PP, BR, BC, NT>(epi, lds + (nk & 1) * ROWS * PITCH, geo); \
+    }                                                                              \
+  } while (0)
+  if (nk % S == 0 && nk >= 2 * S) {
+    // Branch-free schedule: every steady-state iteration refills its ring
+    // slot, the last S iterations load nothing.  With no load behind a
+    // branch the compiler's wait counts stay exact (a conditional refill
+    // makes it wait for the loads just issued), so S - 1 K tiles of loads
+    // stay in flight behind the MFMAs.
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const int t = t0 + j;
-      if (t < nk) {
-        // slot j held tile t, already in LDS: refill it with tile t+S
-        if (t + S < nk) GEMM_GLOAD(t + S, j);
-        if constexpr (kPre) {
-          if (t == nk - 1) {
-            bwd_data_prefetch<PP>(epi, pre, geo, lane);
-            if constexpr (kYlds)
-              if (epi.yact >= 0)
-                bwd_data_prefetch_lds<PP, BR, BC, NT>(epi, lds + (nk & 1) * ROWS * PITCH, geo);
-          }
-        }
-        const unsigned char* base = lds + (t & 1) * ROWS * PITCH;
+    for (int j = 0; j < S; ++j) GEMM_GLOAD(j, j);
+    GEMM_SSTORE(0, 0);
+    __syncthreads();
+    for (int t0 = 0; t0 < nk - S; t0 += S) {
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-          u32x4 fa[NPL][MR], fb[NPL][MC];
-#pragma unroll
-          for (int p = 0; p < NPL; ++p) {
-#pragma unroll
-            for (int mi = 0; mi < MR; ++mi)
-              fa[p][mi] = *(const u32x4*)(base + (arow + p * BR + mi * 32) * PITCH + kk * 32 + koff);
-#pragma unroll
-            for (int mj = 0; mj < MC; ++mj)
-              fb[p][mj] = *(const u32x4*)(base + (brow + p * BC + mj * 32) * PITCH + kk * 32 + koff);
-          }
-#pragma unroll
-          for (int mi = 0; mi < MR; ++mi)
-#pragma unroll
-            for (int mj = 0; mj < MC; ++mj) {
-              Mma<T>::run(acc[mi][mj], fa[0][mi], fb[0][mj]);
-              if constexpr (PP::SPLIT) {
-                Mma<T>::run(acc[mi][mj], fa[NPL - 1][mi], fb[0][mj]);
-                Mma<T>::run(acc[mi][mj], fa[0][mi], fb[NPL - 1][mj]);
-              }
-            }
-        }
-        if (t + 1 < nk) GEMM_SSTORE((t + 1) & 1, (j + 1) % S);
+      for (int j = 0; j < S; ++j) {
+        const int t = t0 + j;
+        GEMM_GLOAD(t + S, j);  // slot j held tile t, already in LDS
+        GEMM_COMPUTE(t & 1);
+        GEMM_SSTORE((t + 1) & 1, (j + 1) % S);
         __syncthreads();
       }
     }
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int t = nk - S + j;
+      if (j == S - 1) GEMM_EPI_PREFETCH();
+      GEMM_COMPUTE(t & 1);
+      if (j + 1 < S) GEMM_SSTORE((t + 1) & 1, (j + 1) % S);
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      if (j < nk) GEMM_GLOAD(j, j);
+    GEMM_SSTORE(0, 0);
+    __syncthreads();
+    for (int t0 = 0; t0 < nk; t0 += S) {
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const int t = t0 + j;
+        if (t < nk) {
+          // slot j held tile t, already in LDS: refill it with tile t+S
+          if (t + S < nk) GEMM_GLOAD(t + S, j);
+          if (t == nk - 1) GEMM_EPI_PREFETCH();
+          GEMM_COMPUTE(t & 1);
+          if (t + 1 < nk) GEMM_SSTORE((t + 1) & 1, (j + 1) % S);
+          __syncthreads();
+        }
+      }
+    }
   }
+#undef GEMM_COMPUTE
+#undef GEMM_EPI_PREFETCH
 #undef GEMM_GLOAD
 #undef GEMM_SSTORE
   if (g.ws != nullptr && g.ksplit > 1) {
@@ -1252,6 +1286,15 @@ template <class PP, int BR, int BC> struct Depth {
   static constexpr int value = (BR * BC >= 128 * 128) ? (PP::SPLIT ? 2 : 3) : (PP::SPLIT ? 3 : 4);
 };
 
+// the 8-wave 128x128 tiles load 32 (split) or 16 VGPRs of operands per thread
+// and ring slot: four slots keep ~3 K tiles of loads in flight per CU (the
+// main loop is bound by load latency at two); the data-gradient epilogue's
+// act' prefetch leaves no room for more than the default
+template <class PP, typename Epi> struct Depth8 {
+  static constexpr int value =
+      std::is_same<Epi, EpiBwdData>::value ? Depth<PP, 128, 128>::value : 4;
+};
+
 template <class PP, typename Epi>
 static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, int gate,
                      hipStream_t s) {
@@ -1274,11 +1317,11 @@ static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, i
     case 3: hipLaunchKernelGGL((gemm_tn<PP, 32, 64, 1, 2, Depth<PP, 32, 64>::value, Epi>), dim3(nb), dim3(128), 0, s, ga, e, ctl, gate); break;
     case 4:
       if constexpr (!kStage)
-        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 4, 2, Depth<PP, 128, 128>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 4, 2, Depth8<PP, Epi>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
       break;
     default:
       if constexpr (!kStage)
-        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 4, Depth<PP, 128, 128>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 4, Depth8<PP, Epi>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
       break;
   }
 }
