@@ -85,6 +85,8 @@ def backward(mlp, p, fwd, dL_du, **solver_kw):
                       save_start=False, save_end=True, save_everystep=False,
                       callback_times=sorted(jumps), affect=affect,
                       record_dense=False, **kw)
+    if not adj.u:  # the adjoint solve failed before T -> t0 (see adj.retcode)
+        return np.full(u_shape, np.nan), np.full(P, np.nan), adj
     yT = adj.u[-1]
     return yT[:nlam].reshape(u_shape), yT[nlam:], adj
 
