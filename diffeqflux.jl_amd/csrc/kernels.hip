@@ -1286,15 +1286,6 @@ template <class PP, int BR, int BC> struct Depth {
   static constexpr int value = (BR * BC >= 128 * 128) ? (PP::SPLIT ? 2 : 3) : (PP::SPLIT ? 3 : 4);
 };
 
-// the 8-wave 128x128 tiles load 32 (split) or 16 VGPRs of operands per thread
-// and ring slot: four slots keep ~3 K tiles of loads in flight per CU (the
-// main loop is bound by load latency at two); the data-gradient epilogue's
-// act' prefetch leaves no room for more than the default
-template <class PP, typename Epi> struct Depth8 {
-  static constexpr int value =
-      std::is_same<Epi, EpiBwdData>::value ? Depth<PP, 128, 128>::value : 4;
-};
-
 template <class PP, typename Epi>
 static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, int gate,
                      hipStream_t s) {
@@ -1317,11 +1308,11 @@ static void launch_t(int cfg, const GemmArgs& g, const Epi& e, const Ctl* ctl, i
     case 3: hipLaunchKernelGGL((gemm_tn<PP, 32, 64, 1, 2, Depth<PP, 32, 64>::value, Epi>), dim3(nb), dim3(128), 0, s, ga, e, ctl, gate); break;
     case 4:
       if constexpr (!kStage)
-        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 4, 2, Depth8<PP, Epi>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 4, 2, Depth<PP, 128, 128>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
       break;
     default:
       if constexpr (!kStage)
-        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 4, Depth8<PP, Epi>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
+        hipLaunchKernelGGL((gemm_tn<PP, 128, 128, 2, 4, Depth<PP, 128, 128>::value, Epi>), dim3(nb), dim3(512), 0, s, ga, e, ctl, gate);
       break;
   }
 }
