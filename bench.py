@@ -505,7 +505,7 @@ def main():
                     help="skip the multiple-shooting workload leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); "
+        print("bench.py: the NeuralODE library runs on one GPU only (no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
