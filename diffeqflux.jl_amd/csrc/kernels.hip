@@ -2236,38 +2236,59 @@ __global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_b
   if (!gate_ok(c, gate)) return;
   float* gcur = c->gsel ? e.gf1 : e.gf0;
   float* gnext = c->gsel ? e.gf0 : e.gf1;
-  if ((int)blockIdx.x < nb_bias) {
-    // the row (and with it the layer) is wave-uniform: readfirstlane keeps the
-    // per-layer argument lookups on scalar loads instead of per-lane loads of
-    // the kernel-argument segment, one dependent round trip each
+  if ((int)blockIdx.x < fa.nbw) {
+    // bias rows of layers with per-row-tile column sums: one wave per row.
+    // The row (and with it the layer) is wave-uniform: readfirstlane keeps
+    // the per-layer argument lookups on scalar loads instead of per-lane loads
+    // of the kernel-argument segment, one dependent round trip each
     const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int lane = threadIdx.x & 63;
-    if (wave >= fa.bias_rows) return;
     int l = 0, row = wave;
-    while (l < fa.nl - 1 && row >= fa.outs[l]) { row -= fa.outs[l]; ++l; }
-    float s = 0.f;
-    if (fa.bpart[l]) {
-      const float* bp = fa.bpart[l] + row;
-      for (int t = lane; t < fa.bpart_rt[l]; t += 64) s += bp[(int64_t)t * fa.bpart_ld[l]];
-    } else {
-      // eight independent partial sums keep eight loads per lane in flight
-      // (one dependent chain of Bp / 64 loads is latency bound); fixed order
-      const T* src = (const T*)fa.dT[l] + (int64_t)row * fa.ldt;
-      float q[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      int b = lane;
-      for (; b + 7 * 64 < fa.Bp; b += 8 * 64) {
-        float v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = get<PP>(src + b + j * 64, fa.dplane[l]);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) q[j] += v[j];
-      }
-      for (int j = 0; b < fa.Bp; b += 64, ++j) q[j] += get<PP>(src + b, fa.dplane[l]);
-      s = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+    for (; l < fa.nl; ++l) {
+      if (!fa.bpart[l]) continue;
+      if (row < fa.outs[l]) break;
+      row -= fa.outs[l];
     }
+    if (l >= fa.nl) return;
+    const float* bp = fa.bpart[l] + row;
+    float s = 0.f;
+    for (int t = lane; t < fa.bpart_rt[l]; t += 64) s += bp[(int64_t)t * fa.bpart_ld[l]];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) fold_one(e, gcur, gnext, fa.b0s[l] + row, -s);
+    return;
+  }
+  if ((int)blockIdx.x < nb_bias) {
+    // bias rows of the other layers (the transposed delta summed over the
+    // batch): one block per row, eight independent partial sums per thread
+    // (eight loads in flight), reduced in a fixed order (lanes, then waves)
+    __shared__ float redw[4];
+    int l = 0, row = (int)blockIdx.x - fa.nbw;
+    for (; l < fa.nl; ++l) {
+      if (fa.bpart[l]) continue;
+      if (row < fa.outs[l]) break;
+      row -= fa.outs[l];
+    }
+    if (l >= fa.nl) return;  // (block-uniform)
+    const T* src = (const T*)fa.dT[l] + (int64_t)row * fa.ldt;
+    float q[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const int nt = (int)blockDim.x;
+    int b = threadIdx.x;
+    for (; b + 7 * nt < fa.Bp; b += 8 * nt) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = get<PP>(src + b + j * nt, fa.dplane[l]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) q[j] += v[j];
+    }
+    for (int j = 0; b < fa.Bp; b += nt, ++j) q[j] += get<PP>(src + b, fa.dplane[l]);
+    float s = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) redw[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      fold_one(e, gcur, gnext, fa.b0s[l] + row, -((redw[0] + redw[1]) + (redw[2] + redw[3])));
     return;
   }
   // split-K partial sums (at most 16 splits): a block takes 64 quads of four
@@ -2330,9 +2351,15 @@ __global__ void k_fold(FoldArgs fa, EpiWgrad e, const Ctl* c, int gate, int nb_b
     bq -= nbk;
   }
 }
-void launch_fold(int prec, const FoldArgs& fa, const EpiWgrad& tmpl, const Ctl* ctl, int gate,
+void launch_fold(int prec, const FoldArgs& fa_in, const EpiWgrad& tmpl, const Ctl* ctl, int gate,
                  hipStream_t s) {
-  const int nb_bias = (fa.bias_rows * 64 + 255) / 256;
+  // blocks: wave-per-row bias rows of the layers with column-sum partials,
+  // then one block per bias row of the other layers, then the split-K sums
+  FoldArgs fa = fa_in;
+  int rows_w = 0, rows_b = 0;
+  for (int l = 0; l < fa.nl; ++l) (fa.bpart[l] ? rows_w : rows_b) += fa.outs[l];
+  fa.nbw = (rows_w * 64 + 255) / 256;
+  const int nb_bias = fa.nbw + rows_b;
   int64_t nsk = 0;  // blocks of 64 quads per split-K layer
   for (int k = 0; k < fa.nsk; ++k) nsk += ((fa.sk_n[k] + 3) / 4 + 63) / 64;
   const int nb = nb_bias + (int)nsk;

@@ -170,6 +170,7 @@ struct FoldArgs {
   // (nullptr: the bias gradient sums the transposed delta over the batch)
   const float* bpart[MAXL];
   int bpart_rt[MAXL], bpart_ld[MAXL];
+  int nbw;                     // wave-per-row bias blocks (set by the launcher)
   int ldt, Bp;
   int nsk;                     // narrow layers with split-K partials
   int64_t sk_w0[MAXL], sk_n[MAXL];  // flat offset / entry count
