@@ -129,3 +129,30 @@ def test_cnf_backward_without_forward_raises():
     h.set_params(np.zeros(h.P, np.float32))
     with pytest.raises(RuntimeError):
         h.cnf_backward(np.zeros((4, 2), np.float32), np.zeros(4, np.float32))
+
+
+def test_cnf_adjoint_full_workload_split_vs_fp32():
+    """The FFJORD workload at full size (B = 8192, 17-512-512-16 tanh,
+    time-concatenated): split-BF16 gradients against the FP32 kernels' on the
+    same fixed-dt step sequence (a size-independent check; the oracle covers
+    B = 128 of this shape above)."""
+    from diffeqflux_hip import Handle, _lib
+    widths, acts, B = [17, 512, 512, 16], ["tanh", "tanh", "identity"], 8192
+    m = MLP(widths, acts)
+    rng = np.random.default_rng(4)
+    p32 = m.glorot_init(rng).astype(np.float32)
+    x = rng.standard_normal((B, 16)).astype(np.float32)
+    e = rng.choice(np.array([-1.0, 1.0], np.float32), size=(B, 16))
+    res = {}
+    for prec in ("fp32", "bf16x3"):
+        h = Handle(widths, acts, _lib.MAP_CNF_TIME, prec, B)
+        h.set_params(p32)
+        lp, z, dl, st = h.cnf_forward(x, e, 0.0, 1.0, adaptive=False, dt=1.0 / 16)
+        gz, gd = cotangents(z, B, "nll")
+        dx, dp, sb = h.cnf_backward(gz, gd)
+        assert st["retcode"] == 1 and sb["retcode"] == 1
+        res[prec] = (lp, dx, dp)
+    el = rel(res["bf16x3"][0], res["fp32"][0])
+    ex, ep = rel(res["bf16x3"][1], res["fp32"][1]), rel(res["bf16x3"][2], res["fp32"][2])
+    print(f"\n  B=8192 split BF16 vs FP32: logpx {el:.2e} dL/dx {ex:.2e} dL/dp {ep:.2e}", flush=True)
+    assert el < 1e-4 and ex < TOL["bf16x3"]["grad"] and ep < TOL["bf16x3"]["grad"]
