@@ -302,3 +302,32 @@ def test_autotuned_plans_match_heuristic_results():
         print(f"  autotune={tuned}: plans {h.plans()}", flush=True)
     for a, b in zip(res[False], res[True]):
         assert rel(a, b) < 1e-5
+
+
+def test_headline_shape_split_vs_fp32_full_batch():
+    """The headline workload at full size (D = 64, 64-1024-1024-1024-64
+    softplus, B = 4096, 32 saveat points): split-BF16 forward states and
+    gradients against the FP32 kernels' on one fixed-dt step sequence (a
+    size-independent check; the oracle covers the same model on a batch
+    sample in the bench and smaller shapes above)."""
+    from diffeqflux_hip import Handle
+    widths, acts, B = [64, 1024, 1024, 1024, 64], ["softplus"] * 3 + ["identity"], 4096
+    m = MLP(widths, acts)
+    rng = np.random.default_rng(6)
+    p32 = m.glorot_init(rng).astype(np.float32)
+    u0 = rng.standard_normal((B, 64)).astype(np.float32)
+    ts = list(np.linspace(0.0, 1.0, 32))
+    c = _cotangents((32, B, 64), seed=7) / B
+    res = {}
+    for prec in ("fp32", "bf16x3"):
+        h = Handle(widths, acts, 0, prec, B)
+        h.set_params(p32)
+        opts = Handle.make_opts(0.0, 1.0, adaptive=False, dt=1.0 / 12)
+        out, t, st = h.forward(u0, opts, ts[1:-1])
+        du0, dp, sb = h.backward(c)
+        assert st["retcode"] == 1 and sb["retcode"] == 1 and out.shape == (32, B, 64)
+        res[prec] = (out, du0, dp)
+    eo, eu, ep = (rel(res["bf16x3"][i], res["fp32"][i]) for i in range(3))
+    print(f"\n  B=4096 split BF16 vs FP32: states {eo:.2e} dL/du0 {eu:.2e} dL/dp {ep:.2e}",
+          flush=True)
+    assert eo < TOL["bf16x3"]["fwd"] and eu < TOL["bf16x3"]["grad"] and ep < TOL["bf16x3"]["grad"]
