@@ -479,7 +479,9 @@ void launch_tan_bwd(node_handle* h, int l, int gate) {
   EpiTanBwd e{};
   e.C = ly.in; e.ldo = ly.inp; e.ldt = h->ldT; e.act = lo.act;
   e.Y = h->H[l - 1]; e.S = h->S[l - 1]; e.Tf = h->Tn[l - 1]; e.py = e.ptf = pd;
-  e.Ab = h->Ab[l - 1]; e.AbT = tan_half(h, h->DlT[l - 1]);
+  // (abar_0 row-major feeds no further tangent GEMM: only its transposed copy)
+  e.Ab = l - 1 >= 1 ? h->Ab[l - 1] : nullptr;
+  e.AbT = tan_half(h, h->DlT[l - 1]);
   e.pab = pd; e.pabt = (int64_t)h->ldT * ly.inp;
   e.src = h->src2[l - 1];
   ProfScope ps(h, PR_BWD_DATA, l);
