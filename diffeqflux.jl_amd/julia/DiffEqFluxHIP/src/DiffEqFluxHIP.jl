@@ -16,13 +16,16 @@ supported case, and falls back to the reference method otherwise:
 
 `ChainRulesCore.rrule(::typeof(hip_neuralode), ...)` is the pullback: it
 calls `node_backward` (the HIP InterpolatingAdjoint, or BacksolveAdjoint as
-selected by `NODE_OPT_SENSEALG`) on the forward tape kept in the handle.  Julia was not available where this package was written; the
+selected by `NODE_OPT_SENSEALG`) on the forward tape kept in the handle.
+`FFJORD` (src/ffjord.jl:55-178) with a `Chain` of `Dense` layers, `Tsit5()`,
+`monte_carlo = true`, no regularisation and the standard-normal base goes to
+`node_cnf_forward`, its pullback to `node_cnf_backward`.  Julia was not available where this package was written; the
 call contract it binds is exercised from Python through the same C ABI.
 """
 module DiffEqFluxHIP
 
 using ChainRulesCore: ChainRulesCore, NoTangent, ZeroTangent, unthunk
-using DiffEqFlux: DiffEqFlux, NeuralODE
+using DiffEqFlux: DiffEqFlux, NeuralODE, FFJORD
 using Lux: Lux, Chain, Dense, WrappedFunction
 using OrdinaryDiffEq: Tsit5
 using SciMLSensitivity: InterpolatingAdjoint, BacksolveAdjoint, ZygoteVJP
@@ -240,6 +243,102 @@ function (n::NeuralODE{<:Chain})(x::AbstractArray{Float32}, p, st)
     out, ts, ret = hip_neuralode(x, pflat, h, o, interior)
     arr = ndims(x) == 1 ? reshape(out, size(out, 1), size(out, 3)) : out
     return HIPSolution(arr, ts, ret), st
+end
+
+# ---- FFJORD (src/ffjord.jl:55-178): monte_carlo = true, no regularisation,
+# standard-normal base distribution, model(z) (NODE_MAP_CNF)
+const NODE_MAP_CNF = Int32(2)
+
+function ffjord_spec(model)
+    spec = chain_spec(model)
+    spec === nothing && return nothing
+    widths, acts, imap = spec
+    (imap == 0 && acts[end] == 0 && length(acts) >= 2 && widths[end] + 1 <= 64) || return nothing
+    return widths, acts
+end
+
+supported(n::FFJORD, x, st) = ffjord_spec(n.model) !== nothing && length(n.args) >= 1 &&
+    n.args[1] isa Tsit5 && n.basedist === nothing && eltype(x) === Float32 && ndims(x) == 2 &&
+    !st.regularize && st.monte_carlo && n.tspan[2] > n.tspan[1]
+
+function ffjord_handle_for(n::FFJORD, B::Int; precision::Int32 = Int32(0))
+    key = (objectid(n), B)
+    get!(HANDLES, key) do
+        widths, acts = ffjord_spec(n.model)
+        w = ntuple(i -> i <= length(widths) ? widths[i] : Int32(0), MAX_LAYERS + 1)
+        a = ntuple(i -> i <= length(acts) ? acts[i] : Int32(0), MAX_LAYERS)
+        Handle(NodeDesc(Int32(length(acts)), w, a, NODE_MAP_CNF, precision, Int64(B)))
+    end
+end
+
+"""
+    hip_ffjord(x, p_flat, e, h, opts) -> (logpx (1 x B), z(T) (D x B), delta_logp (1 x B))
+
+The FFJORD log-likelihood solve on the GPU (node_cnf_forward) with the
+Hutchinson probe `e` (D x B) held for the whole solve.  Differentiable through
+the `rrule` below (node_cnf_backward: the layer's InterpolatingAdjoint with the
+second-order VJP of the trace term, same probe).
+"""
+function hip_ffjord(x::Matrix{Float32}, p::AbstractVector{Float32}, e::Matrix{Float32}, h::Handle,
+                    o::SolveOpts)
+    pv = Vector{Float32}(p)
+    check(h, ccall((:node_set_params, libnode()), Int32, (Ptr{Cvoid}, Ptr{Float32}, Int64),
+                   h.ptr, pv, length(pv)), "node_set_params")
+    D, B = size(x)
+    z = Matrix{Float32}(undef, D, B)
+    dl = Vector{Float32}(undef, B)
+    lp = Vector{Float32}(undef, B)
+    st = Stats()
+    rc = ccall((:node_cnf_forward, libnode()), Int32,
+               (Ptr{Cvoid}, Ref{SolveOpts}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
+                Ptr{Float32}, Ptr{Float32}, Ref{Stats}), h.ptr, o, x, e, z, dl, lp, st)
+    check(h, rc, "node_cnf_forward")
+    st.retcode == 1 || error("FFJORD solve failed: $(RETCODES[st.retcode + 1])")
+    return reshape(lp, 1, B), z, reshape(dl, 1, B)
+end
+
+function CRC.rrule(::typeof(hip_ffjord), x, p, e, h::Handle, o::SolveOpts)
+    lp, z, dl = hip_ffjord(x, p, e, h, o)
+    function hip_ffjord_pullback(Δ)
+        tang(t, like) = (t = unthunk(t); t isa Union{NoTangent, ZeroTangent} ? zero(like) : t)
+        Δlp, Δz, Δdl = tang(Δ[1], lp), tang(Δ[2], z), tang(Δ[3], dl)
+        # logpx = -|z|^2 / 2 - D/2 log(2π) - delta_logp
+        gz = Matrix{Float32}(Δz .- Δlp .* z)
+        gd = Vector{Float32}(vec(Δdl .- Δlp))
+        dx = Matrix{Float32}(undef, size(x))
+        dp = Vector{Float64}(undef, h.P)
+        st = Stats()
+        rc = ccall((:node_cnf_backward, libnode()), Int32,
+                   (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float64}, Ref{Stats}),
+                   h.ptr, gz, gd, dx, dp, st)
+        check(h, rc, "node_cnf_backward")
+        return NoTangent(), dx, Float32.(dp), NoTangent(), NoTangent(), NoTangent()
+    end
+    return (lp, z, dl), hip_ffjord_pullback
+end
+
+function make_opts(n::FFJORD)
+    kw = n.kwargs
+    t0, t1 = Float64.(n.tspan)
+    return SolveOpts(t0, t1, get(kw, :abstol, 1e-6), get(kw, :reltol, 1e-3),
+                     Int32(get(kw, :adaptive, true)), Int32(0), Int32(1), Int32(0),
+                     Float64(get(kw, :dt, 0.0)), Float64(get(kw, :dtmax, 0.0)),
+                     Float64(get(kw, :dtmin, 0.0)), Int64(get(kw, :maxiters, 0)))
+end
+
+# the drop-in for FFJORD: one Gaussian probe per call, held for the whole solve
+# (the reference redraws randn! inside every right-hand-side call, :96)
+function (n::FFJORD{<:Chain})(x::AbstractMatrix{Float32}, ps, st)
+    supported(n, x, st) || return invoke(n, Tuple{Any, Any, Any}, x, ps, st)  # reference path
+    D, B = size(x)
+    h = ffjord_handle_for(n, B)
+    pflat = ps isa AbstractVector ? Vector{Float32}(ps) :
+        reduce(vcat, [vcat(vec(l.weight), l.bias) for l in values(ps)
+                      if l isa NamedTuple && haskey(l, :weight)])
+    e = CRC.@ignore_derivatives randn(Float32, D, B)
+    lp, _, dl = hip_ffjord(Matrix{Float32}(x), pflat, e, h, make_opts(n))
+    return (lp, dl, dl), (; model = st.model, regularize = st.regularize,
+                          monte_carlo = st.monte_carlo)
 end
 
 end # module
