@@ -145,7 +145,8 @@ struct node_handle {
   bool use_graphs = true;
   hipGraphExec_t fwd_graph = nullptr, adj_graph = nullptr;
   const void* fwd_graph_sig[3] = {nullptr, nullptr, nullptr};  // tape, save times, outputs
-  const void* adj_graph_sig[3] = {nullptr, nullptr, nullptr};  // tape, save times, cotangents
+  // tape, save times, cotangents, saved states (BacksolveAdjoint checkpoints)
+  const void* adj_graph_sig[4] = {nullptr, nullptr, nullptr, nullptr};
   int graph_attempts = 4;
 
   // ragged batches: flat parameters and the last per-trajectory forward
@@ -1138,7 +1139,7 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   };
   const bool graphs = h->use_graphs && h->prof_mask == 0;
   const int ga = h->graph_attempts;
-  const void* asig[3] = {h->tape, h->save_times, d_dLdout};
+  const void* asig[4] = {h->tape, h->save_times, d_dLdout, h->out_dev};
   if (h->adj_graph && std::memcmp(asig, h->adj_graph_sig, sizeof(asig)) != 0) {
     (void)hipGraphExecDestroy(h->adj_graph);
     h->adj_graph = nullptr;
@@ -1648,6 +1649,7 @@ int32_t node_forward_device(node_handle_t* h, const node_solve_opts* opts, const
   if (!h) return NODE_ERR_INVALID;
   std::lock_guard<std::mutex> g(h->mtx);
   if (!d_u0) return fail(h, NODE_ERR_INVALID, "u0 is NULL");
+  if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "CNF handle: use node_cnf_forward");
   if (n_saveat < 0 || (n_saveat > 0 && !saveat)) return fail(h, NODE_ERR_INVALID, "bad saveat");
   int rc = forward_impl(h, opts, d_u0, saveat, n_saveat, stats);
   if (rc) return rc;

@@ -106,3 +106,26 @@ def test_ffjord_layer_api():
     rlp, _, _, _ = FF.forward(m, ps.astype(np.float64), x.astype(np.float64), 0.0, 1.0,
                               probe=e.astype(np.float64), time_concat=True, abstol=1e-5, reltol=1e-5)
     assert np.abs(lp[0] - rlp).max() / np.abs(rlp).max() < 1e-3
+
+
+def test_cnf_handle_refuses_plain_forward_entry_points():
+    """A CNF handle holds the state [z; log p] and needs its probe: the plain
+    NeuralODE forward entry points (host and device) refuse it before touching
+    any buffer."""
+    import ctypes
+    import torch
+    from diffeqflux_hip import Handle, _lib
+    *_, h, _ = setup("z_only", "fp32")
+    opts = Handle.make_opts(0.0, 1.0)
+    st = _lib.Stats()
+    u = torch.zeros((h.B, 2), device="cuda")
+    out = torch.zeros((2, h.B, 2), device="cuda")
+    rc = h.lib.node_forward_device(h.h, ctypes.byref(opts), ctypes.c_void_p(u.data_ptr()), None,
+                                   0, ctypes.c_void_p(out.data_ptr()), 2, ctypes.byref(st))
+    assert rc == -5, rc
+    assert b"node_cnf_forward" in h.lib.node_last_error(h.h)
+    uh = np.zeros((h.B, 2), np.float32)
+    oh = np.zeros((2, h.B, 2), np.float32)
+    rc = h.lib.node_forward(h.h, ctypes.byref(opts), uh.ctypes.data_as(ctypes.c_void_p), None, 0,
+                            oh.ctypes.data_as(ctypes.c_void_p), 2, ctypes.byref(st))
+    assert rc == -5, rc

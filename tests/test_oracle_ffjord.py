@@ -98,7 +98,8 @@ def _small_cnf(acts, tcat, seed=0):
     return m, p, D, rng
 
 
-@pytest.mark.parametrize("acts", [["tanh", "tanh", "identity"], ["softplus", "sigmoid", "identity"]])
+@pytest.mark.parametrize("acts", [["tanh", "tanh", "identity"], ["softplus", "sigmoid", "identity"],
+                                  ["relu", "tanh", "identity"], ["tanh", "relu", "identity"]])
 @pytest.mark.parametrize("tcat", [False, True])
 def test_trace_vjp_matches_torch_double_backward(acts, tcat):
     import torch
@@ -112,7 +113,7 @@ def test_trace_vjp_matches_torch_double_backward(acts, tcat):
     xt = torch.tensor(x, requires_grad=True)
     pt = torch.tensor(p, requires_grad=True)
     acts_t = {"tanh": torch.tanh, "softplus": torch.nn.functional.softplus,
-              "sigmoid": torch.sigmoid, "identity": lambda v: v}
+              "sigmoid": torch.sigmoid, "relu": torch.relu, "identity": lambda v: v}
 
     def f(xx):
         h = xx

@@ -7,7 +7,7 @@ import torch
 from oracle.mlp import MLP, MAP_CUBE, MAP_IDENTITY
 
 
-@pytest.mark.parametrize("act", ["tanh", "softplus", "sigmoid", "identity"])
+@pytest.mark.parametrize("act", ["tanh", "softplus", "sigmoid", "relu", "identity"])
 @pytest.mark.parametrize("imap", [MAP_IDENTITY, MAP_CUBE])
 def test_vjp_vs_torch_autograd(act, imap):
     rng = np.random.default_rng(5)
@@ -22,17 +22,19 @@ def test_vjp_vs_torch_autograd(act, imap):
     tp = torch.tensor(p, requires_grad=True)
     h = tu ** 3 if imap == MAP_CUBE else tu
     fn = {"tanh": torch.tanh, "softplus": torch.nn.functional.softplus,
-          "sigmoid": torch.sigmoid, "identity": lambda z: z}
+          "sigmoid": torch.sigmoid, "relu": torch.relu, "identity": lambda z: z}
     for li, (i, j) in enumerate(zip(m.widths[:-1], m.widths[1:])):
         w0, b0, e = m.offsets[li]
         h = fn[m_act(m, li)](h @ tp[w0:b0].reshape(i, j) + tp[b0:e])
     (h * torch.tensor(lam)).sum().backward()
+    if act == "relu":  # the gradient must see the kink: some units are off
+        assert 0 < int((gp == 0).sum()) < gp.size
     assert np.allclose(gx, tu.grad.numpy(), rtol=1e-12, atol=1e-13)
     assert np.allclose(gp, tp.grad.numpy(), rtol=1e-12, atol=1e-13)
 
 
 def m_act(m, li):
-    inv = {0: "identity", 1: "tanh", 2: "softplus", 4: "sigmoid"}
+    inv = {0: "identity", 1: "tanh", 2: "softplus", 3: "relu", 4: "sigmoid"}
     return inv[m.acts[li]]
 
 

@@ -95,6 +95,60 @@ def test_initdt_heuristic_closed_form():
     assert abs(dt - expect) < 1e-15
 
 
+def _hairer_wanner_ii4(f, x0, y0, atol, rtol, order):
+    """Starting step size, Hairer-Norsett-Wanner, Solving ODEs I, II.4 (the
+    algorithm steps (a)-(f) as printed), with an explicit loop norm."""
+    n = len(y0)
+    sc = [atol + abs(v) * rtol for v in y0]
+    norm = lambda v: math.sqrt(sum((v[i] / sc[i]) ** 2 for i in range(n)) / n)
+    fx0 = f(y0, x0)
+    d0, d1 = norm(y0), norm(fx0)                                   # (a)
+    h0 = 1e-6 if (d0 < 1e-5 or d1 < 1e-5) else 0.01 * (d0 / d1)    # (b)
+    y1 = [y0[i] + h0 * fx0[i] for i in range(n)]                   # (c)
+    fx1 = f(y1, x0 + h0)
+    d2 = norm([fx1[i] - fx0[i] for i in range(n)]) / h0            # (d)
+    mx = max(d1, d2)
+    h1 = max(1e-6, h0 * 1e-3) if mx <= 1e-15 else (0.01 / mx) ** (1.0 / order)  # (e)
+    return min(100 * h0, h1)                                       # (f)
+
+
+_INITDT_CASES = {
+    # linear decay: the (d)-(e) branch binds, d2 = ||A f0|| whatever h0
+    "fast_decay": (lambda y: y * np.array([-40.0, -3.0, -0.5]), [1.0, -2.0, 0.25], False),
+    # near rest with a large forcing: ||y0|| << ||f(y0)||, so 100 h0 binds and
+    # a swapped d0 / d1 changes the step by orders of magnitude
+    "near_rest": (lambda y: np.stack([np.sin(3 * y[..., 1]) + 5.0, -y[..., 0] ** 2], -1),
+                  [1e-3, 2e-3], True),
+    "saturating": (lambda y: 50 * np.tanh(y * np.array([-40.0, -3.0, -0.5])) + 0.1 * y ** 2,
+                   [1e-2, -2e-2, 2.5e-3], True),
+}
+
+
+@pytest.mark.parametrize("case", list(_INITDT_CASES))
+def test_initdt_matches_hairer_wanner_with_distinct_norms(case):
+    """Problems whose ||y0|| and ||f(y0)|| differ by orders of magnitude: the
+    oracle's initial step equals the textbook algorithm (order 5 for Tsit5),
+    and where the 100 h0 cap binds, a swapped d0 / d1 would not."""
+    atol, rtol = 1e-6, 1e-3
+    f1, y0l, discriminates = _INITDT_CASES[case]
+    f = lambda y, t: f1(y)
+    y0 = np.array(y0l)
+    fl = lambda y, t: list(f1(np.asarray(y, float)))
+    expect = _hairer_wanner_ii4(fl, 0.0, list(y0), atol, rtol, 5)
+    dt, _ = tsit5.initdt(f, y0, f(y0, 0.0), 0.0, 1.0, 10.0, atol, rtol, 1e-16)
+    assert abs(dt - expect) <= 1e-13 * expect
+    sc = atol + np.abs(y0) * rtol
+    d0 = float(np.sqrt(np.mean((y0 / sc) ** 2)))
+    d1 = float(np.sqrt(np.mean((f(y0, 0.0) / sc) ** 2)))
+    assert abs(math.log(d0 / d1)) > 1.0
+    if discriminates:  # the same algorithm with d0 and d1 swapped lands elsewhere
+        h0_sw = 0.01 * d1 / d0
+        y1 = y0 + h0_sw * f(y0, 0.0)
+        d2_sw = float(np.sqrt(np.mean(((f(y1, h0_sw) - f(y0, 0.0)) / sc) ** 2))) / h0_sw
+        alt = min(100 * h0_sw, (0.01 / max(d1, d2_sw)) ** 0.2)
+        assert abs(alt - expect) > 0.5 * expect
+
+
 def test_retcode_maxiters():
     _, u0, f = _linear()
     sol = tsit5.solve(f, u0, 0.0, 1.0, maxiters=3)
