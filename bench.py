@@ -26,6 +26,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -88,14 +89,20 @@ def make_inputs(B, seed=1):
     return u0, ts, teacher_targets(u0, ts)
 
 
-def flops_per_launch(role, layer, B, precision, wgrad_precision=None):
-    """Algorithmic FLOPs of one launch of the (role, layer) GEMM: 2*B*in*out.
-    For split BF16 the MFMA executes three BF16 passes; the roofline counts
-    the MFMA work actually issued (3x) against the BF16 peak.  Role 4 (weight
-    gradients) runs in ``wgrad_precision`` when that is given."""
+def flops_per_launch(role, layer, B):
+    """Algorithmic FLOPs of one launch of the (role, layer) GEMM: 2*B*in*out
+    (the work the Dense chain or its pullback asks for, whatever the number of
+    MFMA passes the precision mode issues for it)."""
     i, o = WIDTHS[layer], WIDTHS[layer + 1]
+    return 2.0 * B * i * o
+
+
+def passes(role, precision, wgrad_precision=None):
+    """BF16 MFMA passes issued per algorithmic product: split BF16 runs three
+    (hi*hi + lo*hi + hi*lo); role 4 (weight gradients) runs in
+    ``wgrad_precision`` when that is given."""
     prec = wgrad_precision if (role == 4 and wgrad_precision) else precision
-    return 2.0 * B * i * o * (3 if prec == "bf16x3" else 1)
+    return 3 if prec == "bf16x3" else 1
 
 
 # ---------------------------------------------------------------- GPU legs
@@ -149,13 +156,29 @@ def gemm_shape(role, layer, B):
     return (i, o, B)           # weight gradient: in x out, K = batch
 
 
-# HBM bytes per launch of a kernel, from two rocprofv3 --pmc runs of this
-# bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the same
-# counters read for a plain 256 MiB device copy (FETCH_SIZE 0.500x of the bytes
-# read, WRITE_SIZE 1.000x of the bytes written): bwd_data at K = 1024 (8-wave
-# 128x128 tiles, act' derived from the layer output) fetches 32,921 KiB x 2.0
-# and writes 32,896 KiB per launch.
-TRAFFIC = {("bf16x3", 3, (4096, 1024, 1024)): (32921 * 2.0 + 32896) * 1024}
+# HBM bytes per launch of the dominant kernel come from the committed PMC
+# summary of this bench's command (two rocprofv3 --pmc runs, FETCH_SIZE then
+# WRITE_SIZE in KiB per dispatch, each scaled by what the counter reads for a
+# plain device copy of known size); the summary carries one machine-readable
+# line "traffic_bytes_per_launch: <precision> <role> <M>x<N>x<K> <bytes>".
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic():
+    out = {}
+    if not os.path.isdir(PROFILES):
+        return out
+    for fn in sorted(os.listdir(PROFILES)):
+        if not fn.startswith("traffic_"):
+            continue
+        with open(os.path.join(PROFILES, fn)) as f:
+            for line in f:
+                mt = re.match(r"\s*traffic_bytes_per_launch:\s*(\S+)\s+(\d+)\s+(\d+)x(\d+)x(\d+)"
+                              r"\s+(\d+)", line)
+                if mt:
+                    shape = tuple(int(mt.group(k)) for k in (3, 4, 5))
+                    out[(mt.group(1), int(mt.group(2)), shape)] = (int(mt.group(6)), fn)
+    return out
 
 
 def profile_dominant(run):
@@ -191,22 +214,27 @@ def roofline_for(run, dom, keys, precision, durations):
     real = d[d >= thr]
     avg_ms = float(real.mean())
     role, layer = keys[0]
-    fl = flops_per_launch(role, layer, run.B, precision, run.wgrad_precision)
+    fl = flops_per_launch(role, layer, run.B)
+    npass = passes(role, precision, run.wgrad_precision)
     achieved = fl / (avg_ms * 1e-3) / 1e12
     peak = PEAK[precision]
     M, N, K = dom[1]
     info = {"kernel": f"{ROLE_NAMES[role]}[layers {','.join(str(k[1] + 1) for k in keys)}] "
                       f"{M}x{N}, K={K}",
             "launches": int(d.size), "launches_with_work": int(real.size),
-            "avg_ms": avg_ms, "flop_per_launch": fl}
-    traffic = TRAFFIC.get((precision if not run.wgrad_precision or role != 4
-                           else run.wgrad_precision, role, dom[1]))
-    if traffic is not None:
-        info["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch, "
-                                  "copy-kernel calibrated")
+            "avg_ms": avg_ms, "flop_per_launch": fl, "mfma_passes": npass,
+            "issued_tflops": round(achieved * npass, 2),
+            "issued_frac": round(achieved * npass / peak, 4)}
+    prec_key = run.wgrad_precision if (role == 4 and run.wgrad_precision) else precision
+    tr = measured_traffic().get((prec_key, role, dom[1]))
+    traffic = None
+    if tr is not None:
+        traffic = tr[0]
+        info["traffic_source"] = (f"profiles/{tr[1]}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per "
+                                  "dispatch, copy-kernel calibrated")
     return {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-            "traffic": round(traffic) if traffic is not None else None}, info
+            "traffic": traffic}, info
 
 
 # ---------------------------------------------------------------- ragged leg
@@ -406,6 +434,7 @@ def ffjord_leg(B=8192, reps=3, precision="bf16x3", cpu_batch=8192):
            "precision": precision, "ms": round(1e3 * dt, 2), "nf": st["nf"],
            "steps": f"{st['naccept']}+{st['nreject']}",
            "rhs_evals_per_s": round(st["nf"] / dt, 1), "samples_per_s": round(B / dt, 1),
+           "tflops": round(st["nf"] * fl / dt / 1e12, 1),
            "tflops_issued": round(st["nf"] * fl * (3 if precision == "bf16x3" else 1) / dt / 1e12, 1),
            "mean_logpx": float(np.mean(lp)),
            "train_step_ms": round(1e3 * tbest[0], 2),
@@ -435,6 +464,255 @@ def ffjord_leg(B=8192, reps=3, precision="bf16x3", cpu_batch=8192):
                                  "sample": f"oracle FP32 FFJORD forward + adjoint on {cpu_batch} of "
                                            f"the samples, {sol.nf}+{adj.nf} evals in {cs:.1f} s"}
     return res
+
+
+# ---------------------------------------------------------------- C1 / C2 legs
+SPIRAL_A = np.array([[-0.1, 2.0], [-2.0, -0.1]])
+
+
+def rk4_states(f, u0, ts, substeps=32):
+    """States of u' = f(u) at the times ts by classical RK4 (synthetic data)."""
+    u = np.asarray(u0, np.float64)
+    out = [u.copy()]
+    for a, b in zip(ts[:-1], ts[1:]):
+        h = (b - a) / substeps
+        for _ in range(substeps):
+            k1 = f(u); k2 = f(u + h / 2 * k1); k3 = f(u + h / 2 * k2); k4 = f(u + h * k3)
+            u = u + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        out.append(u.copy())
+    return np.stack(out).astype(np.float32)
+
+
+def glorot(widths, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, o in zip(widths[:-1], widths[1:]):
+        lim = math.sqrt(6.0 / (i + o))
+        out += [rng.uniform(-lim, lim, i * o), np.zeros(o)]
+    return np.concatenate(out).astype(np.float32)
+
+
+class SmallRunner:
+    """forward + MSE cotangent + adjoint of one workload on the GPU (device
+    tensors, FP32 handle)."""
+
+    def __init__(self, widths, acts, imap, B, p, u0, ts, tgt, tol, precision="fp32"):
+        import torch
+        from diffeqflux_hip import Handle
+        self.torch = torch
+        self.h = Handle(widths, acts, imap, precision, B)
+        self.h.set_params(p)
+        self.opts = Handle.make_opts(ts[0], ts[-1], **tol)
+        self.interior = list(ts[1:-1])
+        self.u0 = torch.from_numpy(u0).cuda()
+        self.tgt = torch.from_numpy(tgt).cuda()
+
+    def step(self):
+        out, _, sf = self.h.forward(self.u0, self.opts, self.interior)
+        cot = (2.0 / out.numel()) * (out - self.tgt)
+        self.torch.cuda.current_stream().synchronize()
+        du0, dp, sa = self.h.backward(cot)
+        return sf, sa, du0, dp
+
+    def time(self, reps, warmup=1):
+        for _ in range(warmup):
+            self.step()
+        self.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nf = 0
+        for _ in range(reps):
+            sf, sa, _, _ = self.step()
+            nf += sf["nf"] + sa["nf"]
+        self.torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, nf / reps, sf, sa
+
+
+def spiral_leg(reps=20):
+    """The docs example on its own terms (B = 1): u0 = [2, 0], tspan (0, 1.5),
+    30 saveat, Chain(x -> x.^3, Dense(2, 50, tanh), Dense(50, 2)) with Glorot
+    weights (seed 0), Tsit5 abstol = reltol = 1e-6, MSE against the true
+    spiral's states; forward + adjoint on the GPU (FP32, latency-bound at one
+    trajectory) and the oracle in numpy FP32 on the host (one core: the whole
+    problem is one 2-vector)."""
+    widths, acts = [2, 50, 2], ["tanh", "identity"]
+    p = glorot(widths, 0)
+    u0 = np.array([[2.0, 0.0]], np.float32)
+    ts = np.linspace(0.0, 1.5, 30)
+    tgt = rk4_states(lambda u: (u ** 3) @ SPIRAL_A, u0, ts)
+    tol = dict(abstol=1e-6, reltol=1e-6)
+    r = SmallRunner(widths, acts, 1, 1, p, u0, ts, tgt, tol)
+    sec, nf, sf, sa = r.time(reps, warmup=2)
+    res = {"workload": "spiral docs example, B=1, 30 saveat, Chain(x.^3, Dense(2,50,tanh), "
+                       "Dense(50,2)), Tsit5 abstol=reltol=1e-6, MSE vs the true spiral, FP32",
+           "ms_per_loss_grad": round(1e3 * sec, 3), "rhs_evals_per_s": round(nf / sec, 1),
+           "steps_fwd_adj": f"{sf['naccept']}+{sf['nreject']}/{sa['naccept']}+{sa['nreject']}",
+           "timing": "device-resident u0/targets, host-synchronous entry points"}
+    from oracle import adjoint as OA
+    from oracle.mlp import MLP, MAP_CUBE
+    m = MLP(widths, acts, input_map=MAP_CUBE)
+    _, _, du0, dp = r.step()
+
+    def cpu(dtype):
+        f = OA.forward(m, p.astype(dtype), u0.astype(dtype), 0.0, 1.5, saveat=ts, **tol)
+        pr = np.array(f.u)
+        cot = (2.0 / pr.size) * (pr - tgt.astype(dtype))
+        gu, gp, a = OA.backward(m, p.astype(dtype), f, list(cot), **tol)
+        return f, a, gp
+    cpu(np.float32)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < 3.0 or n < 3:
+        f, a, _ = cpu(np.float32)
+        n += 1
+    cs = (time.perf_counter() - t0) / n
+    gp64 = cpu(np.float64)[2]
+    res["grad_rel_err"] = float(f"{np.linalg.norm(dp.cpu().numpy() - gp64) / np.linalg.norm(gp64):.3e}")
+    res["cpu_baseline"] = {"value": round((f.nf + a.nf) / cs, 1), "unit": "RHS evals/s",
+                           "ms_per_loss_grad": round(1e3 * cs, 2), "cores": 1, "kind": "port",
+                           "sample": f"oracle numpy FP32, the whole problem, mean of {n} runs "
+                                     "after one warm-up"}
+    return res
+
+
+def plain_leg(reps=5, cpu_batch=256):
+    """Batched plain FP32 NeuralODE: D = 32, MLP 32-256-256-32 tanh (Glorot,
+    seed 0), B = 4096 u0 ~ N(0, 1), tspan (0, 1), 20 saveat, Tsit5 rtol 1e-5
+    atol 1e-7, MSE against the saveat states of the seeded linear-plus-tanh
+    teacher (RK4); forward + adjoint on the GPU in FP32, and the oracle in
+    numpy FP32 on a batch sample as the CPU baseline."""
+    D, B = 32, 4096
+    widths, acts = [32, 256, 256, 32], ["tanh", "tanh", "identity"]
+    p = glorot(widths, 0)
+    ts = np.linspace(0.0, 1.0, 20)
+    tol = dict(abstol=1e-7, reltol=1e-5)
+
+    def data(n):
+        u0 = np.random.default_rng(1).standard_normal((n, D)).astype(np.float32)
+        rng = np.random.default_rng(1)
+        A = -np.eye(D) + 0.5 * rng.standard_normal((D, D)) / math.sqrt(D)
+        C = rng.standard_normal((D, D)) / math.sqrt(D)
+        return u0, rk4_states(lambda u: u @ A.T + np.tanh(u @ C.T), u0, ts, substeps=8)
+    u0, tgt = data(B)
+    r = SmallRunner(widths, acts, 0, B, p, u0, ts, tgt, tol)
+    sec, nf, sf, sa = r.time(reps)
+    res = {"workload": "D=32, MLP 32-256-256-32 tanh, B=4096, tspan (0,1), 20 saveat, Tsit5 "
+                       "rtol 1e-5 atol 1e-7, MSE vs teacher ODE, FP32 MFMA",
+           "rhs_evals_per_s": round(nf / sec, 1), "traj_per_s": round(B / sec, 1),
+           "ms_per_step": round(1e3 * sec, 2),
+           "steps_fwd_adj": f"{sf['naccept']}+{sf['nreject']}/{sa['naccept']}+{sa['nreject']}",
+           "timing": "inputs resident in HBM"}
+    from oracle import adjoint as OA
+    from oracle.mlp import MLP
+    m = MLP(widths, acts)
+    us, tgs = data(cpu_batch)
+    rs = SmallRunner(widths, acts, 0, cpu_batch, p, us, ts, tgs, tol)
+    _, _, du0, dp = rs.step()
+
+    def cpu(dtype):
+        f = OA.forward(m, p.astype(dtype), us.astype(dtype), 0.0, 1.0, saveat=ts, **tol)
+        pr = np.array(f.u)
+        cot = (2.0 / pr.size) * (pr - tgs.astype(dtype))
+        gu, gp, a = OA.backward(m, p.astype(dtype), f, list(cot), **tol)
+        return f, a, gp
+    t0 = time.perf_counter()
+    f, a, _ = cpu(np.float32)
+    cs = time.perf_counter() - t0
+    gp64 = cpu(np.float64)[2]
+    res["grad_rel_err"] = float(f"{np.linalg.norm(dp.cpu().numpy() - gp64) / np.linalg.norm(gp64):.3e}")
+    res["grad_rel_err_sample"] = f"B={cpu_batch} sample vs FP64 oracle"
+    res["cpu_baseline"] = {"value": round((f.nf + a.nf) / cs * cpu_batch / B, 2),
+                           "unit": "RHS evals/s (full-batch equivalent)", "cores": blas_cores(),
+                           "kind": "port",
+                           "sample": f"one FP32 forward+adjoint solve of the oracle at B={cpu_batch}, "
+                                     f"{f.nf + a.nf} evals in {cs:.1f} s, rate scaled by "
+                                     f"{cpu_batch}/{B}"}
+    return res
+
+
+# ---------------------------------------------------------------- C3 finite differences
+FD_B, FD_DIRS, FD_TOL = 8, 64, dict(abstol=1e-12, reltol=1e-9)
+
+
+def fd_direction(i, P):
+    """Unit direction i of the finite-difference check: N(0, I) / norm, seeded by (2, i)."""
+    v = np.random.default_rng([2, i]).standard_normal(P)
+    return v / np.linalg.norm(v)
+
+
+_FD = {}
+
+
+def _fd_init():
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except Exception:
+        pass
+    from oracle.mlp import MLP
+    u0, ts, tgt = make_inputs(FD_B)
+    _FD.update(m=MLP(WIDTHS, ACTS), p=glorot_params().astype(np.float64),
+               u0=u0.astype(np.float64), ts=ts, tgt=tgt.astype(np.float64))
+
+
+def _fd_loss(task):
+    """MSE loss of the FP64 oracle solve at p + s * eps * v_i (tight tolerance)."""
+    from oracle import adjoint as OA
+    i, s, eps = task
+    m, p = _FD["m"], _FD["p"]
+    pp = p + (s * eps) * fd_direction(i, p.size) if s else p
+    f = OA.forward(m, pp, _FD["u0"], TSPAN[0], TSPAN[1], saveat=_FD["ts"], **FD_TOL)
+    return float(np.mean((np.array(f.u) - _FD["tgt"]) ** 2))
+
+
+def fd_leg(gpu_grad):
+    """C3's gradient against central finite differences of the FP64 oracle
+    along FD_DIRS random unit directions of parameter space, on a batch
+    sample (B = FD_B of the same u0 / targets): the FP64 loss is solved at a
+    tight tolerance (rtol 1e-9) so that its finite differences follow the
+    exact ODE's loss; eps is picked by a sweep on direction 0 (the value whose
+    central difference changes least when eps is halved).  Reported: the GPU
+    gradient's directional derivatives (its own tolerance, rtol 1e-4) against
+    the differences, and the FP64 oracle adjoint at the tight tolerance
+    against the same differences (which pins them)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from oracle import adjoint as OA
+    from oracle.mlp import MLP
+    P = int(gpu_grad.size)
+    nproc = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    # fresh interpreters (this process holds a GPU context); a worker that
+    # dies raises here instead of being replaced forever
+    with ProcessPoolExecutor(nproc, mp_context=mp.get_context("spawn"),
+                             initializer=_fd_init) as pool:
+        sweep = [1e-2, 5e-3, 1e-3, 5e-4, 1e-4, 5e-5]
+        lv = list(pool.map(_fd_loss, [(0, s, e) for e in sweep for s in (1, -1)]))
+        fd0 = [(lv[2 * k] - lv[2 * k + 1]) / (2 * e) for k, e in enumerate(sweep)]
+        k = int(np.argmin([abs(fd0[k] - fd0[k + 1]) for k in range(len(sweep) - 1)]))
+        eps = sweep[k]
+        lv = list(pool.map(_fd_loss, [(i, s, eps) for i in range(FD_DIRS) for s in (1, -1)]))
+    fd = np.array([(lv[2 * i] - lv[2 * i + 1]) / (2 * eps) for i in range(FD_DIRS)])
+    t_fd = time.perf_counter() - t0
+    # FP64 oracle adjoint at the tight tolerance on the same sample
+    m = MLP(WIDTHS, ACTS)
+    u0, ts, tgt = make_inputs(FD_B)
+    p64 = glorot_params().astype(np.float64)
+    f = OA.forward(m, p64, u0.astype(np.float64), TSPAN[0], TSPAN[1], saveat=ts, **FD_TOL)
+    pr = np.array(f.u)
+    cot = (2.0 / pr.size) * (pr - tgt.astype(np.float64))
+    _, g64, _ = OA.backward(m, p64, f, list(cot), **FD_TOL)
+    V = [fd_direction(i, P) for i in range(FD_DIRS)]
+    dg = np.array([float(np.dot(gpu_grad, v)) for v in V])
+    do = np.array([float(np.dot(g64, v)) for v in V])
+    scale = float(np.abs(fd).max())
+    return {"sample": f"B={FD_B} of the headline inputs, {FD_DIRS} unit directions "
+                      f"N(0,I)/norm (seeds (2,i)), FP64 oracle loss at rtol 1e-9 atol 1e-12",
+            "eps": eps, "eps_sweep": {f"{e:g}": float(f"{v:.6e}") for e, v in zip(sweep, fd0)},
+            "gpu_max_rel_err": float(f"{np.abs(dg - fd).max() / scale:.3e}"),
+            "gpu_median_rel_err": float(f"{np.median(np.abs(dg - fd) / np.abs(fd)):.3e}"),
+            "gpu_cosine": float(f"{np.dot(dg, fd) / np.linalg.norm(dg) / np.linalg.norm(fd):.8f}"),
+            "oracle_adjoint_max_rel_err": float(f"{np.abs(do - fd).max() / scale:.3e}"),
+            "cpu_s": round(t_fd, 1), "processes": nproc}
 
 
 # ---------------------------------------------------------------- CPU leg
@@ -503,6 +781,11 @@ def main():
     ap.add_argument("--no-ffjord", action="store_true", help="skip the FFJORD workload leg")
     ap.add_argument("--no-multiple-shoot", action="store_true",
                     help="skip the multiple-shooting workload leg")
+    ap.add_argument("--no-spiral", action="store_true", help="skip the spiral (B=1) leg")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP32 D=32 B=4096 workload leg")
+    ap.add_argument("--no-fd", action="store_true",
+                    help="skip the finite-difference check of the headline gradient")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the NeuralODE library runs on one GPU only (no multi-GPU path); "
@@ -527,6 +810,8 @@ def main():
     # timed region: step attempts replayed from captured HIP graphs, no hooks
     dt, nf, nfa, last = run.timed(args.steps, args.warmup)
     value = (nf + nfa) / dt
+    S = sum(i * o for i, o in zip(WIDTHS[:-1], WIDTHS[1:]))
+    solve_flops = 2.0 * args.batch * S * nf + 6.0 * args.batch * S * nfa
     # roofline pass: the same steps again with HIP events around every launch
     # of the dominant GEMM (event hooks launch kernels one by one)
     dom, keys, tot = profile_dominant(run)
@@ -584,6 +869,18 @@ def main():
         extra["grad_rel_err"] = extra[f"grad_rel_err_{head}"]
         log(f"[bench] cpu baseline {cpu}; grad rel err {errs}")
 
+    if not args.no_fd:
+        rf = Runner(args.precision, FD_B, p, *make_inputs(FD_B), wgp)
+        _, _, _, dpf = rf.step()
+        extra["fd64"] = fd_leg(dpf.cpu().numpy())
+        log(f"[bench] fd64: {extra['fd64']}")
+        del rf
+    if not args.no_spiral:
+        extra["spiral"] = spiral_leg()
+        log(f"[bench] spiral: {extra['spiral']}")
+    if not args.no_plain:
+        extra["plain"] = plain_leg()
+        log(f"[bench] plain: {extra['plain']}")
     if not args.no_ragged:
         extra["ragged"] = ragged_leg()
         log(f"[bench] ragged: {extra['ragged']}")
@@ -611,6 +908,10 @@ def main():
                    "batch": args.batch, "precision": args.precision,
                    "wgrad_precision": wgp or args.precision},
         "traj_per_s": round(args.batch * args.steps / dt, 2),
+        # whole solve: algorithmic FLOPs (forward RHS 2BS, adjoint VJP 6BS,
+        # S = sum in*out) per second of the timed steps, against the peak
+        "solve_tflops": round(solve_flops / dt / 1e12, 2),
+        "solve_mfma_frac": round(solve_flops / dt / 1e12 / PEAK[args.precision], 4),
         "nf_fwd": sf["nf"], "nf_adj": sa["nf"],
         "steps_fwd": f"{sf['naccept']}+{sf['nreject']}",
         "steps_adj": f"{sa['naccept']}+{sa['nreject']}",

@@ -100,6 +100,15 @@ __device__ __forceinline__ bool gate_ok(const Ctl* c, int gate) {
   return true;
 }
 
+// Tape entry of forward step n: the entries form a ring of c->tape_ring slots
+// when that is set (a forward solve that keeps only checkpoints: the stage
+// kernels touch entries n and n + 1, the pending saves n - 1 and n), else the
+// record of every step.
+__device__ __forceinline__ int64_t tape_slot(const Ctl* c, int64_t n) {
+  const int64_t r = c->tape_ring;
+  return r > 0 ? n % r : n;
+}
+
 // Activations from the hardware exp2 / log2 / reciprocal (v_exp_f32,
 // v_log_f32, v_rcp_f32, ~1 ulp each): the epilogues run them once per output
 // element at low occupancy, where the full-range library forms dominated the
@@ -589,7 +598,7 @@ __device__ __forceinline__ void epi_apply(const EpiTanBwd& e, f32x16 (&acc)[MR][
           else if (e.act == 4) sq = t * (1.f - 2.f * y);
           const int v = q * 4 + p;
           const float tb = acc[mi][mj][v];
-          e.src[o] = cv ? tb * sq : 0.f;
+          e.src[o] = (e.src_acc ? e.src[o] : 0.f) + (cv ? tb * sq : 0.f);
           acc[mi][mj][v] = cv ? tb * d : 0.f;
         }
       }
@@ -605,7 +614,8 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
   const int rb = geo.rb, cb = geo.cb;
   using T = typename PP::T;
   const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
-  float* ent = e.tape + n * e.ent;
+  float* ent = e.tape + tape_slot(ctl, n) * e.ent;
+  float* nxt = e.tape + tape_slot(ctl, n + 1) * e.ent;  // entry n + 1
   const float dt = (float)ctl->dt;
   const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
   const int s = e.stage;
@@ -635,7 +645,7 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
 #pragma unroll
             for (int j = 0; j < 6; ++j) ut += cBT[j] * ent[(1 + j) * e.sv + idx];
             ut *= dt;
-            const float up = ent[idx], un = ent[e.ent + idx];
+            const float up = ent[idx], un = nxt[idx];
             const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
             if (r < e.B) {
               const double a = (double)(ut / sk);
@@ -658,9 +668,9 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
           if (s < 6) {
             const float u = uv[mi][mj][v];
             put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, map_f(e.map, u));
-            if (s == 5) ent[e.ent + idx] = u;            // candidate tape[n+1].uprev
+            if (s == 5) nxt[idx] = u;            // candidate tape[n+1].uprev
           } else {
-            ent[e.ent + e.sv + idx] = k;                  // FSAL: tape[n+1].k1
+            nxt[e.sv + idx] = k;                  // FSAL: tape[n+1].k1
           }
         }
       }
@@ -686,14 +696,14 @@ __device__ __forceinline__ void epi_apply(const EpiFwdStage& e, f32x16 (&acc)[MR
               if (j < s) sm += cA[s + 1][j] * ent[(1 + j) * e.sv + idx];
             const float u = ent[idx] + dt * sm;
             put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, map_f(e.map, u));
-            if (s == 5) ent[e.ent + idx] = u;            // candidate tape[n+1].uprev
+            if (s == 5) nxt[idx] = u;            // candidate tape[n+1].uprev
           } else {
-            ent[e.ent + e.sv + idx] = k;                  // FSAL: tape[n+1].k1
+            nxt[e.sv + idx] = k;                  // FSAL: tape[n+1].k1
             float ut = cBT[6] * k;
 #pragma unroll
             for (int j = 0; j < 6; ++j) ut += cBT[j] * ent[(1 + j) * e.sv + idx];
             ut *= dt;
-            const float up = ent[idx], un = ent[e.ent + idx];
+            const float up = ent[idx], un = nxt[idx];
             const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
             if (live) {
               const double a = (double)(ut / sk);
@@ -906,31 +916,97 @@ __device__ __forceinline__ void epi_apply(const EpiCnfStage& e, f32x16 (&acc)[MR
   const int rb = geo.rb, cb = geo.cb;
   using T = typename PP::T;
   const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
-  float* ent = e.tape + n * e.ent;
+  float* ent = e.tape + tape_slot(ctl, n) * e.ent;
+  float* nxt = e.tape + tape_slot(ctl, n + 1) * e.ent;  // entry n + 1
   const float dt = (float)ctl->dt;
   const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
   const int s = e.stage;
   T* X0 = (T*)e.X0;
-  // per-sample trace e^T (df/dz) e: products staged in LDS, summed per row
-  // (one tile spans every column of a row: the plan keeps BC >= the width)
+  // The GEMM gives w = e^T df/dx per sample (x the model input, w's first dm
+  // columns the z part).  Per-row sums go through LDS (one tile spans every
+  // column of a row: the plan keeps BC >= the width), one sum at a time.
   float* pl = (float*)lds;
-  float* tr = pl + BR * (BC + 1);
+  float* tr = pl + BR * (BC + 1);   // -> trace per row
+  float* q1 = tr + BR;              // -> |f|^2 per row (regularize)
+  float* q2 = q1 + BR;              // -> |w|^2 per row (regularize)
+  const bool last = e.pass + 1 >= e.npass;
+  if (e.npass > 1 && !last) {
+    // exact trace, not the last unit probe: accumulate J[pass][pass] and
+    // e[pass] * J[pass][:] (the regulariser's e^T J); no stage
+    EPI_LOOP_BEGIN
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p, v = q * 4 + p;
+        if (c < e.dm) {
+          const float w = acc[mi][mj][v];
+          if (c == e.pass) e.trc[r] += w;
+          if (e.reg) e.wacc[(int64_t)r * e.dm + c] += e.e[(int64_t)r * e.dm + e.pass] * w;
+        }
+      }
+    EPI_LOOP_END
+    return;
+  }
   __syncthreads();
+  // w_eff: this pass's w (Hutchinson), or the accumulated e^T J (exact)
   EPI_LOOP_BEGIN
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       const int r = r0 + p, v = q * 4 + p;
-      const float pr = c < e.dm ? acc[mi][mj][v] * e.e[(int64_t)r * e.dm + c] : 0.f;
-      pl[(r - geo.row0) * (BC + 1) + (c - geo.col0)] = pr;
+      float w = c < e.dm ? acc[mi][mj][v] : 0.f;
+      if (e.npass > 1 && c < e.dm) {
+        if (c == e.pass) { tr[r - geo.row0] = e.trc[r] + w; e.trc[r] = 0.f; }
+        if (e.reg) {
+          const int64_t o = (int64_t)r * e.dm + c;
+          w = e.wacc[o] + e.e[(int64_t)r * e.dm + e.pass] * w;
+          e.wacc[o] = 0.f;
+        }
+      }
+      acc[mi][mj][v] = w;
+      if (e.npass == 1)
+        pl[(r - geo.row0) * (BC + 1) + (c - geo.col0)] = c < e.dm ? w * e.e[(int64_t)r * e.dm + c] : 0.f;
     }
   EPI_LOOP_END
   __syncthreads();
-  for (int i = threadIdx.x; i < BR; i += NT) {
-    float sm = 0.f;
-    for (int j = 0; j < BC; ++j) sm += pl[i * (BC + 1) + j];
-    tr[i] = sm;
+  if (e.npass == 1) {
+    for (int i = threadIdx.x; i < BR; i += NT) {
+      float sm = 0.f;
+      for (int j = 0; j < BC; ++j) sm += pl[i * (BC + 1) + j];
+      tr[i] = sm;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  if (e.reg) {
+    EPI_LOOP_BEGIN
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p, v = q * 4 + p;
+        const float fv = c < e.dm ? e.f[(int64_t)r * e.dm + c] : 0.f;
+        pl[(r - geo.row0) * (BC + 1) + (c - geo.col0)] = fv * fv;
+      }
+    EPI_LOOP_END
+    __syncthreads();
+    for (int i = threadIdx.x; i < BR; i += NT) {
+      float sm = 0.f;
+      for (int j = 0; j < BC; ++j) sm += pl[i * (BC + 1) + j];
+      q1[i] = sm;
+    }
+    __syncthreads();
+    EPI_LOOP_BEGIN
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + p, v = q * 4 + p;
+        const float w = acc[mi][mj][v];
+        pl[(r - geo.row0) * (BC + 1) + (c - geo.col0)] = w * w;
+      }
+    EPI_LOOP_END
+    __syncthreads();
+    for (int i = threadIdx.x; i < BR; i += NT) {
+      float sm = 0.f;
+      for (int j = 0; j < BC; ++j) sm += pl[i * (BC + 1) + j];
+      q2[i] = sm;
+    }
+    __syncthreads();
+  }
   // stage values k = [f; -trace] (pass 1: loads / arithmetic, pass 2: stores)
   double sum = 0.0, flag = 0.0;
   f32x16 uv[MR][MC];
@@ -942,7 +1018,9 @@ __device__ __forceinline__ void epi_apply(const EpiCnfStage& e, f32x16 (&acc)[MR
       acc[mi][mj][v] = 0.f;
       if (c < e.D) {
         const int64_t idx = (int64_t)r * e.D + c;
-        const float k = c < e.dm ? e.f[(int64_t)r * e.dm + c] : -tr[r - geo.row0];
+        const int rl = r - geo.row0;
+        const float k = c < e.dm ? e.f[(int64_t)r * e.dm + c]
+                        : c == e.dm ? -tr[rl] : c == e.dm + 1 ? q1[rl] : sqrtf(q2[rl]);
         acc[mi][mj][v] = k;
         const bool live = r < e.B;
         if (e.mode == SM_STEP) {
@@ -957,7 +1035,7 @@ __device__ __forceinline__ void epi_apply(const EpiCnfStage& e, f32x16 (&acc)[MR
 #pragma unroll
             for (int j = 0; j < 6; ++j) ut += cBT[j] * ent[(1 + j) * e.sv + idx];
             ut *= dt;
-            const float up = ent[idx], un = ent[e.ent + idx];
+            const float up = ent[idx], un = nxt[idx];
             const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
             if (live) {
               const double a = (double)(ut / sk);
@@ -993,10 +1071,11 @@ __device__ __forceinline__ void epi_apply(const EpiCnfStage& e, f32x16 (&acc)[MR
           if (s < 6) {
             const float u = uv[mi][mj][v];
             if (c < e.dm) put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, u);
-            else if (e.tcat && r < e.B) put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, (float)tnext);
-            if (s == 5) ent[e.ent + idx] = u;            // candidate tape[n+1].uprev
+            else if (e.tcat && c == e.dm && r < e.B)
+              put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, (float)tnext);
+            if (s == 5) nxt[idx] = u;            // candidate tape[n+1].uprev
           } else {
-            ent[e.ent + e.sv + idx] = k;                  // FSAL: tape[n+1].k1
+            nxt[e.sv + idx] = k;                  // FSAL: tape[n+1].k1
           }
         } else if (e.mode == SM_INIT_F0) {
           ent[e.sv + idx] = k;
@@ -1369,7 +1448,8 @@ __global__ void __launch_bounds__(ST_THREADS)
   using T = typename PP::T;
   const int tid = threadIdx.x, cl = tid & 63, rq = tid >> 6;
   const int64_t n = (e.mode == SM_STEP) ? ctl->n : 0;
-  float* ent = e.tape + n * e.ent;
+  float* ent = e.tape + tape_slot(ctl, n) * e.ent;
+  float* nxt = e.tape + tape_slot(ctl, n + 1) * e.ent;  // entry n + 1
   const float dt = (float)ctl->dt;
   const float atol = (float)ctl->abstol, rtol = (float)ctl->reltol;
   const int s = e.stage;
@@ -1400,7 +1480,7 @@ __global__ void __launch_bounds__(ST_THREADS)
 #pragma unroll
           for (int i = 0; i < 6; ++i) ut += cBT[i] * ent[(1 + i) * e.sv + idx];
           ut *= dt;
-          const float up = ent[idx], un = ent[e.ent + idx];
+          const float up = ent[idx], un = nxt[idx];
           const float sk = atol + fmaxf(fabsf(up), fabsf(un)) * rtol;
           if (live) {
             const double a = (double)(ut / sk);
@@ -1430,9 +1510,9 @@ __global__ void __launch_bounds__(ST_THREADS)
         ent[(1 + s) * e.sv + idx] = k;
         if (s < 6) {
           put<PP>(X0 + (int64_t)r * e.ldx + c, e.px, map_f(e.map, uv[j]));
-          if (s == 5) ent[e.ent + idx] = uv[j];           // candidate tape[n+1].uprev
+          if (s == 5) nxt[idx] = uv[j];           // candidate tape[n+1].uprev
         } else {
-          ent[e.ent + e.sv + idx] = k;                     // FSAL: tape[n+1].k1
+          nxt[e.sv + idx] = k;                     // FSAL: tape[n+1].k1
         }
       } else if (e.mode == SM_INIT_F0) {
         ent[e.sv + idx] = k;
@@ -1711,8 +1791,8 @@ __global__ void k_fwd_step_begin(StateGeom sg, const float* __restrict__ tape,
   const int64_t lo = ctl->save_lo, hi = ctl->save_hi;
   const double t = ctl->t;
   const int64_t cur = ctl->n;
-  const float* ecur = tape + cur * sg.ent;
-  const float* es = tape + ctl->save_step * sg.ent;
+  const float* ecur = tape + tape_slot(ctl, cur) * sg.ent;
+  const float* es = tape + tape_slot(ctl, ctl->save_step) * sg.ent;
   const double tprev = ctl->tprev, dts = ctl->dt_step;
   const int64_t nreal = (int64_t)sg.B * sg.D;
   for (int64_t si = lo; si < hi; ++si) {
@@ -1947,7 +2027,7 @@ __global__ void k_fwd_controller(Ctl* c, const double* part, int np, const doubl
     if (!c->adaptive) c->dtpropose = c->dt;
     c->save_step = c->n;
     c->n += 1;
-    ft[c->n] = c->t;
+    ft[tape_slot(c, c->n)] = c->t;
     int64_t k = c->save_next;
     c->save_lo = k;
     while (k < c->n_save && c->tdir * save_times[k] <= c->tdir * c->t) ++k;
@@ -2179,7 +2259,41 @@ __global__ void k_adj_controller(Ctl* c, const double* pl, int nl, const double*
   }
   tstop = c->save_next >= 0 ? cbt[c->save_next] : c->t_end;
   loopheader(c, tstop);
+  if (acc && c->has_pause && c->t == c->pause_t && !c->done) {
+    // checkpoint reached: the host re-solves the forward interval below it
+    // and resumes (k_adj_resume sets the stage interpolation data)
+    c->paused = 1;
+    c->done = 1;
+    return;
+  }
   for (int s = 0; s < 7; ++s) set_interp(c, s, c->t + dC[s] * c->dt, ft);
+}
+
+// Resume a paused adjoint on the forward interval just re-solved (n_fwd
+// steps on the tape): the control block saved at the pause, the next
+// checkpoint to pause at, and the forward-interpolation data of the next
+// attempt's stages.  Stage 0 (the FSAL re-evaluation at the checkpoint time,
+// where the cotangent jump has just been applied) reads the checkpoint state
+// itself, which the host has placed in tape entry n_fwd + 1: at the
+// checkpoint the adjoint is still on the interval above it, whose solution
+// starts from that state.
+__global__ void k_adj_resume(Ctl* c, const Ctl* saved, int64_t n_fwd, double pause_t,
+                             int has_pause, const double* ft) {
+  if (threadIdx.x != 0) return;
+  *c = *saved;
+  c->done = 0;
+  c->paused = 0;
+  c->n_fwd = n_fwd;
+  c->pause_t = pause_t;
+  c->has_pause = has_pause;
+  for (int s = 0; s < 7; ++s) set_interp(c, s, c->t + dC[s] * c->dt, ft);
+  c->st_step[0] = (int)(n_fwd + 1);
+  c->st_exact[0] = 1;
+}
+void launch_adj_resume(Ctl* ctl, const Ctl* saved, int64_t n_fwd, double pause_t, int has_pause,
+                       const double* fwd_times, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_resume, dim3(1), dim3(64), 0, s, ctl, saved, n_fwd, pause_t, has_pause,
+                     fwd_times);
 }
 void launch_adj_controller(Ctl* ctl, const double* part, int n_lam, const double* part_mu,
                            int n_mu, double n_elems, const double* cb_times,
@@ -2399,7 +2513,7 @@ __global__ void k_prep_lambda(StateGeom sg, const float* lam, typename PP::T* L,
     const int r = (int)(i / sg.D), c = (int)(i % sg.D);
     const float v = r < sg.B ? lam[i] : 0.f;
     put<PP>(L + (int64_t)r * ldl + c, pl, v);
-    put<PP>(LT + (int64_t)c * ldlt + r, plt, v);
+    if (LT) put<PP>(LT + (int64_t)c * ldlt + r, plt, v);
   }
 }
 void launch_prep_lambda(const StateGeom& sg, const float* lam, void* L, void* LT, int ldl,

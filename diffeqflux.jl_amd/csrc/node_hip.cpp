@@ -67,6 +67,17 @@ struct node_handle {
   float* probe = nullptr;  // CNF probe e, FP32 [Bp][dm]
   float* fbuf = nullptr;   // CNF model output f, FP32 [Bp][dm]
   std::vector<float> probe_h;  // host copy of the last CNF forward's probe [B][dm]
+  // FFJORD variants (input_map flags): regularize adds the state rows r1, r2
+  // (r1' = |f|^2, r2' = |e^T df/dz|); the exact trace runs one unit-probe pass
+  // per state dimension
+  bool cnf_reg = false, cnf_exact = false;
+  int cnf_dm = 0;              // z dimension (model output width)
+  int npr = 1;                 // probe passes per right-hand side / tangent chains per VJP
+  std::vector<void*> LmS;      // exact trace: unit-probe seeds at the last layer (forward)
+  std::vector<void*> E0s, AbLs;  // tangent seeds [e_k; 0] and -lambda_dlogp e_k (adjoint)
+  float* trc = nullptr;        // exact trace accumulators: trace [Bp], e^T J [Bp][dm]
+  float* wacc = nullptr;
+  std::vector<float> cnf_lam;  // r1(T), r2(T) of the last forward, [2][B]
   // CNF adjoint (second-order VJP of the Hutchinson term): tangent t_l and
   // its cotangent row-major per layer (operand format), the FP32 second-order
   // sources, and [e; 0] at layer 0's input width
@@ -114,6 +125,22 @@ struct node_handle {
   float* dLdout_dev = nullptr;
   int64_t dLdout_cap = 0;
   float* dp32 = nullptr;
+
+  // InterpolatingAdjoint(checkpointing = true): the forward keeps a ring of
+  // tape entries plus the checkpoint states (u0, the saved states, u(t1));
+  // the adjoint re-solves each interval between consecutive checkpoints into
+  // the tape before integrating over it
+  bool ckpt = false;          // NODE_OPT_CHECKPOINTING
+  bool fwd_ring = false;      // the last forward kept checkpoints only
+  int64_t tape_budget = 0;    // NODE_OPT_TAPE_BUDGET: bytes of tape (0: unlimited)
+  float* ck_u0 = nullptr;     // u(t0), FP32 [B][D]
+  float* ck_uT = nullptr;     // u(t1), FP32 [B][D]
+  Ctl* ctl_save = nullptr;    // the paused adjoint's control block
+  std::vector<double> ck_t;   // checkpoint times t0 < ... < t1
+  std::vector<int64_t> ck_src;  // state of each checkpoint: saved index, -1 u0, -2 u(t1)
+  int64_t ck_n = 0;           // steps of the interval on the tape
+  double ck_last_dt = 0.0;    // last step of the previous re-solve (next one's initial dt)
+  int64_t ck_nf = 0;          // RHS evaluations of the re-solves of the last backward
 
   bool have_params = false, have_forward = false;
   node_solve_opts fwd_opts{};
@@ -284,7 +311,7 @@ StateGeom geom(const node_handle* h) {
   g.map = h->desc.input_map; g.prec = h->prec;
   g.sv = (int64_t)h->Bp * h->D;
   g.ent = 8 * g.sv;
-  g.dm = h->cnf ? h->D - 1 : h->D;
+  g.dm = h->cnf ? h->cnf_dm : h->D;
   g.tcat = h->desc.input_map == NODE_MAP_CNF_TIME ? 1 : 0;
   g.x0w = h->layers[0].inp;
   return g;
@@ -297,12 +324,16 @@ int validate_desc(const node_desc* d, std::string& msg) {
     if (d->widths[i] < 1 || d->widths[i] > 65536) { msg = "widths must be 1..65536"; return NODE_ERR_INVALID; }
   for (int i = 0; i < d->n_layers; ++i)
     if (d->acts[i] < 0 || d->acts[i] > 4) { msg = "unknown activation"; return NODE_ERR_INVALID; }
-  if (d->input_map == NODE_MAP_CNF || d->input_map == NODE_MAP_CNF_TIME) {
+  const int base_map = d->input_map & ~(NODE_MAP_CNF_REGULARIZE | NODE_MAP_CNF_EXACT_TRACE);
+  if (base_map == NODE_MAP_CNF || base_map == NODE_MAP_CNF_TIME) {
     const int dm = d->widths[d->n_layers];
-    if (d->widths[0] != dm + (d->input_map == NODE_MAP_CNF_TIME ? 1 : 0)) { msg = "CNF model: input width must be D (or D + 1 with time)"; return NODE_ERR_INVALID; }
+    const int extra = (d->input_map & NODE_MAP_CNF_REGULARIZE) ? 3 : 1;
+    if (d->widths[0] != dm + (base_map == NODE_MAP_CNF_TIME ? 1 : 0)) { msg = "CNF model: input width must be D (or D + 1 with time)"; return NODE_ERR_INVALID; }
     if (d->n_layers < 2) { msg = "CNF model needs at least 2 Dense layers"; return NODE_ERR_INVALID; }
     if (d->acts[d->n_layers - 1] != NODE_ACT_IDENTITY) { msg = "CNF model: the last layer must be linear"; return NODE_ERR_INVALID; }
-    if (dm + 1 > 64) { msg = "CNF state dimension D + 1 must be <= 64"; return NODE_ERR_INVALID; }
+    if (dm + extra > 64) { msg = "CNF state dimension (D + 1, D + 3 with regularize) must be <= 64"; return NODE_ERR_INVALID; }
+  } else if (base_map != d->input_map) {
+    msg = "NODE_MAP_CNF_REGULARIZE / NODE_MAP_CNF_EXACT_TRACE apply to CNF maps only"; return NODE_ERR_INVALID;
   } else {
     if (d->widths[0] != d->widths[d->n_layers]) { msg = "output width must equal input width (ODE state dimension)"; return NODE_ERR_INVALID; }
     if (d->input_map != NODE_MAP_IDENTITY && d->input_map != NODE_MAP_CUBE) { msg = "unknown input_map"; return NODE_ERR_INVALID; }
@@ -444,21 +475,24 @@ void launch_bwd_data(node_handle* h, int l, int gate, bool bias_sums = true) {
   }
 }
 
-// pointer to the second (tangent) half of a CNF handle's transposed operand
-void* tan_half(const node_handle* h, void* T) { return (char*)T + (size_t)h->Bp * h->esz; }
+// pointer to tangent slice k of a CNF handle's transposed operand (columns
+// (1 + k) Bp ..: the primal rows come first, then one slice per probe chain)
+void* tan_half(const node_handle* h, void* T, int k = 0) {
+  return (char*)T + (size_t)(1 + k) * h->Bp * h->esz;
+}
 
 // CNF adjoint, tangent forward of hidden layer l: t_l = act'_l * (t_{l-1} W_l^T)
 // with t_{-1} = [e; 0], row-major and transposed (second half of H_l^T)
-void launch_tan_fwd(node_handle* h, int l, int gate) {
+void launch_tan_fwd(node_handle* h, int l, int gate, int k = 0) {
   const Layer& ly = h->layers[l];
   GemmArgs g{};
-  g.A = l == 0 ? h->E0 : h->Tn[l - 1];
+  g.A = l == 0 ? h->E0s[k] : h->Tn[l - 1];
   g.lda = ly.inp; g.B = ly.Wr; g.ldb = ly.inp; g.K = ly.inp;
   g.pa = (int64_t)h->Bp * ly.inp; g.pb = (int64_t)ly.outp * ly.inp; g.ksplit = 1;
   GemmPlan p = plan_role(h, PR_ADJ_HIDDEN, l, h->Bp, ly.outp, 0);
   g.row_tiles = p.rt; g.col_tiles = p.ct;
   const int64_t ph = (int64_t)h->Bp * ly.outp;
-  EpiBwdData e{ly.out, ly.outp, h->ldT, h->S[l], h->Tn[l], tan_half(h, h->HT[l]), ph,
+  EpiBwdData e{ly.out, ly.outp, h->ldT, h->S[l], h->Tn[l], tan_half(h, h->HT[l], k), ph,
                (int64_t)h->ldT * ly.outp, nullptr, h->H[l], ph,
                act_from_output(ly.act) ? ly.act : -1};
   ProfScope ps(h, PR_ADJ_HIDDEN, l);
@@ -467,11 +501,11 @@ void launch_tan_fwd(node_handle* h, int l, int gate) {
 
 // CNF adjoint, tangent backward through layer l >= 1 into layer l - 1:
 // abar_{l-1} = (abar_l W_l) act'_{l-1} (+ its second-order source)
-void launch_tan_bwd(node_handle* h, int l, int gate) {
+void launch_tan_bwd(node_handle* h, int l, int gate, int k = 0) {
   const Layer& ly = h->layers[l];
   const Layer& lo = h->layers[l - 1];
   GemmArgs g{};
-  g.A = h->Ab[l];
+  g.A = l == h->L - 1 ? h->AbLs[k] : h->Ab[l];
   g.lda = ly.outp; g.B = ly.Wc; g.ldb = ly.outp; g.K = ly.outp;
   g.pa = (int64_t)h->Bp * ly.outp; g.pb = (int64_t)ly.inp * ly.outp; g.ksplit = 1;
   GemmPlan p = plan_role(h, PR_BWD_DATA, l, h->Bp, ly.inp, 0);
@@ -482,9 +516,10 @@ void launch_tan_bwd(node_handle* h, int l, int gate) {
   e.Y = h->H[l - 1]; e.S = h->S[l - 1]; e.Tf = h->Tn[l - 1]; e.py = e.ptf = pd;
   // (abar_0 row-major feeds no further tangent GEMM: only its transposed copy)
   e.Ab = l - 1 >= 1 ? h->Ab[l - 1] : nullptr;
-  e.AbT = tan_half(h, h->DlT[l - 1]);
+  e.AbT = tan_half(h, h->DlT[l - 1], k);
   e.pab = pd; e.pabt = (int64_t)h->ldT * ly.inp;
   e.src = h->src2[l - 1];
+  e.src_acc = k > 0;  // the second-order sources of all probe chains add up
   ProfScope ps(h, PR_BWD_DATA, l);
   launch_gemm_tanbwd(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
 }
@@ -600,9 +635,11 @@ void vjp_eval(node_handle* h, int stage, int mode, int gate, int wflags, float c
               float cb1, float ce1, double* part, float* plain_out, float* gptr) {
   const int L = h->L;
   for (int l = 0; l < L - 1; ++l) launch_hidden(h, l, gate, true);
-  if (h->cnf_adj) {  // tangent chain of the Hutchinson term, forward then reverse
-    for (int l = 0; l < L - 1; ++l) launch_tan_fwd(h, l, gate);
-    for (int l = L - 1; l >= 1; --l) launch_tan_bwd(h, l, gate);
+  if (h->cnf_adj) {  // tangent chains of the trace term (one per probe), forward then reverse
+    for (int k = 0; k < h->npr; ++k) {
+      for (int l = 0; l < L - 1; ++l) launch_tan_fwd(h, l, gate, k);
+      for (int l = L - 1; l >= 1; --l) launch_tan_bwd(h, l, gate, k);
+    }
   }
   // BacksolveAdjoint: f(u_s) of this stage (the last layer on the recomputed
   // hidden state) for u's own Runge-Kutta stages
@@ -624,6 +661,17 @@ int ensure_tape(node_handle* h, int64_t entries) {
   if (entries <= h->tape_cap) return 0;
   int64_t cap = std::max<int64_t>(entries, std::max<int64_t>(16, 2 * h->tape_cap));
   const int64_t ent_bytes = 8 * (int64_t)h->Bp * h->D * sizeof(float);
+  if (h->tape_budget > 0) {
+    const int64_t lim = h->tape_budget / ent_bytes;
+    if (entries > lim)
+      return fail(h, NODE_ERR_CAPACITY,
+                  "the forward tape needs " + std::to_string(entries) + " entries of " +
+                      std::to_string(ent_bytes) + " bytes, beyond the tape budget (" +
+                      std::to_string(h->tape_budget) +
+                      " bytes); InterpolatingAdjoint(checkpointing = true) keeps only the "
+                      "checkpoints and re-solves one interval at a time");
+    cap = std::min(cap, lim);
+  }
   float* nt = nullptr;
   double* nft = nullptr;
   if (int rc = dalloc_t(h, &nt, (size_t)(cap * ent_bytes))) return rc;
@@ -792,10 +840,17 @@ int ensure_adjoint(node_handle* h) {
     h->Tn.assign(L - 1, nullptr);
     h->Ab.assign(L, nullptr);
     h->src2.assign(L - 1, nullptr);
+    h->E0s.assign(h->npr, nullptr);
+    h->AbLs.assign(h->npr, nullptr);
+    h->E0s[0] = h->E0;
+    for (int k = 1; k < h->npr; ++k) {
+      if (int rc = dalloc(h, &h->E0s[k], unit * h->layers[0].inp)) return rc;
+      if (int rc = dalloc(h, &h->AbLs[k], unit * h->layers[L - 1].outp)) return rc;
+    }
     for (int l = 0; l < L; ++l) {
       const int op = h->layers[l].outp;
       if (int rc = dalloc(h, &h->Ab[l], unit * op)) return rc;
-      if (l == L - 1) break;
+      if (l == L - 1) { h->AbLs[0] = h->Ab[l]; break; }
       if (int rc = dalloc(h, &h->Tn[l], unit * op)) return rc;
       if (int rc = dalloc_t(h, &h->src2[l], (size_t)h->Bp * op * sizeof(float))) return rc;
     }
@@ -899,39 +954,24 @@ int ensure_out(node_handle* h, int64_t n) {
 }
 
 // ---------------------------------------------------------------- forward
-int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
-                 const double* saveat, int64_t n_saveat, node_stats* st) {
-  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
-  if (!o) return fail(h, NODE_ERR_INVALID, "opts is NULL");
-  if (int rc = maybe_autotune(h)) return rc;
-  if (!(o->t1 != o->t0) || !std::isfinite(o->t0) || !std::isfinite(o->t1))
-    return fail(h, NODE_ERR_INVALID, "tspan must be finite with t1 != t0");
-  if (!o->adaptive && !(o->dt != 0.0)) return fail(h, NODE_ERR_INVALID, "fixed-step solve needs dt");
-  if (o->save_everystep && n_saveat > 0)
-    return fail(h, NODE_ERR_UNSUPPORTED, "save_everystep together with saveat is not supported");
-  const double tdir = o->t1 > o->t0 ? 1.0 : -1.0;
-  std::vector<double> ts;
-  if (o->save_start) ts.push_back(o->t0);
-  for (int64_t i = 0; i < n_saveat; ++i) {
-    const double s = saveat[i];
-    if (!(tdir * s > tdir * o->t0 && tdir * s < tdir * o->t1))
-      return fail(h, NODE_ERR_INVALID, "saveat entries must lie strictly inside tspan");
-    if (i > 0 && !(tdir * s > tdir * saveat[i - 1]))
-      return fail(h, NODE_ERR_INVALID, "saveat must be strictly monotone in the direction of time");
-    ts.push_back(s);
-  }
-  if (o->save_end && !o->save_everystep) ts.push_back(o->t1);
-  h->everystep = o->save_everystep != 0;
-  if (int rc = upload_save_times(h, ts)) return rc;
-  if (int rc = ensure_out(h, (int64_t)ts.size())) return rc;
-  if (int rc = ensure_tape(h, 16)) return rc;
+constexpr int64_t TAPE_RING = 4;  // ring slots of a forward that keeps checkpoints only
 
+// The forward Tsit5 loop on the device from d_u0 ([B][D]) over o.t0 -> o.t1
+// with o's tolerances and step options; the first n_save of the uploaded save
+// times are saved into out_dev.  ring > 0: the tape entries form a ring of
+// that many slots (no step record survives the solve).  The final control
+// block is left in h->hctl.
+int fwd_run(node_handle* h, const node_solve_opts& o, const float* d_u0, int64_t n_save,
+            int64_t ring) {
   const StateGeom sg = geom(h);
   Ctl c;
-  init_ctl(h, c, *o, o->t0, o->t1);
-  c.n_save = h->everystep ? 0 : (int64_t)ts.size();
+  init_ctl(h, c, o, o.t0, o.t1);
+  c.n_save = n_save;
+  c.tape_ring = ring;
+  // (a few entries to start; the loop grows the record as the steps need it)
+  if (int rc = ensure_tape(h, ring > 0 ? ring : 4)) return rc;
   HIPCHK(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->fwd_times, &o->t0, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->fwd_times, &o.t0, sizeof(double), hipMemcpyHostToDevice, h->stream));
   double* p0 = h->part;
   double* px = h->part + SEC;
   const int nb = state_blocks(sg);
@@ -964,9 +1004,21 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
     if (enqueued > 0)
       chunk = next_chunk(*h->hctl, std::max<int64_t>(0, h->hctl->n_save - h->hctl->save_next),
                          graphs ? ga : 1);
+    bool eager = !graphs;
+    if (ring == 0 && h->tape_budget > 0) {
+      // a tape budget: enqueue no more attempts than the budget has entries
+      // for (one by one when fewer than a captured graph's worth are left)
+      const int64_t room = h->tape_budget / (8 * (int64_t)h->Bp * h->D * (int64_t)sizeof(float)) -
+                           (h->hctl->n + 2);
+      if (room < chunk) {
+        chunk = std::max<int64_t>(room, 1);
+        eager = true;
+      }
+    }
     enqueued += chunk;
-    if (int rc = ensure_tape(h, h->hctl->n + chunk + 2)) return rc;
-    if (graphs) {
+    if (ring == 0)
+      if (int rc = ensure_tape(h, h->hctl->n + chunk + 2)) return rc;
+    if (!eager) {
       const void* sig[3] = {h->tape, h->save_times, h->out_dev};
       if (h->fwd_graph && std::memcmp(sig, h->fwd_graph_sig, sizeof(sig)) != 0) {
         (void)hipGraphExecDestroy(h->fwd_graph);
@@ -989,9 +1041,62 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
   }
   // pending saves of the final step
   launch_fwd_step_begin(sg, h->tape, h->X0, xpl, h->save_times, h->out_dev, h->ctl, h->stream);
-  if (int rc = poll_ctl(h)) return rc;
+  return poll_ctl(h);
+}
+
+int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
+                 const double* saveat, int64_t n_saveat, node_stats* st) {
+  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  if (!o) return fail(h, NODE_ERR_INVALID, "opts is NULL");
+  if (int rc = maybe_autotune(h)) return rc;
+  if (!(o->t1 != o->t0) || !std::isfinite(o->t0) || !std::isfinite(o->t1))
+    return fail(h, NODE_ERR_INVALID, "tspan must be finite with t1 != t0");
+  if (!o->adaptive && !(o->dt != 0.0)) return fail(h, NODE_ERR_INVALID, "fixed-step solve needs dt");
+  if (o->save_everystep && n_saveat > 0)
+    return fail(h, NODE_ERR_UNSUPPORTED, "save_everystep together with saveat is not supported");
+  const bool ring = h->ckpt && h->sense == NODE_SENSE_INTERPOLATING && !h->cnf;
+  if (ring && o->save_everystep)
+    return fail(h, NODE_ERR_UNSUPPORTED,
+                "InterpolatingAdjoint(checkpointing = true) with save_everystep is not supported "
+                "(its checkpoints would be every step)");
+  if (ring && !(o->t1 > o->t0))
+    return fail(h, NODE_ERR_UNSUPPORTED, "checkpointing needs t1 > t0");
+  const double tdir = o->t1 > o->t0 ? 1.0 : -1.0;
+  std::vector<double> ts;
+  if (o->save_start) ts.push_back(o->t0);
+  for (int64_t i = 0; i < n_saveat; ++i) {
+    const double s = saveat[i];
+    if (!(tdir * s > tdir * o->t0 && tdir * s < tdir * o->t1))
+      return fail(h, NODE_ERR_INVALID, "saveat entries must lie strictly inside tspan");
+    if (i > 0 && !(tdir * s > tdir * saveat[i - 1]))
+      return fail(h, NODE_ERR_INVALID, "saveat must be strictly monotone in the direction of time");
+    ts.push_back(s);
+  }
+  if (o->save_end && !o->save_everystep) ts.push_back(o->t1);
+  h->everystep = o->save_everystep != 0;
+  h->fwd_ring = false;
+  if (int rc = upload_save_times(h, ts)) return rc;
+  if (int rc = ensure_out(h, (int64_t)ts.size())) return rc;
+  const int64_t sv = (int64_t)h->B * h->D;
+  if (ring) {
+    if (!h->ck_u0) {
+      if (int rc = dalloc_t(h, &h->ck_u0, sv * sizeof(float))) return rc;
+      if (int rc = dalloc_t(h, &h->ck_uT, sv * sizeof(float))) return rc;
+      if (int rc = dalloc_t(h, &h->ctl_save, sizeof(Ctl))) return rc;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->ck_u0, d_u0, sv * sizeof(float), hipMemcpyDeviceToDevice,
+                             h->stream));
+  }
+  if (int rc = fwd_run(h, *o, d_u0, h->everystep ? 0 : (int64_t)ts.size(), ring ? TAPE_RING : 0))
+    return rc;
   const Ctl& r = *h->hctl;
   h->n_fwd = r.n;
+  if (ring) {  // u(t1): the uprev of the final entry of the ring
+    const int64_t ent = 8 * (int64_t)h->Bp * h->D;
+    HIPCHK(h, hipMemcpyAsync(h->ck_uT, h->tape + (r.n % TAPE_RING) * ent, sv * sizeof(float),
+                             hipMemcpyDeviceToDevice, h->stream));
+  }
+  const StateGeom sg = geom(h);
   if (h->everystep) {
     // every accepted step: states are tape[j].uprev, j = 0..n
     std::vector<double> ft(r.n + 1);
@@ -1016,8 +1121,64 @@ int forward_impl(node_handle* h, const node_solve_opts* o, const float* d_u0,
   s.retcode = r.retcode; s.t_final = r.t;
   h->fwd_stats = s;
   h->fwd_opts = *o;
+  h->fwd_ring = ring;
   h->have_forward = true;
   if (st) *st = s;
+  return 0;
+}
+
+// ---------------------------------------------------------------- checkpoints
+// Device state of checkpoint k ([B][D]).
+const float* ck_state(const node_handle* h, size_t k) {
+  const int64_t src = h->ck_src[k];
+  if (src == -1) return h->ck_u0;
+  if (src == -2) return h->ck_uT;
+  return h->out_dev + src * (int64_t)h->B * h->D;
+}
+
+// Checkpoints of the last forward: t0, the saved times strictly inside the
+// span, t1 (SciMLSensitivity's checkpoints = the solution's saved times, the
+// span's ends added so that every adjoint stage time lies in an interval).
+void ck_plan(node_handle* h) {
+  const node_solve_opts& o = h->fwd_opts;
+  h->ck_t.assign(1, o.t0);
+  h->ck_src.assign(1, -1);
+  for (size_t i = 0; i < h->h_save.size(); ++i)
+    if (h->h_save[i] > o.t0 && h->h_save[i] < o.t1) {
+      h->ck_t.push_back(h->h_save[i]);
+      h->ck_src.push_back((int64_t)i);
+    }
+  h->ck_t.push_back(o.t1);
+  h->ck_src.push_back(-2);
+}
+
+// Re-solve the forward over checkpoint interval k (ck_t[k] -> ck_t[k+1]) from
+// its checkpoint state into the tape: the forward's tolerances and step mode,
+// dtmax / dtmin of the interval problem, initial dt dt_init (0: the
+// heuristic).  Then the state of checkpoint k + 1 goes to tape entry n + 1,
+// where the resumed adjoint's FSAL re-evaluation at that time reads it.
+int ck_resolve(node_handle* h, size_t k, double dt_init) {
+  node_solve_opts o = h->fwd_opts;
+  o.t0 = h->ck_t[k];
+  o.t1 = h->ck_t[k + 1];
+  o.dtmax = 0.0;
+  o.dtmin = 0.0;
+  if (o.adaptive) o.dt = dt_init;
+  o.save_start = o.save_end = o.save_everystep = 0;
+  if (int rc = fwd_run(h, o, ck_state(h, k), 0, 0)) return rc;
+  const Ctl r = *h->hctl;
+  if (r.retcode != NODE_RET_SUCCESS)
+    return fail(h, NODE_ERR_STATE, "checkpoint re-solve of the forward interval [" +
+                                       std::to_string(o.t0) + ", " + std::to_string(o.t1) +
+                                       "] failed (retcode " + std::to_string(r.retcode) + ")");
+  h->ck_n = r.n;
+  h->ck_last_dt = std::fabs(r.t - r.tprev);  // span of the last step
+  h->ck_nf += r.nf;
+  if (int rc = ensure_tape(h, r.n + 2)) return rc;
+  const int64_t ent = 8 * (int64_t)h->Bp * h->D;
+  HIPCHK(h, hipMemcpyAsync(h->tape + (r.n + 1) * ent, ck_state(h, k + 1),
+                           (int64_t)h->B * h->D * sizeof(float), hipMemcpyDeviceToDevice,
+                           h->stream));
   return 0;
 }
 
@@ -1044,22 +1205,31 @@ void cnf_eval(node_handle* h, int stage, int mode, int gate, double* part) {
     ProfScope ps(h, PR_FWD_STAGE, l);
     launch_gemm_fwdstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
   }
-  for (int l = L - 1; l >= 1; --l) launch_bwd_data(h, l, gate, false);
-  const Layer& l0 = h->layers[0];
-  GemmArgs g{};
-  g.A = h->Dl[0];
-  g.lda = l0.outp; g.B = l0.Wc; g.ldb = l0.outp; g.K = l0.outp;
-  g.pa = (int64_t)h->Bp * l0.outp; g.pb = (int64_t)l0.inp * l0.outp; g.ksplit = 1;
-  GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, l0.inp, 2);
-  g.row_tiles = p.rt; g.col_tiles = p.ct;
-  const StateGeom sg = geom(h);
-  EpiCnfStage e{};
-  e.D = h->D; e.B = h->B; e.dm = sg.dm; e.tcat = sg.tcat; e.stage = stage; e.mode = mode;
-  e.tape = h->tape; e.ent = sg.ent; e.sv = sg.sv;
-  e.X0 = h->X0; e.ldx = l0.inp; e.px = (int64_t)h->Bp * l0.inp;
-  e.f = h->fbuf; e.e = h->probe; e.part = part;
-  ProfScope ps(h, PR_BWD_LAST, 0);
-  launch_gemm_cnfstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  // probe backprop(s): the Hutchinson probe, or each unit probe of the exact
+  // trace in turn (the last pass forms the stage)
+  const int npass = h->cnf_exact ? h->cnf_dm : 1;
+  void* lm = h->Lm;
+  for (int k = 0; k < npass; ++k) {
+    if (h->cnf_exact) h->Lm = h->LmS[k];
+    for (int l = L - 1; l >= 1; --l) launch_bwd_data(h, l, gate, false);
+    const Layer& l0 = h->layers[0];
+    GemmArgs g{};
+    g.A = h->Dl[0];
+    g.lda = l0.outp; g.B = l0.Wc; g.ldb = l0.outp; g.K = l0.outp;
+    g.pa = (int64_t)h->Bp * l0.outp; g.pb = (int64_t)l0.inp * l0.outp; g.ksplit = 1;
+    GemmPlan p = plan_role(h, PR_BWD_LAST, 0, h->Bp, l0.inp, 2);
+    g.row_tiles = p.rt; g.col_tiles = p.ct;
+    const StateGeom sg = geom(h);
+    EpiCnfStage e{};
+    e.D = h->D; e.B = h->B; e.dm = sg.dm; e.tcat = sg.tcat; e.stage = stage; e.mode = mode;
+    e.tape = h->tape; e.ent = sg.ent; e.sv = sg.sv;
+    e.X0 = h->X0; e.ldx = l0.inp; e.px = (int64_t)h->Bp * l0.inp;
+    e.f = h->fbuf; e.e = h->probe; e.part = part;
+    e.reg = h->cnf_reg ? 1 : 0; e.pass = k; e.npass = npass; e.trc = h->trc; e.wacc = h->wacc;
+    ProfScope ps(h, PR_BWD_LAST, 0);
+    launch_gemm_cnfstage(h->prec, p.cfg, g, e, h->ctl, gate, h->stream);
+  }
+  h->Lm = lm;
 }
 
 int cnf_last_tiles(node_handle* h) {
@@ -1086,13 +1256,31 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
     if (int rc = dalloc_t(h, &h->bs_fu, sv * 4)) return rc;
   }
   if (int rc = maybe_autotune(h)) return rc;
+  const bool ck = h->fwd_ring;
+  if (ck && h->sense != NODE_SENSE_INTERPOLATING)
+    return fail(h, NODE_ERR_STATE, "the last forward kept checkpoints only "
+                                   "(InterpolatingAdjoint(checkpointing = true)); solve it again "
+                                   "for another sensealg");
+  if (!ck && h->ckpt && h->sense == NODE_SENSE_INTERPOLATING && !h->cnf)
+    return fail(h, NODE_ERR_STATE, "checkpointing was switched on after the forward solve");
   const node_solve_opts& o = h->fwd_opts;
   const StateGeom sg = geom(h);
   const int64_t ncb = (int64_t)h->h_save.size();
+  size_t kcur = 0;  // checkpoint interval on the tape
+  h->ck_nf = 0;
+  if (ck) {  // the last interval first; the adjoint pauses at its lower end
+    ck_plan(h);
+    kcur = h->ck_t.size() - 2;
+    if (int rc = ck_resolve(h, kcur, 0.0)) return rc;
+  }
   Ctl c;
   init_ctl(h, c, o, o.t1, o.t0);
-  c.n_fwd = h->n_fwd;
+  c.n_fwd = ck ? h->ck_n : h->n_fwd;
   c.n_save = ncb;
+  if (ck && kcur > 0) {
+    c.has_pause = 1;
+    c.pause_t = h->ck_t[kcur];
+  }
   int64_t next = ncb - 1;
   if (next >= 0 && h->h_save[next] == o.t1) { c.jump_idx = (int)next; --next; }
   c.save_next = next;
@@ -1139,21 +1327,42 @@ int backward_impl(node_handle* h, const float* d_dLdout, node_stats* st) {
   };
   const bool graphs = h->use_graphs && h->prof_mask == 0;
   const int ga = h->graph_attempts;
-  const void* asig[4] = {h->tape, h->save_times, d_dLdout, h->out_dev};
-  if (h->adj_graph && std::memcmp(asig, h->adj_graph_sig, sizeof(asig)) != 0) {
-    (void)hipGraphExecDestroy(h->adj_graph);
-    h->adj_graph = nullptr;
-  }
   int64_t chunk = graphs ? ga : 2, enqueued = 0;
+  bool resumed = false;
   for (;;) {
     if (int rc = poll_ctl(h)) return rc;
-    if (h->hctl->done) break;
+    if (h->hctl->done) {
+      if (!(ck && h->hctl->paused)) break;
+      // checkpoint reached: re-solve the forward interval below it, resume
+      HIPCHK(h, hipMemcpyAsync(h->ctl_save, h->ctl, sizeof(Ctl), hipMemcpyDeviceToDevice,
+                               h->stream));
+      --kcur;
+      if (int rc = ck_resolve(h, kcur, h->ck_last_dt)) return rc;
+      // a re-solve may have grown (moved) the tape: the attempts' buffer
+      // table follows it (the captured graph is re-captured below)
+      ab = adj_bufs(h);
+      ab.dLdout = d_dLdout;
+      launch_adj_resume(h->ctl, h->ctl_save, h->ck_n, kcur > 0 ? h->ck_t[kcur] : 0.0,
+                        kcur > 0 ? 1 : 0, h->fwd_times, h->stream);
+      HIPCHK(h, hipGetLastError());
+      resumed = true;
+      continue;
+    }
     if (enqueued > c.maxiters + 64)
       return fail(h, NODE_ERR_STATE, "adjoint step controller did not terminate (internal error)");
-    if (enqueued > 0)
+    if (enqueued > 0 && !resumed)
       chunk = next_chunk(*h->hctl, h->hctl->save_next + 1, graphs ? ga : 1);
+    else if (resumed)
+      chunk = graphs ? ga : 2;
+    resumed = false;
     enqueued += chunk;
     if (graphs) {
+      // the tape may have moved (a checkpoint re-solve grew it)
+      const void* asig[4] = {h->tape, h->save_times, d_dLdout, h->out_dev};
+      if (h->adj_graph && std::memcmp(asig, h->adj_graph_sig, sizeof(asig)) != 0) {
+        (void)hipGraphExecDestroy(h->adj_graph);
+        h->adj_graph = nullptr;
+      }
       if (!h->adj_graph) {
         hipGraph_t gr = nullptr;
         HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
@@ -1477,13 +1686,20 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
   for (auto& row : h->tks)
     for (int& c : row) c = 1;
   h->desc = *desc;
+  h->cnf_reg = (desc->input_map & NODE_MAP_CNF_REGULARIZE) != 0;
+  h->cnf_exact = (desc->input_map & NODE_MAP_CNF_EXACT_TRACE) != 0;
+  h->desc.input_map &= ~(NODE_MAP_CNF_REGULARIZE | NODE_MAP_CNF_EXACT_TRACE);
   h->L = desc->n_layers;
-  h->cnf = desc->input_map == NODE_MAP_CNF || desc->input_map == NODE_MAP_CNF_TIME;
-  h->D = h->cnf ? desc->widths[desc->n_layers] + 1 : desc->widths[0];  // ODE state width
+  h->cnf = h->desc.input_map == NODE_MAP_CNF || h->desc.input_map == NODE_MAP_CNF_TIME;
+  h->cnf_dm = desc->widths[desc->n_layers];
+  // ODE state width: [z; log p] (+ r1, r2 with regularize) for a CNF
+  h->D = h->cnf ? h->cnf_dm + (h->cnf_reg ? 3 : 1) : desc->widths[0];
+  h->npr = h->cnf && h->cnf_exact ? h->cnf_dm : 1;
   h->Dp = (int)rup(h->D, KPAD);
   h->B = (int)desc->batch;
   h->Bp = (int)rup(h->B, BPAD);
-  h->ldT = h->cnf ? 2 * h->Bp : h->Bp;
+  // transposed operands: primal rows, then one slice per tangent chain (CNF)
+  h->ldT = h->cnf ? (1 + h->npr) * h->Bp : h->Bp;
   h->prec = desc->precision;
   h->wg_prec = h->prec;
   h->esz = h->prec == NODE_PREC_FP32 ? 4 : 2;
@@ -1689,21 +1905,43 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
   if (!h) return NODE_ERR_INVALID;
   std::lock_guard<std::mutex> g(h->mtx);
   if (!h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "not a CNF handle (input_map NODE_MAP_CNF*)");
-  if (!opts || !x || !probe) return fail(h, NODE_ERR_INVALID, "null argument");
+  const bool need_probe = !h->cnf_exact || h->cnf_reg;  // the exact trace alone needs none
+  if (!opts || !x || (need_probe && !probe)) return fail(h, NODE_ERR_INVALID, "null argument");
   if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
   h->cnf_fwd = false;
   if (int rc = ensure_adjoint(h)) return rc;  // Lm holds the probe for the data-gradient GEMMs
   if (int rc = maybe_autotune(h)) return rc;
-  const int dm = h->D - 1, B = h->B;
+  const int dm = h->cnf_dm, B = h->B;
   if (!h->probe && dalloc_t(h, &h->probe, (size_t)h->Bp * dm * sizeof(float))) return NODE_ERR_HIP;
   if (!h->fbuf && dalloc_t(h, &h->fbuf, (size_t)h->Bp * dm * sizeof(float))) return NODE_ERR_HIP;
-  HIPCHK(h, hipMemcpyAsync(h->probe, probe, (size_t)B * dm * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  std::vector<float> ph((size_t)B * dm, 0.f);
+  if (probe) std::copy(probe, probe + (size_t)B * dm, ph.begin());
+  HIPCHK(h, hipMemcpyAsync(h->probe, ph.data(), ph.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
   StateGeom sgm = geom(h);
   sgm.D = dm;
   const Layer& ll = h->layers[h->L - 1];
   const int64_t pl = (int64_t)h->Bp * ll.outp;
   launch_prep_lambda(sgm, h->probe, h->Lm, h->LmT, ll.outp, h->ldT, pl, (int64_t)h->ldT * ll.outp,
                      h->stream);
+  if (h->cnf_exact) {
+    // unit-probe seeds e_k at the last layer, and the trace accumulators
+    const size_t unit = (size_t)h->Bp * h->esz * h->planes * ll.outp;
+    if (h->LmS.empty()) {
+      h->LmS.assign(dm, nullptr);
+      for (int k = 0; k < dm; ++k)
+        if (int rc = dalloc(h, &h->LmS[k], unit)) return rc;
+      if (int rc = dalloc_t(h, &h->trc, (size_t)h->Bp * sizeof(float))) return rc;
+      if (int rc = dalloc_t(h, &h->wacc, (size_t)h->Bp * dm * sizeof(float))) return rc;
+    }
+    std::vector<float> oh((size_t)B * dm);
+    for (int k = 0; k < dm; ++k) {
+      for (int b = 0; b < B; ++b)
+        for (int c = 0; c < dm; ++c) oh[(size_t)b * dm + c] = c == k ? 1.f : 0.f;
+      HIPCHK(h, hipMemcpyAsync(h->fbuf, oh.data(), oh.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+      launch_prep_lambda(sgm, h->fbuf, h->LmS[k], nullptr, ll.outp, 0, pl, 0, h->stream);
+      HIPCHK(h, hipStreamSynchronize(h->stream));  // oh is rewritten for the next seed
+    }
+  }
   // u0 = [x; 0] staged through the scratch state buffer
   std::vector<float> u0((size_t)B * h->D, 0.f);
   for (int b = 0; b < B; ++b)
@@ -1733,7 +1971,13 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
     if (delta_logp) delta_logp[b] = dl;
     if (logpx) logpx[b] = (float)(-0.5 * ss - lc - (double)dl);
   }
-  h->probe_h.assign(probe, probe + (size_t)B * dm);
+  h->cnf_lam.assign(2 * (size_t)B, 0.f);
+  if (h->cnf_reg)
+    for (int b = 0; b < B; ++b) {
+      h->cnf_lam[b] = fin[(size_t)b * h->D + dm + 1];
+      h->cnf_lam[(size_t)B + b] = fin[(size_t)b * h->D + dm + 2];
+    }
+  h->probe_h = ph;
   h->cnf_fwd = st.retcode == NODE_RET_SUCCESS;
   if (stats) *stats = st;
   return 0;
@@ -1747,7 +1991,7 @@ int32_t node_cnf_backward(node_handle_t* h, const float* dL_dz, const float* dL_
   if (!dL_dz || !dL_ddlogp) return fail(h, NODE_ERR_INVALID, "null argument");
   if (!h->cnf_fwd || !h->have_forward)
     return fail(h, NODE_ERR_STATE, "no successful CNF forward solve to differentiate");
-  const int dm = h->D - 1, B = h->B, L = h->L;
+  const int dm = h->cnf_dm, B = h->B, L = h->L;
   const int64_t n = (int64_t)B * h->D;
   if (n > h->dLdout_cap) {
     dfree(h, h->dLdout_dev);
@@ -1755,28 +1999,42 @@ int32_t node_cnf_backward(node_handle_t* h, const float* dL_dz, const float* dL_
     if (int rc = dalloc_t(h, &h->dLdout_dev, n * sizeof(float))) return rc;
     h->dLdout_cap = n;
   }
-  // cotangent of the saved end state [z; Δlogp], and the tangent chain's seed
-  // abar_{L-1} = -λ_Δ e (λ_Δ is constant over the adjoint solve: dλ_Δ/dt = 0)
-  std::vector<float> cot((size_t)n), seed((size_t)B * dm);
+  // cotangent of the saved end state [z; Δlogp] (the regulariser rows, when
+  // present, get none: their outputs carry no gradient on this path)
+  std::vector<float> cot((size_t)n, 0.f);
   for (int b = 0; b < B; ++b) {
-    for (int c = 0; c < dm; ++c) {
-      cot[(size_t)b * h->D + c] = dL_dz[(size_t)b * dm + c];
-      seed[(size_t)b * dm + c] = -dL_ddlogp[b] * h->probe_h[(size_t)b * dm + c];
-    }
+    for (int c = 0; c < dm; ++c) cot[(size_t)b * h->D + c] = dL_dz[(size_t)b * dm + c];
     cot[(size_t)b * h->D + dm] = dL_ddlogp[b];
   }
   HIPCHK(h, hipMemcpyAsync(h->dLdout_dev, cot.data(), cot.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->fbuf, seed.data(), seed.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
   StateGeom sgm = geom(h);
   sgm.D = dm;
   const Layer& l0 = h->layers[0];
   const Layer& ll = h->layers[L - 1];
-  // t_{-1} = [e; 0]: row-major (layer 0's tangent input) and transposed
-  // (second half of X0^T, layer 0's weight gradient)
-  launch_prep_lambda(sgm, h->probe, h->E0, tan_half(h, h->X0T), l0.inp, h->ldT,
-                     (int64_t)h->Bp * l0.inp, (int64_t)h->ldT * h->Dp, h->stream);
-  launch_prep_lambda(sgm, h->fbuf, h->Ab[L - 1], tan_half(h, h->LmT), ll.outp, h->ldT,
-                     (int64_t)h->Bp * ll.outp, (int64_t)h->ldT * h->Dp, h->stream);
+  // per tangent chain k (the probe e, or unit probe e_k of the exact trace):
+  // t_{-1} = [e_k; 0] row-major (layer 0's tangent input) and transposed
+  // (slice k of X0^T, layer 0's weight gradient), and the reverse seed
+  // abar_{L-1} = -λ_Δ e_k (λ_Δ is constant over the adjoint solve) with its
+  // transposed copy (slice k of Lm^T); the seeds hold for the whole solve
+  std::vector<float> tv((size_t)B * dm), sv((size_t)B * dm);
+  for (int k = 0; k < h->npr; ++k) {
+    for (int b = 0; b < B; ++b)
+      for (int c = 0; c < dm; ++c) {
+        const float e = h->cnf_exact ? (c == k ? 1.f : 0.f) : h->probe_h[(size_t)b * dm + c];
+        tv[(size_t)b * dm + c] = e;
+        sv[(size_t)b * dm + c] = -dL_ddlogp[b] * e;
+      }
+    HIPCHK(h, hipMemcpyAsync(h->probe, tv.data(), tv.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->fbuf, sv.data(), sv.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    launch_prep_lambda(sgm, h->probe, h->E0s[k], tan_half(h, h->X0T, k), l0.inp, h->ldT,
+                       (int64_t)h->Bp * l0.inp, (int64_t)h->ldT * h->Dp, h->stream);
+    launch_prep_lambda(sgm, h->fbuf, h->AbLs[k], tan_half(h, h->LmT, k), ll.outp, h->ldT,
+                       (int64_t)h->Bp * ll.outp, (int64_t)h->ldT * h->Dp, h->stream);
+    HIPCHK(h, hipStreamSynchronize(h->stream));  // tv / sv are rewritten for the next chain
+  }
+  // the device probe back to the forward's (the exact trace's regulariser row)
+  HIPCHK(h, hipMemcpyAsync(h->probe, h->probe_h.data(), h->probe_h.size() * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipGetLastError());
   h->cnf_adj = true;
   int rc = backward_impl(h, h->dLdout_dev, stats);
@@ -1787,6 +2045,17 @@ int32_t node_cnf_backward(node_handle_t* h, const float* dL_dz, const float* dL_
   if (dL_dx)
     for (int b = 0; b < B; ++b)
       for (int c = 0; c < dm; ++c) dL_dx[(size_t)b * dm + c] = lam0[(size_t)b * h->D + c];
+  return 0;
+}
+
+int32_t node_cnf_regularizers(node_handle_t* h, float* lam1, float* lam2) {
+  if (!h) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->cnf || !h->cnf_reg)
+    return fail(h, NODE_ERR_UNSUPPORTED, "not a CNF handle with NODE_MAP_CNF_REGULARIZE");
+  if (!h->cnf_fwd) return fail(h, NODE_ERR_STATE, "no successful CNF forward solve");
+  if (lam1) std::copy(h->cnf_lam.begin(), h->cnf_lam.begin() + h->B, lam1);
+  if (lam2) std::copy(h->cnf_lam.begin() + h->B, h->cnf_lam.end(), lam2);
   return 0;
 }
 
@@ -1938,6 +2207,15 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
   } else if (key == NODE_OPT_GRAPH_ATTEMPTS) {
     if (value < 1 || value > 16) return fail(h, NODE_ERR_INVALID, "graph attempts must be 1..16");
     h->graph_attempts = (int)value;
+  } else if (key == NODE_OPT_CHECKPOINTING) {
+    if (h->cnf && value != 0)
+      return fail(h, NODE_ERR_UNSUPPORTED, "checkpointing: not available on a CNF handle");
+    h->ckpt = value != 0;
+    return 0;  // read by the next forward; the captured graphs stay valid
+  } else if (key == NODE_OPT_TAPE_BUDGET) {
+    if (value < 0) return fail(h, NODE_ERR_INVALID, "tape budget must be >= 0 bytes");
+    h->tape_budget = value;
+    return 0;
   } else {
     return fail(h, NODE_ERR_INVALID, "unknown option");
   }

@@ -44,6 +44,13 @@ struct Ctl {
   int32_t st_exact[8];         // 1: exact stored state tape[st_step].uprev
   float st_dt[8];
   float st_w[8][7];
+  // forward: tape entries form a ring of this many slots (0: one entry per step)
+  int64_t tape_ring;
+  // adjoint with forward-state checkpoints: the solve pauses (done = paused =
+  // 1) after an accepted step lands on pause_t, so that the host can re-solve
+  // the forward interval below it before the adjoint continues
+  double pause_t;
+  int32_t has_pause, paused;
 };
 
 struct GemmArgs {
@@ -88,6 +95,12 @@ struct EpiCnfStage {
   void* X0; int ldx; int64_t px;
   const float* f; const float* e;          // [Bp][dm] FP32 model output / probe
   double* part;
+  // regularize: the state is [z; log p; r1; r2] with r1' = |f|^2, r2' = |w|
+  // (w = e^T df/dz); exact trace: npass unit-probe passes (pass = this one's
+  // probe index), the GEMM giving row `pass` of the Jacobian, accumulated in
+  // trc [Bp] (trace) and wacc [Bp][dm] (e^T J); the last pass forms the stage
+  int reg, pass, npass;
+  float* trc; float* wacc;
 };
 
 // hidden data gradient: delta = acc * act'(z)  (row-major + transposed);
@@ -120,6 +133,7 @@ struct EpiTanBwd {
   int64_t py, ptf;                               // plane strides of Y / Tf
   void* Ab; void* AbT; int64_t pab, pabt;
   float* src;
+  int src_acc;                                   // add to src (later probes of an exact trace)
 };
 
 // layer-1 data gradient: k^lambda_s = -(acc * map'(u)), next stage inputs
@@ -267,6 +281,8 @@ void launch_adj_step_begin(const StateGeom& sg, const AdjBufs& ab, const Ctl* ct
 void launch_mu_norm(int mode, const AdjBufs& ab, const Ctl* ctl, double* part,
                     int nblk, int gate, hipStream_t s);
 int mu_blocks(int64_t P);
+void launch_adj_resume(Ctl* ctl, const Ctl* saved, int64_t n_fwd, double pause_t, int has_pause,
+                       const double* fwd_times, hipStream_t s);
 void launch_adj_controller(Ctl* ctl, const double* part, int n_lam,
                            const double* part_mu, int n_mu, double n_elems,
                            const double* cb_times, const double* fwd_times,

@@ -594,8 +594,8 @@ __device__ __forceinline__ void rg_mu_entry(int64_t idx, float kb, float ke, flo
 
 template <int MODE>
 __device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, double dt,
-                             double atol, double rtol, const float* mu, float* mu_new,
-                             double* flag, int lane) {
+                             double atol, double rtol, const float* __restrict__ mu,
+                             float* __restrict__ mu_new, double* flag, int lane) {
   // The stage sums are outer products over the stages j:
   //   k_b[i][o] = sum_j cb_j delta_j[o] h_j[i],  k_e likewise with ce_j,
   // i.e. for every 16x16 block of (layer input i, output o) a GEMM with
@@ -670,16 +670,24 @@ __device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, dou
 #pragma unroll
       for (int q = 0; q < 2; ++q)
         ha[q] = use[q] ? sbs[blk[q] * m.sv + m.sh[l] + 16 * it + r] : 0.f;
+      // every mu entry of this input block first (up to 4 RG_HT independent
+      // loads per lane in flight: the pass streams mu and is bound by how many
+      // loads wait at once), then the MFMAs and the entries
+      float m0a[RG_HT][4];
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        const int o = 16 * ot + r;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * it + 4 * g + v;
+          m0a[ot][v] = (ot < nto && o < out && i < in) ? mu[w0 + (int64_t)i * out + o] : 0.f;
+        }
+      }
 #pragma unroll
       for (int ot = 0; ot < RG_HT; ++ot) {
         if (ot < nto) {
           const int o = 16 * ot + r;
-          float m0v[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = 16 * it + 4 * g + v;
-            m0v[v] = (o < out && i < in) ? mu[w0 + (int64_t)i * out + o] : 0.f;
-          }
+          const float* m0v = m0a[ot];
           f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb;
           kb = rg_mfma(ha[0], db[ot][0], kb);
           kb = rg_mfma(ha[1], db[ot][1], kb);

@@ -6,12 +6,16 @@ the HIP library (``src/ffjord.jl:55-178``)::
     (logpx, λ₁, λ₂), st = ffjord(x, ps, st)
 
 ``x`` is ``(B, D)`` (a Julia ``D x B`` matrix).  ``st`` carries
-``regularize`` / ``monte_carlo`` as in the reference (``:60-65``); the HIP
-path runs ``monte_carlo = true`` without regularisation.  The Hutchinson
-probe is drawn once per call as a Rademacher vector per sample (``st["rng"]``,
-a numpy Generator, or ``st["probe"]`` to pass one) and held fixed for the
-whole solve; the reference redraws ``randn!`` on every right-hand-side call
-(``:96``).  A model whose input width is ``D + 1`` gets ``[z; t]``
+``regularize`` / ``monte_carlo`` as in the reference (``:60-65``):
+``monte_carlo = false`` takes the exact trace (D unit-probe passes per
+right-hand side on the GPU), ``regularize = true`` integrates ``‖f‖²`` and
+``‖eᵀ ∂f/∂z‖`` as two more state rows and returns them as ``λ₁``, ``λ₂``
+(without it ``λ₁ = λ₂ = Δlogp``, ``:166-167``).  The probe (Hutchinson, and
+the regulariser's ``e``) is drawn once per call as a Rademacher vector per
+sample (``st["rng"]``, a numpy Generator, or ``st["probe"]`` to pass one) and
+held fixed for the whole solve; the reference redraws ``randn!`` on every
+right-hand-side call (``:96``).  The regularisers carry no gradient on the
+HIP path: a loss that differentiates ``λ₁`` / ``λ₂`` raises.  A model whose input width is ``D + 1`` gets ``[z; t]``
 (time-concatenated network); a ``D``-wide input gets ``z`` as in the
 reference (``:88``).
 
@@ -58,20 +62,24 @@ class FFJORD:
     def setup(self, rng, init="pytorch"):
         return self.model.init_params(rng, init), {"regularize": False, "monte_carlo": True}
 
-    def _handle(self, B):
-        h = self._handles.get(B)
+    def _handle(self, B, reg=False, mc=True):
+        key = (B, bool(reg), bool(mc))
+        h = self._handles.get(key)
         if h is None:
             imap = _lib.MAP_CNF_TIME if self.time_concat else _lib.MAP_CNF
+            if reg:
+                imap |= _lib.MAP_CNF_REGULARIZE
+            if not mc:
+                imap |= _lib.MAP_CNF_EXACT_TRACE
             h = Handle(self.model.widths, self.model.acts, imap, self.precision, B)
-            self._handles[B] = h
+            h.cnf_reg = bool(reg)
+            self._handles[key] = h
         return h
 
     def __call__(self, x, ps, st=None):
         st = dict(st or {"regularize": False, "monte_carlo": True})
-        if st.get("regularize", False):
-            raise NotImplementedError("regularize = true is not on the HIP path")
-        if not st.get("monte_carlo", True):
-            raise NotImplementedError("the exact-trace (monte_carlo = false) form is not on the HIP path")
+        reg = bool(st.get("regularize", False))
+        mc = bool(st.get("monte_carlo", True))
         x_in = x
         x = x.detach().cpu().numpy() if hasattr(x, "detach") else x
         x = np.asarray(x, np.float32)
@@ -84,24 +92,28 @@ class FFJORD:
         if probe is None:
             rng = st.get("rng") or np.random.default_rng()
             probe = rng.choice(np.array([-1.0, 1.0], np.float32), size=(B, self.D))
-        h = self._handle(B)
+        h = self._handle(B, reg, mc)
         try:
             import torch
             diff = isinstance(x_in, torch.Tensor) or isinstance(ps, torch.Tensor)
         except ImportError:
             diff = False
         if diff:
-            z, dl, stats = _FFJORDFn.apply(x_in, ps, self, h, probe)
+            z, dl, l1, l2, stats = _FFJORDFn.apply(x_in, ps, self, h, probe)
             D = self.D
             logpz = -0.5 * (z * z).sum(dim=1) - 0.5 * D * float(np.log(2 * np.pi))
             lp = (logpz - dl).reshape(1, B)
             dl = dl.reshape(1, B)
             st["stats"] = stats
+            if reg:
+                return (lp, l1.reshape(1, B), l2.reshape(1, B)), st
             return (lp, dl, dl), st
         h.set_params(np.asarray(ps, np.float32))
-        lp, z, dl, stats = self._solve(h, x, probe)
+        lp, z, dl, l1, l2, stats = self._solve(h, x, probe)
         st["stats"] = stats
         dl = dl.reshape(1, B)
+        if reg:
+            return (lp.reshape(1, B), l1.reshape(1, B), l2.reshape(1, B)), st
         return (lp.reshape(1, B), dl, dl), st
 
     def _solve(self, h, x, probe):
@@ -113,7 +125,10 @@ class FFJORD:
                                          maxiters=kw.get("maxiters", 0))
         if stats["retcode"] != 1:
             raise RuntimeError(f"FFJORD solve failed: {_lib.RET_NAMES.get(stats['retcode'])}")
-        return lp, z, dl, stats
+        l1 = l2 = None
+        if getattr(h, "cnf_reg", False):
+            l1, l2 = h.cnf_regularizers()
+        return lp, z, dl, l1, l2, stats
 
 
 def _FFJORDFn_factory():
@@ -130,20 +145,27 @@ def _FFJORDFn_factory():
             pt = p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, np.float32))
             h.set_params(pt.detach().cpu().numpy().astype(np.float32))
             xn = xt.detach().cpu().numpy().astype(np.float32)
-            _, z, dl, stats = ffj._solve(h, xn, probe)
+            _, z, dl, l1, l2, stats = ffj._solve(h, xn, probe)
+            if l1 is None:
+                l1 = l2 = np.zeros_like(dl)
             ctx.h, ctx.gen = h, h.fwd_gen
             ctx.xshape, ctx.xdtype, ctx.xdev = xt.shape, xt.dtype, xt.device
             ctx.pdtype, ctx.pdev = pt.dtype, pt.device
-            ctx.mark_non_differentiable  # noqa: B018
-            return (torch.from_numpy(z).to(xt.device), torch.from_numpy(dl).to(xt.device), stats)
+            dev = xt.device
+            return (torch.from_numpy(z).to(dev), torch.from_numpy(dl).to(dev),
+                    torch.from_numpy(l1).to(dev), torch.from_numpy(l2).to(dev), stats)
 
         @staticmethod
-        def backward(ctx, gz, gdl, _gs=None):
+        def backward(ctx, gz, gdl, gl1=None, gl2=None, _gs=None):
             h = ctx.h
             if h.fwd_gen != ctx.gen:
                 raise RuntimeError("FFJORD backward: the layer's handle ran another solve since "
                                    "this one (the handle keeps one tape); call backward before "
                                    "the next forward of the same layer and batch")
+            for gl in (gl1, gl2):
+                if gl is not None and bool((gl != 0).any()):
+                    raise NotImplementedError("FFJORD: gradients through the regularisers "
+                                              "(λ₁, λ₂) are not on the HIP path")
             gz = torch.zeros(ctx.xshape) if gz is None else gz
             gdl = torch.zeros(ctx.xshape[0]) if gdl is None else gdl
             dx, dp, _ = h.cnf_backward(gz.detach().cpu().numpy(), gdl.detach().cpu().numpy())

@@ -163,7 +163,7 @@ class Handle:
         (logpx (B,), z(T) (B, D), delta_logp (B,), stats dict)."""
         D = self.widths[-1]
         x = _f32(x).reshape(self.B, D)
-        e = _f32(probe).reshape(self.B, D)
+        e = _f32(probe).reshape(self.B, D) if probe is not None else None
         z = np.empty((self.B, D), np.float32)
         dl = np.empty(self.B, np.float32)
         lp = np.empty(self.B, np.float32)
@@ -171,10 +171,20 @@ class Handle:
                            dtmax=dtmax, dtmin=dtmin, maxiters=maxiters)
         st = _lib.Stats()
         self.fwd_gen += 1
-        self._check(self.lib.node_cnf_forward(self.h, ctypes.byref(o), _ptr(x), _ptr(e), _ptr(z),
+        self._check(self.lib.node_cnf_forward(self.h, ctypes.byref(o), _ptr(x),
+                                              _ptr(e) if e is not None else None, _ptr(z),
                                               _ptr(dl), _ptr(lp), ctypes.byref(st)),
                     "node_cnf_forward")
         return lp, z, dl, st.as_dict()
+
+    def cnf_regularizers(self):
+        """(r1(T), r2(T)) per sample of the last ``cnf_forward`` on a
+        regularised CNF handle (the layer's λ₁, λ₂)."""
+        l1 = np.empty(self.B, np.float32)
+        l2 = np.empty(self.B, np.float32)
+        self._check(self.lib.node_cnf_regularizers(self.h, _ptr(l1), _ptr(l2)),
+                    "node_cnf_regularizers")
+        return l1, l2
 
     def cnf_backward(self, dL_dz, dL_ddlogp):
         """InterpolatingAdjoint of the last ``cnf_forward`` (same probe).
@@ -259,10 +269,14 @@ class Handle:
     def set_option(self, key, value):
         """key: 'graphs' (bool), 'graph_attempts' (1..16), 'autotune' (bool),
         'wgrad_precision' ('fp32' / 'bf16' / 'bf16x3') or 'sensealg'
-        ('interpolating' / 'backsolve' / 'backsolve_nockpt')."""
+        ('interpolating' / 'backsolve' / 'backsolve_nockpt'), 'checkpointing'
+        (bool: InterpolatingAdjoint(checkpointing = true), the forward keeps
+        checkpoint states only and the adjoint re-solves each interval) or
+        'tape_budget' (bytes of forward tape, 0 = unlimited)."""
         k = {"graphs": _lib.OPT_GRAPHS, "graph_attempts": _lib.OPT_GRAPH_ATTEMPTS,
              "autotune": _lib.OPT_AUTOTUNE, "wgrad_precision": _lib.OPT_WGRAD_PRECISION,
-             "sensealg": _lib.OPT_SENSEALG}[key]
+             "sensealg": _lib.OPT_SENSEALG, "checkpointing": _lib.OPT_CHECKPOINTING,
+             "tape_budget": _lib.OPT_TAPE_BUDGET}[key]
         if key == "wgrad_precision" and isinstance(value, str):
             value = _lib.PREC[value]
         if key == "sensealg" and isinstance(value, str):

@@ -50,7 +50,8 @@ class InterpolatingAdjoint:
         self.checkpointing = checkpointing
 
     def __repr__(self):
-        return f"InterpolatingAdjoint(autojacvec={self.autojacvec!r})"
+        return (f"InterpolatingAdjoint(autojacvec={self.autojacvec!r}, "
+                f"checkpointing={self.checkpointing})")
 
 
 class _Unsupported:
@@ -152,9 +153,14 @@ class Chain:
 
 # ---------------------------------------------------------------- solution
 class ODESolution:
-    """What downstream code uses of SciMLBase's ODESolution: ``.u``, ``.t``,
-    ``.retcode``, ``Array(sol)`` (``np.asarray`` / ``.array``), ``len``,
-    indexing and ``sum``."""
+    """What downstream code uses of SciMLBase's ODESolution (returned at
+    ``src/neural_de.jl:54-60``): ``.u`` (one state per saved time, e.g.
+    ``sol.u[-1]`` as ``sol.u[end]`` in docs/src/examples/mnist_neural_ode.md:43),
+    ``.t``, ``.retcode``, ``.stats``, ``Array(sol)`` (``np.asarray`` /
+    ``.array``), ``len``, indexing, ``sum``, and ``sol(t)``: the state at any
+    time of the span by linear interpolation between the saved states (what
+    SciMLBase gives for a solution saved at ``saveat`` times, which carries no
+    dense output)."""
 
     def __init__(self, array, t, retcode, stats):
         self.array = array            # (Nt, B, D) numpy or torch
@@ -180,6 +186,23 @@ class ODESolution:
 
     def sum(self):
         return self.array.sum()
+
+    def __call__(self, t):
+        ts = self.t
+        if len(ts) == 0:
+            raise ValueError("solution holds no saved states")
+        t = float(t)
+        lo, hi = (ts[0], ts[-1]) if ts[-1] >= ts[0] else (ts[-1], ts[0])
+        if not lo <= t <= hi:
+            raise ValueError(f"t = {t} lies outside the saved times [{lo}, {hi}]: "
+                             "the solution cannot extrapolate")
+        i = int(np.searchsorted(ts, t, side="right")) - 1 if ts[-1] >= ts[0] else \
+            int(np.searchsorted(-ts, -t, side="right")) - 1
+        i = min(max(i, 0), len(ts) - 1)
+        if ts[i] == t or i == len(ts) - 1:
+            return self.array[i]
+        w = (t - ts[i]) / (ts[i + 1] - ts[i])
+        return self.array[i] + w * (self.array[i + 1] - self.array[i])
 
 
 def save_plan(t0, t1, saveat=None, save_everystep=None, save_start=None, save_end=None):
@@ -288,7 +311,7 @@ class NeuralODE:
         vjp_ok = sens.autojacvec is None or isinstance(sens.autojacvec, ZygoteVJP) \
             if isinstance(sens, (InterpolatingAdjoint, BacksolveAdjoint)) else False
         if isinstance(sens, InterpolatingAdjoint) and isinstance(sens.autojacvec, ZygoteVJP):
-            return "interpolating"
+            return "interpolating_ckpt" if sens.checkpointing else "interpolating"
         if isinstance(sens, BacksolveAdjoint) and vjp_ok and not sens.noisemixing:
             return "backsolve" if sens.checkpointing else "backsolve_nockpt"
         raise NotImplementedError(f"sensealg {sens!r} is not on the HIP path")
@@ -301,7 +324,8 @@ class NeuralODE:
         if (x.shape[-1] if not vec else x.shape[0]) != D:
             raise ValueError(f"u0 must have D={D} entries per trajectory")
         h = self._handle(B)
-        h.set_option("sensealg", mode)
+        h.set_option("sensealg", "interpolating" if mode == "interpolating_ckpt" else mode)
+        h.set_option("checkpointing", mode == "interpolating_ckpt")
         opts, interior = self._opts()
         try:
             import torch

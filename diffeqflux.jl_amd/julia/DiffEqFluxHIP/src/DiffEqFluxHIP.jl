@@ -8,18 +8,34 @@ supported case, and falls back to the reference method otherwise:
   * model: a Lux `Chain` of `Dense` layers (identity / tanh / softplus / relu /
     sigmoid), optionally preceded by `x -> x.^3`, with equal input/output width;
   * alg: `Tsit5()`;
-  * sensealg: the default `InterpolatingAdjoint(autojacvec = ZygoteVJP())`, or
-    `BacksolveAdjoint(; checkpointing)` (autojacvec unset or `ZygoteVJP()`);
+  * sensealg: the default `InterpolatingAdjoint(autojacvec = ZygoteVJP())`, with
+    or without `checkpointing` (the forward keeps checkpoint states only; the
+    adjoint re-solves each interval), or `BacksolveAdjoint(; checkpointing)`
+    (autojacvec unset or `ZygoteVJP()`);
+  * GEMM precision: the keyword `hip_precision` (`:fp32`, `:bf16`, `:bf16x3`;
+    no counterpart in the reference, removed before any reference call) or
+    the module default `DiffEqFluxHIP.PRECISION[]` (`:fp32`);
   * kwargs: saveat (range, vector or number), save_everystep, save_start,
     save_end, abstol, reltol, dt, adaptive, dtmax, dtmin, maxiters;
   * u0: `Vector{Float32}` (D) or `Matrix{Float32}` (D x B), t1 > t0.
 
 `ChainRulesCore.rrule(::typeof(hip_neuralode), ...)` is the pullback: it
 calls `node_backward` (the HIP InterpolatingAdjoint, or BacksolveAdjoint as
-selected by `NODE_OPT_SENSEALG`) on the forward tape kept in the handle.
-`FFJORD` (src/ffjord.jl:55-178) with a `Chain` of `Dense` layers, `Tsit5()`,
-`monte_carlo = true`, no regularisation and the standard-normal base goes to
-`node_cnf_forward`, its pullback to `node_cnf_backward`.  Julia was not available where this package was written; the
+selected by `NODE_OPT_SENSEALG`) on the forward tape kept in the handle.  A
+handle keeps the tape of its latest forward only: a pullback whose forward was
+superseded (the same layer called again at the same batch size before the
+first pullback ran) raises instead of differentiating the wrong solve.  The
+call returns a SciMLBase `ODESolution` built from the saved states
+(`build_solution`: `.u`, `.t`, `.retcode`, `.stats`, `Array(sol)`, indexing,
+`sol(t)` by linear interpolation between the saved states as for any solve
+without dense output).
+`FFJORD` (src/ffjord.jl:55-178) with a `Chain` of `Dense` layers, `Tsit5()`
+and the standard-normal base goes to `node_cnf_forward` (`monte_carlo = false`:
+the exact trace, `NODE_MAP_CNF_EXACT_TRACE`; `regularize = true`: the two
+regulariser rows, `NODE_MAP_CNF_REGULARIZE`, returned as λ₁, λ₂ -- the rows,
+where the reference's `selectdim(pred, N, ...)` picks sample columns), its
+pullback to `node_cnf_backward` (the regularisers carry no gradient there: a
+non-zero cotangent for λ₁ / λ₂ raises).  Julia was not available where this package was written; the
 call contract it binds is exercised from Python through the same C ABI.
 """
 module DiffEqFluxHIP
@@ -28,6 +44,7 @@ using ChainRulesCore: ChainRulesCore, NoTangent, ZeroTangent, unthunk
 using DiffEqFlux: DiffEqFlux, NeuralODE, FFJORD
 using Lux: Lux, Chain, Dense, WrappedFunction
 using OrdinaryDiffEq: Tsit5
+using SciMLBase: SciMLBase, ODEProblem, ODEFunction, ReturnCode
 using SciMLSensitivity: InterpolatingAdjoint, BacksolveAdjoint, ZygoteVJP
 
 const CRC = ChainRulesCore
@@ -71,15 +88,23 @@ mutable struct Handle
     B::Int
     P::Int
     saved_t::Vector{Float64}
+    gen::Int            # forward solves run on this handle (its tape is the latest's)
     function Handle(desc::NodeDesc)
         r = Ref{Ptr{Cvoid}}(C_NULL)
         rc = ccall((:node_create, libnode()), Int32, (Ref{NodeDesc}, Ref{Ptr{Cvoid}}), desc, r)
         rc == 0 || error("node_create failed ($rc)")
         P = ccall((:node_num_params, libnode()), Int64, (Ref{NodeDesc},), desc)
-        h = new(r[], Int(desc.widths[1]), Int(desc.batch), Int(P), Float64[])
+        h = new(r[], Int(desc.widths[1]), Int(desc.batch), Int(P), Float64[], 0)
         finalizer(x -> ccall((:node_destroy, libnode()), Int32, (Ptr{Cvoid},), x.ptr), h)
         return h
     end
+end
+
+function check_current(h::Handle, gen::Int, what)
+    h.gen == gen && return nothing
+    error("$what: the layer ran another forward solve at this batch size since this one " *
+          "(a handle keeps the tape of its latest forward); take the gradient before calling " *
+          "the layer again")
 end
 
 function check(h::Handle, rc, what)
@@ -108,14 +133,28 @@ end
 
 # NODE_OPT_SENSEALG value of the layer's sensealg (nothing: not on the HIP path)
 const NODE_OPT_SENSEALG = Int32(5)
+const NODE_OPT_CHECKPOINTING = Int32(6)
+# (NODE_SENSE_*, checkpointing flag) of the layer's sensealg; nothing: not on the HIP path
 function sense_mode(n::NeuralODE)
-    haskey(n.kwargs, :sensealg) || return Int64(0)
+    haskey(n.kwargs, :sensealg) || return (Int64(0), Int64(0))
     s = n.kwargs[:sensealg]
-    s isa InterpolatingAdjoint && s.autojacvec isa ZygoteVJP && return Int64(0)
+    s isa InterpolatingAdjoint && s.autojacvec isa ZygoteVJP &&
+        return (Int64(0), Int64(s.checkpointing ? 1 : 0))
     if s isa BacksolveAdjoint && (s.autojacvec === nothing || s.autojacvec isa ZygoteVJP)
-        return s.checkpointing ? Int64(1) : Int64(2)
+        return (s.checkpointing ? Int64(1) : Int64(2), Int64(0))
     end
     return nothing
+end
+
+# GEMM precision (NODE_PREC_*): the hip_precision keyword, else the module default
+const PRECISION = Ref{Symbol}(:fp32)
+const PREC = Dict(:fp32 => Int32(0), :bf16 => Int32(1), :bf16x3 => Int32(2))
+hip_precision(n) = PREC[get(n.kwargs, :hip_precision, PRECISION[])]
+# the layer without the keywords that only this package reads (for the reference path)
+function reference_layer(n::NeuralODE)
+    haskey(n.kwargs, :hip_precision) || return n
+    kw = Base.structdiff(NamedTuple(n.kwargs), NamedTuple{(:hip_precision,)})
+    return NeuralODE(n.model, n.tspan, n.args...; kw...)
 end
 
 supported(n::NeuralODE, x) = chain_spec(n.model) !== nothing && length(n.args) >= 1 &&
@@ -147,10 +186,10 @@ function make_opts(n::NeuralODE)
     return o, interior
 end
 
-const HANDLES = Dict{Tuple{UInt, Int}, Handle}()
+const HANDLES = Dict{Tuple{UInt, Int, Int32}, Handle}()
 
 function handle_for(n::NeuralODE, B::Int; precision::Int32 = Int32(0))
-    key = (objectid(n), B)
+    key = (objectid(n), B, precision)
     get!(HANDLES, key) do
         widths, acts, imap = chain_spec(n.model)
         w = ntuple(i -> i <= length(widths) ? widths[i] : Int32(0), MAX_LAYERS + 1)
@@ -178,6 +217,7 @@ function hip_neuralode(x::AbstractArray{Float32}, p::AbstractVector{Float32}, h:
          Ref{Stats}), h.ptr, o, u0, interior, length(interior),
         cap > 0 ? pointer(out) : C_NULL, cap, st)
     check(h, rc, "node_forward")
+    h.gen += 1
     if o.save_everystep != 0
         out = Array{Float32}(undef, h.D, h.B, st.nsaved)
         check(h, ccall((:node_copy_saved, libnode()), Int32, (Ptr{Cvoid}, Ptr{Float32}, Int64),
@@ -186,12 +226,14 @@ function hip_neuralode(x::AbstractArray{Float32}, p::AbstractVector{Float32}, h:
     ts = Vector{Float64}(undef, st.nsaved)
     check(h, ccall((:node_saved_times, libnode()), Int32, (Ptr{Cvoid}, Ptr{Float64}, Int64),
                    h.ptr, ts, length(ts)), "node_saved_times")
-    return out[:, :, 1:st.nsaved], ts, RETCODES[st.retcode + 1]
+    return out[:, :, 1:st.nsaved], ts, RETCODES[st.retcode + 1], st
 end
 
 function CRC.rrule(::typeof(hip_neuralode), x, p, h::Handle, o::SolveOpts, interior)
-    out, ts, ret = hip_neuralode(x, p, h, o, interior)
+    out, ts, ret, fst = hip_neuralode(x, p, h, o, interior)
+    gen = h.gen
     function hip_neuralode_pullback(Δ)
+        check_current(h, gen, "NeuralODE pullback")
         Δout = unthunk(Δ[1])
         Δout isa Union{NoTangent, ZeroTangent} && (Δout = zero(out))
         g = Array{Float32}(Δout)
@@ -205,74 +247,78 @@ function CRC.rrule(::typeof(hip_neuralode), x, p, h::Handle, o::SolveOpts, inter
         return NoTangent(), reshape(du0, size(x)), Float32.(dp), NoTangent(), NoTangent(),
                NoTangent()
     end
-    return (out, ts, ret), hip_neuralode_pullback
+    return (out, ts, ret, fst), hip_neuralode_pullback
 end
 
-"""
-Solution returned on the HIP path: `.u`, `.t`, `.retcode`, `Array(sol)`,
-`last`, `sum`, indexing -- what downstream code uses of an `ODESolution`
-(docs/src/examples/neural_ode.md:37, GPUs.md:107, cuda_tests.jl:52).
-"""
-struct HIPSolution{A, T}
-    array::A
-    t::T
-    retcode::Symbol
-end
-Base.Array(s::HIPSolution) = s.array
-Base.getproperty(s::HIPSolution, f::Symbol) =
-    f === :u ? [selectdim(getfield(s, :array), ndims(getfield(s, :array)), i)
-                for i in 1:size(getfield(s, :array), ndims(getfield(s, :array)))] :
-    getfield(s, f)
-Base.last(s::HIPSolution) = selectdim(s.array, ndims(s.array), size(s.array, ndims(s.array)))
-Base.sum(s::HIPSolution) = sum(s.array)
-Base.length(s::HIPSolution) = length(s.t)
+const RETCODE_ENUM = (ReturnCode.Default, ReturnCode.Success, ReturnCode.Terminated,
+                      ReturnCode.DtNaN, ReturnCode.MaxIters, ReturnCode.DtLessThanMin,
+                      ReturnCode.Unstable)
 
 # the drop-in: a more specific call method for supported NeuralODEs
 function (n::NeuralODE{<:Chain})(x::AbstractArray{Float32}, p, st)
-    supported(n, x) || return invoke(n, Tuple{Any, Any, Any}, x, p, st)  # reference path
+    modes = sense_mode(n)
+    (supported(n, x) && modes !== nothing) ||
+        return invoke(reference_layer(n), Tuple{Any, Any, Any}, x, p, st)  # reference path
     B = ndims(x) == 1 ? 1 : size(x, 2)
-    h = handle_for(n, B)
+    h = handle_for(n, B; precision = hip_precision(n))
     check(h, ccall((:node_set_option, libnode()), Int32, (Ptr{Cvoid}, Int32, Int64),
-                   h.ptr, NODE_OPT_SENSEALG, sense_mode(n)), "node_set_option")
+                   h.ptr, NODE_OPT_SENSEALG, modes[1]), "node_set_option")
+    check(h, ccall((:node_set_option, libnode()), Int32, (Ptr{Cvoid}, Int32, Int64),
+                   h.ptr, NODE_OPT_CHECKPOINTING, modes[2]), "node_set_option")
     o, interior = make_opts(n)
     # flat Lux/ComponentArray order (test/BasicNeuralDE/neural_de_tests.jl:45); for a
     # ComponentArray `Vector(p)` is already that vector and Zygote restructures dp
     pflat = p isa AbstractVector ? Vector{Float32}(p) :
         reduce(vcat, [vcat(vec(l.weight), l.bias) for l in values(p)
                       if l isa NamedTuple && haskey(l, :weight)])
-    out, ts, ret = hip_neuralode(x, pflat, h, o, interior)
-    arr = ndims(x) == 1 ? reshape(out, size(out, 1), size(out, 3)) : out
-    return HIPSolution(arr, ts, ret), st
+    out, ts, ret, fst = hip_neuralode(x, pflat, h, o, interior)
+    # the ODESolution of src/neural_de.jl:54-60, built from the saved states
+    # (no dense output: sol(t) interpolates linearly between them)
+    u = ndims(x) == 1 ? [out[:, 1, i] for i in axes(out, 3)] :
+                        [out[:, :, i] for i in axes(out, 3)]
+    dudt(u, p, t) = first(n.model(u, p, st))
+    prob = ODEProblem{false}(ODEFunction{false}(dudt), x, n.tspan, p)
+    stats = SciMLBase.DEStats(; nf = fst.nf, naccept = fst.naccept, nreject = fst.nreject)
+    sol = SciMLBase.build_solution(prob, n.args[1], ts, u; retcode = RETCODE_ENUM[ret_index(ret)],
+                                   stats = stats, dense = false)
+    return sol, st
 end
+ret_index(ret::Symbol) = findfirst(==(ret), RETCODES)
 
-# ---- FFJORD (src/ffjord.jl:55-178): monte_carlo = true, no regularisation,
-# standard-normal base distribution, model(z) (NODE_MAP_CNF)
+# ---- FFJORD (src/ffjord.jl:55-178): standard-normal base distribution,
+# model(z) (NODE_MAP_CNF); regularize / monte_carlo = false as input_map flags
 const NODE_MAP_CNF = Int32(2)
+const NODE_MAP_CNF_REGULARIZE = Int32(0x10)
+const NODE_MAP_CNF_EXACT_TRACE = Int32(0x20)
 
 function ffjord_spec(model)
     spec = chain_spec(model)
     spec === nothing && return nothing
     widths, acts, imap = spec
-    (imap == 0 && acts[end] == 0 && length(acts) >= 2 && widths[end] + 1 <= 64) || return nothing
+    (imap == 0 && acts[end] == 0 && length(acts) >= 2 && widths[end] + 3 <= 64) || return nothing
     return widths, acts
 end
 
 supported(n::FFJORD, x, st) = ffjord_spec(n.model) !== nothing && length(n.args) >= 1 &&
     n.args[1] isa Tsit5 && n.basedist === nothing && eltype(x) === Float32 && ndims(x) == 2 &&
-    !st.regularize && st.monte_carlo && n.tspan[2] > n.tspan[1]
+    n.tspan[2] > n.tspan[1]
 
-function ffjord_handle_for(n::FFJORD, B::Int; precision::Int32 = Int32(0))
-    key = (objectid(n), B)
+function ffjord_handle_for(n::FFJORD, B::Int, regularize::Bool, monte_carlo::Bool;
+                           precision::Int32 = Int32(0))
+    imap = NODE_MAP_CNF | (regularize ? NODE_MAP_CNF_REGULARIZE : Int32(0)) |
+           (monte_carlo ? Int32(0) : NODE_MAP_CNF_EXACT_TRACE)
+    key = (objectid(n), B, precision | (imap << 8))
     get!(HANDLES, key) do
         widths, acts = ffjord_spec(n.model)
         w = ntuple(i -> i <= length(widths) ? widths[i] : Int32(0), MAX_LAYERS + 1)
         a = ntuple(i -> i <= length(acts) ? acts[i] : Int32(0), MAX_LAYERS)
-        Handle(NodeDesc(Int32(length(acts)), w, a, NODE_MAP_CNF, precision, Int64(B)))
+        Handle(NodeDesc(Int32(length(acts)), w, a, imap, precision, Int64(B)))
     end
 end
 
 """
-    hip_ffjord(x, p_flat, e, h, opts) -> (logpx (1 x B), z(T) (D x B), delta_logp (1 x B))
+    hip_ffjord(x, p_flat, e, h, opts, regularize) -> (logpx (1 x B), z(T) (D x B),
+                                                       delta_logp (1 x B), λ₁ (1 x B), λ₂ (1 x B))
 
 The FFJORD log-likelihood solve on the GPU (node_cnf_forward) with the
 Hutchinson probe `e` (D x B) held for the whole solve.  Differentiable through
@@ -280,7 +326,7 @@ the `rrule` below (node_cnf_backward: the layer's InterpolatingAdjoint with the
 second-order VJP of the trace term, same probe).
 """
 function hip_ffjord(x::Matrix{Float32}, p::AbstractVector{Float32}, e::Matrix{Float32}, h::Handle,
-                    o::SolveOpts)
+                    o::SolveOpts, regularize::Bool)
     pv = Vector{Float32}(p)
     check(h, ccall((:node_set_params, libnode()), Int32, (Ptr{Cvoid}, Ptr{Float32}, Int64),
                    h.ptr, pv, length(pv)), "node_set_params")
@@ -293,15 +339,31 @@ function hip_ffjord(x::Matrix{Float32}, p::AbstractVector{Float32}, e::Matrix{Fl
                (Ptr{Cvoid}, Ref{SolveOpts}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
                 Ptr{Float32}, Ptr{Float32}, Ref{Stats}), h.ptr, o, x, e, z, dl, lp, st)
     check(h, rc, "node_cnf_forward")
+    h.gen += 1
     st.retcode == 1 || error("FFJORD solve failed: $(RETCODES[st.retcode + 1])")
-    return reshape(lp, 1, B), z, reshape(dl, 1, B)
+    l1, l2 = dl, dl                 # λ₁ = λ₂ = Δlogp without regularisation (:166-167)
+    if regularize
+        l1, l2 = Vector{Float32}(undef, B), Vector{Float32}(undef, B)
+        check(h, ccall((:node_cnf_regularizers, libnode()), Int32,
+                       (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}), h.ptr, l1, l2),
+              "node_cnf_regularizers")
+    end
+    return reshape(lp, 1, B), z, reshape(dl, 1, B), reshape(l1, 1, B), reshape(l2, 1, B)
 end
 
-function CRC.rrule(::typeof(hip_ffjord), x, p, e, h::Handle, o::SolveOpts)
-    lp, z, dl = hip_ffjord(x, p, e, h, o)
+function CRC.rrule(::typeof(hip_ffjord), x, p, e, h::Handle, o::SolveOpts, regularize::Bool)
+    lp, z, dl, l1, l2 = hip_ffjord(x, p, e, h, o, regularize)
+    gen = h.gen
     function hip_ffjord_pullback(Δ)
+        check_current(h, gen, "FFJORD pullback")
         tang(t, like) = (t = unthunk(t); t isa Union{NoTangent, ZeroTangent} ? zero(like) : t)
         Δlp, Δz, Δdl = tang(Δ[1], lp), tang(Δ[2], z), tang(Δ[3], dl)
+        if regularize
+            iszero(tang(Δ[4], l1)) && iszero(tang(Δ[5], l2)) ||
+                error("FFJORD: gradients through the regularisers (λ₁, λ₂) are not on the HIP path")
+        else  # λ₁ = λ₂ = Δlogp
+            Δdl = Δdl .+ tang(Δ[4], l1) .+ tang(Δ[5], l2)
+        end
         # logpx = -|z|^2 / 2 - D/2 log(2π) - delta_logp
         gz = Matrix{Float32}(Δz .- Δlp .* z)
         gd = Vector{Float32}(vec(Δdl .- Δlp))
@@ -312,9 +374,9 @@ function CRC.rrule(::typeof(hip_ffjord), x, p, e, h::Handle, o::SolveOpts)
                    (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float64}, Ref{Stats}),
                    h.ptr, gz, gd, dx, dp, st)
         check(h, rc, "node_cnf_backward")
-        return NoTangent(), dx, Float32.(dp), NoTangent(), NoTangent(), NoTangent()
+        return NoTangent(), dx, Float32.(dp), NoTangent(), NoTangent(), NoTangent(), NoTangent()
     end
-    return (lp, z, dl), hip_ffjord_pullback
+    return (lp, z, dl, l1, l2), hip_ffjord_pullback
 end
 
 function make_opts(n::FFJORD)
@@ -331,13 +393,13 @@ end
 function (n::FFJORD{<:Chain})(x::AbstractMatrix{Float32}, ps, st)
     supported(n, x, st) || return invoke(n, Tuple{Any, Any, Any}, x, ps, st)  # reference path
     D, B = size(x)
-    h = ffjord_handle_for(n, B)
+    h = ffjord_handle_for(n, B, st.regularize, st.monte_carlo)
     pflat = ps isa AbstractVector ? Vector{Float32}(ps) :
         reduce(vcat, [vcat(vec(l.weight), l.bias) for l in values(ps)
                       if l isa NamedTuple && haskey(l, :weight)])
     e = CRC.@ignore_derivatives randn(Float32, D, B)
-    lp, _, dl = hip_ffjord(Matrix{Float32}(x), pflat, e, h, make_opts(n))
-    return (lp, dl, dl), (; model = st.model, regularize = st.regularize,
+    lp, _, dl, l1, l2 = hip_ffjord(Matrix{Float32}(x), pflat, e, h, make_opts(n), st.regularize)
+    return (lp, l1, l2), (; model = st.model, regularize = st.regularize,
                           monte_carlo = st.monte_carlo)
 end
 

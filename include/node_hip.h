@@ -51,6 +51,21 @@ enum { NODE_MAP_IDENTITY = 0, NODE_MAP_CUBE = 1,
  * [z; t] (NODE_MAP_CNF_TIME, widths[0] == D + 1) to dz/dt; the ODE state is
  * [z; log p] (D + 1 <= 64 entries per sample), the last layer is linear. */
        NODE_MAP_CNF = 2, NODE_MAP_CNF_TIME = 3 };
+/* flags OR-ed onto NODE_MAP_CNF / NODE_MAP_CNF_TIME (the layer state of
+ * src/ffjord.jl:138-178, regularize and monte_carlo):
+ *  NODE_MAP_CNF_REGULARIZE : regularize = true (src/ffjord.jl:131-132,
+ *    149-169): the ODE state is [z; log p; r1; r2] with r1' = |f|^2 and
+ *    r2' = |e^T df/dz| per sample (D + 3 <= 64); node_cnf_regularizers gives
+ *    r1(T), r2(T) (the layer's lambda_1, lambda_2; the reference selects them
+ *    with selectdim(pred, N, ...), the batch dimension, which for a matrix picks
+ *    two sample columns -- the rows are what the regularisers are).  The
+ *    adjoint takes no cotangents for them (zero: they carry no gradient).
+ *  NODE_MAP_CNF_EXACT_TRACE : monte_carlo = false (src/ffjord.jl:116-126): the
+ *    trace of df/dz exactly, from D unit-probe passes per right-hand side
+ *    (and D tangent chains per adjoint VJP); the probe argument of
+ *    node_cnf_forward is then only read for the r2 regulariser (e^T J, the
+ *    reference's randn! draw) and may be NULL without regularize. */
+enum { NODE_MAP_CNF_REGULARIZE = 0x10, NODE_MAP_CNF_EXACT_TRACE = 0x20 };
 /* GEMM precision of the right-hand side and its VJPs.  State, stage sums and
  * adjoint state stay FP32 in all modes.
  *  FP32   : FP32 operands on FP32 MFMA.
@@ -166,6 +181,12 @@ int32_t node_cnf_forward(node_handle_t* h, const node_solve_opts* opts, const fl
 int32_t node_cnf_backward(node_handle_t* h, const float* dL_dz, const float* dL_ddlogp,
                           float* dL_dx, double* dL_dp, node_stats* stats);
 
+/* r1(T) and r2(T) per sample (B each) of the last node_cnf_forward on a
+ * NODE_MAP_CNF_REGULARIZE handle: the integrated |f|^2 and |e^T df/dz| -- the
+ * lambda_1, lambda_2 outputs of (n::FFJORD)(x, ps, st) with regularize = true
+ * (src/ffjord.jl:161-166).  Either may be NULL. */
+int32_t node_cnf_regularizers(node_handle_t* h, float* lam1, float* lam2);
+
 /* Times of the states saved by the last forward (n = stats->nsaved). */
 int32_t node_saved_times(node_handle_t* h, double* t_out, int64_t cap);
 /* Copy the saved states of the last forward (D x B x n) to host memory. */
@@ -256,8 +277,28 @@ int32_t node_ragged_backward(node_handle_t* h, const float* dLdout, float* dLdu0
  *                            back from u(T) with the adjoint, reset to the forward's
  *                            saved state at each saved time; NODE_SENSE_BACKSOLVE_NOCKPT
  *                            BacksolveAdjoint(checkpointing = false). */
+/*   NODE_OPT_CHECKPOINTING   1: InterpolatingAdjoint(checkpointing = true) (the
+ *                            reference re-exports it, src/DiffEqFlux.jl:53; the
+ *                            sensealg keyword at src/neural_de.jl:57).  The next
+ *                            node_forward keeps no step record, only the checkpoint
+ *                            states (u0, the saved states, u(t1)) and a ring of four
+ *                            tape entries; node_backward re-solves the forward over
+ *                            each interval between consecutive checkpoints (t0, the
+ *                            saved times inside the span, t1), last interval first,
+ *                            from its checkpoint state with the forward's tolerances
+ *                            (initial dt: the heuristic, then the last step of the
+ *                            previous re-solve), and integrates the adjoint over it
+ *                            with u(t) from that re-solve.  Device memory then
+ *                            depends on the longest interval, not on the whole
+ *                            solve.  Not with save_everystep; not on a CNF handle;
+ *                            applies to NODE_SENSE_INTERPOLATING only.
+ *   NODE_OPT_TAPE_BUDGET     bytes the forward tape may take (0: unlimited, the
+ *                            default).  A solve whose tape would exceed it fails with
+ *                            NODE_ERR_CAPACITY (with checkpointing, only an interval
+ *                            re-solve can need more than a few entries). */
 enum { NODE_OPT_GRAPHS = 1, NODE_OPT_GRAPH_ATTEMPTS = 2, NODE_OPT_AUTOTUNE = 3,
-       NODE_OPT_WGRAD_PRECISION = 4, NODE_OPT_SENSEALG = 5 };
+       NODE_OPT_WGRAD_PRECISION = 4, NODE_OPT_SENSEALG = 5, NODE_OPT_CHECKPOINTING = 6,
+       NODE_OPT_TAPE_BUDGET = 7 };
 enum { NODE_SENSE_INTERPOLATING = 0, NODE_SENSE_BACKSOLVE = 1, NODE_SENSE_BACKSOLVE_NOCKPT = 2 };
 int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value);
 /* Tile plan chosen for a GEMM role (roles as for node_prof_enable below) and

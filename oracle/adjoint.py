@@ -51,12 +51,75 @@ def forward(mlp, p, u0, t0, t1, saveat=None, save_everystep=None,
                        save_end=se, save_everystep=sev, **solver_kw)
 
 
-def backward(mlp, p, fwd, dL_du, **solver_kw):
+class Checkpoints:
+    """``u(t)`` for ``InterpolatingAdjoint(checkpointing = true)``, restated
+    (third-party: SciMLSensitivity >= 7.112, the checkpointed branch of
+    ``ODEInterpolatingAdjointSensitivityFunction``; the flag is re-exported by
+    the reference at ``src/DiffEqFlux.jl:53`` and reaches the solve through
+    the ``sensealg`` keyword, ``src/neural_de.jl:57``).
+
+    Checkpoints: ``t0``, the forward's saved times strictly inside the span,
+    ``t1``, with their states (``u0``, the saved states, ``u(t1)``); nothing
+    else of the forward solution is used.  The adjoint starts on the last
+    interval, re-solved from its checkpoint state with the forward's
+    tolerances (initial dt by the heuristic).  ``u(t)`` comes from the current
+    interval's solution while ``t`` is not below the interval's start (at the
+    start itself: the checkpoint state); a time below it moves to the interval
+    holding ``t``, re-solved from its checkpoint with the last step of the
+    previous re-solve as initial dt.  The initial-dt probe of the adjoint
+    (``probe``) reads the current interval's interpolant wherever it falls.
+
+    The exact checkpoint placement and re-solve options of SciMLSensitivity
+    need Julia (absent): parity unpinned for those, pinned to solver
+    tolerance (tests/test_oracle_adjoint.py: closed forms on linear ODEs,
+    fixed-dt identity with the dense adjoint, agreement with it at tolerance,
+    a planted wrong checkpoint).
+    """
+
+    def __init__(self, mlp, p, fwd, abstol=1e-6, reltol=1e-3, adaptive=True, dt=None,
+                 maxiters=100000, **_):
+        self.f = lambda y, t: mlp.forward(y, p)
+        t0, t1 = fwd.t0, fwd.t1
+        u0 = fwd.steps[0].y0 if fwd.steps else fwd.u[0]
+        self.t = [t0] + [t for t in fwd.t if t0 < t < t1] + [t1]
+        self.u = [u0] + [u for t, u in zip(fwd.t, fwd.u) if t0 < t < t1] + [fwd.interp(t1)]
+        self.kw = dict(abstol=abstol, reltol=reltol, adaptive=adaptive, maxiters=maxiters)
+        self.fixed_dt = None if adaptive else dt
+        self.nf = 0
+        self.resolves = []
+        self.k = len(self.t) - 2
+        self.cur = self._resolve(self.k, None)
+
+    def _resolve(self, k, dt):
+        if self.fixed_dt is not None:
+            dt = self.fixed_dt
+        s = tsit5.solve(self.f, self.u[k], self.t[k], self.t[k + 1], saveat_interior=(),
+                        save_start=False, save_end=True, dt=dt, **self.kw)
+        if s.retcode != tsit5.RET_SUCCESS:
+            raise RuntimeError(f"checkpoint re-solve of [{self.t[k]}, {self.t[k + 1]}] failed")
+        self.nf += s.nf
+        self.resolves.append(k)
+        return s
+
+    def __call__(self, t, probe=False):
+        if not probe and t < self.t[self.k]:
+            last = abs(self.cur._times[-1] - self.cur._times[-2])
+            while t < self.t[self.k]:
+                self.k -= 1
+            self.cur = self._resolve(self.k, last)
+        return self.cur.interp(t)
+
+
+def backward(mlp, p, fwd, dL_du, checkpointing=False, **solver_kw):
     """InterpolatingAdjoint backward pass.
 
     ``fwd``: the forward Solution (its ``t`` are the saved times);
     ``dL_du``: cotangents, one (B, D) array per saved time.
-    Returns (dL/du0 (B, D), dL/dp (P,), adjoint Solution)."""
+    ``checkpointing``: ``u(t)`` from interval re-solves (:class:`Checkpoints`)
+    instead of the forward's dense record (forward times increasing only).
+    Returns (dL/du0 (B, D), dL/dp (P,), adjoint Solution); with
+    checkpointing the Solution carries ``ck_nf`` (re-solve RHS evaluations)
+    and ``ck_resolves`` (interval indices in order)."""
     u_shape = np.shape(fwd.u[0]) if fwd.u else np.shape(fwd.steps[0].y0)
     nlam = int(np.prod(u_shape))
     P = mlp.nparams
@@ -67,9 +130,11 @@ def backward(mlp, p, fwd, dL_du, **solver_kw):
         jumps[t] = jumps.get(t, 0) + g
     dtype = np.result_type(fwd.steps[0].y0 if fwd.steps else fwd.u[0], p)
 
-    def f(y, t):
+    ck = Checkpoints(mlp, p, fwd, **solver_kw) if checkpointing else None
+
+    def rhs(y, t, probe=False):
         lam = y[:nlam].reshape(u_shape)
-        u = fwd.interp(t)
+        u = ck(t, probe) if ck is not None else fwd.interp(t)
         gx, gp = mlp.vjp(u, p, lam)
         return np.concatenate([-gx.reshape(-1), -gp]).astype(dtype, copy=False)
 
@@ -81,10 +146,12 @@ def backward(mlp, p, fwd, dL_du, **solver_kw):
     y0 = np.zeros(nlam + P, dtype=dtype)
     kw = dict(solver_kw)
     kw.pop("saveat", None)
-    adj = tsit5.solve(f, y0, fwd.t1, fwd.t0, saveat_interior=(),
+    adj = tsit5.solve(rhs, y0, fwd.t1, fwd.t0, saveat_interior=(),
                       save_start=False, save_end=True, save_everystep=False,
                       callback_times=sorted(jumps), affect=affect,
-                      record_dense=False, **kw)
+                      record_dense=False, initdt_f=lambda y, t: rhs(y, t, True), **kw)
+    if ck is not None:
+        adj.ck_nf, adj.ck_resolves = ck.nf, ck.resolves
     if not adj.u:  # the adjoint solve failed before T -> t0 (see adj.retcode)
         return np.full(u_shape, np.nan), np.full(P, np.nan), adj
     yT = adj.u[-1]

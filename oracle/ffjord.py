@@ -16,11 +16,21 @@ Follows ``src/ffjord.jl``:
   * ``__forward_ffjord`` (:138-178): solve from ``[x; 0]`` saving only the end
     state, ``logpz = -|z|²/2 - D/2 log 2π`` (standard-normal base), and
     ``logpx = logpz - Δlogp``.
+  * ``regularize`` (``:131-132``, ``:149-169``): two more state rows per
+    sample, ``r1' = ‖f‖²`` and ``r2' = ‖eᵀ ∂f/∂z‖`` (``__norm_batched``), whose
+    values at T are the layer's ``λ₁``, ``λ₂``.  (The reference selects them
+    with ``selectdim(pred, N, ...)``, the batch dimension, which for a matrix
+    input picks two sample columns rather than the two regulariser rows; the
+    rows -- what the regularisers are -- are returned here.)
   * ``backward``: the InterpolatingAdjoint of that solve (``:143``) with the
-    second-order VJP of the Hutchinson term (``trace_vjp``).  Pinned by torch
-    FP64 double-backward of the same augmented right-hand side, central finite
-    differences of ``forward``'s loss, and a planted mistake (the
-    second-order source dropped) that the finite differences catch.
+    second-order VJP of the Hutchinson term (``trace_vjp``) or of the exact
+    trace and the regularisers (``rhs_vjp``: bilinear forms ``aᵀ J b`` through
+    one tangent chain per pair).  Pinned by torch FP64 double-backward of the
+    same augmented right-hand side (every combination of ``monte_carlo`` and
+    ``regularize``), central finite differences of ``forward``'s loss, closed
+    forms on a linear flow (``Δlogp = -tr(A) T``, ``r2(T) = T ‖Aᵀe‖``,
+    ``r1(T) = ∫‖A e^{At} x‖²``), and a planted mistake (the second-order
+    source dropped) that the finite differences catch.
 """
 import math
 
@@ -65,15 +75,56 @@ def exact_trace_rhs(mlp, p, D, time_concat=False):
     return f
 
 
-def forward(mlp, p, x, t0, t1, probe=None, time_concat=False, **solver_kw):
-    """-> (logpx (B,), z(T) (B, D), Δlogp(T) (B,), Solution)."""
+def ffjord_rhs(mlp, p, D, probe=None, time_concat=False, regularize=False, monte_carlo=True):
+    """The augmented right-hand side of ``__ffjord`` (``src/ffjord.jl:81-134``)
+    on the state ``[z; Δlogp]`` (``regularize``: ``[z; Δlogp; r1; r2]``):
+    ``[f; -tr(∂f/∂z); (‖f‖²; ‖eᵀ ∂f/∂z‖)]`` per sample (rows of ``y``).
+    ``monte_carlo``: Hutchinson ``eᵀJe`` with the fixed ``probe`` (``:95-115``);
+    else the exact trace of the Jacobian (``:116-126``: the reference's batched
+    Jacobian, here its D rows ``e_dᵀJ`` from D unit-probe VJPs), whose
+    regulariser row ``eᵀJ`` uses ``probe`` as the reference's ``randn!`` draw."""
+    def f(y, t):
+        z = y[:, :D]
+        x = model_input(z, t, time_concat)
+        fz = mlp.forward(x, p)
+        if monte_carlo:
+            ej, _ = mlp.vjp(x, p, probe.astype(y.dtype))
+            w = ej[:, :D]
+            tr = np.sum(w * probe, axis=1)
+        else:
+            tr = np.zeros(z.shape[0], dtype=y.dtype)
+            w = np.zeros_like(z)
+            for d in range(D):
+                e = np.zeros_like(z)
+                e[:, d] = 1.0
+                jd, _ = mlp.vjp(x, p, e)
+                tr += jd[:, d]
+                if regularize:
+                    w += probe[:, d:d + 1].astype(y.dtype) * jd[:, :D]
+        cols = [fz, -tr[:, None]]
+        if regularize:
+            cols += [np.sum(fz * fz, axis=1)[:, None], np.linalg.norm(w, axis=1)[:, None]]
+        return np.concatenate(cols, axis=1)
+    return f
+
+
+def forward(mlp, p, x, t0, t1, probe=None, time_concat=False, regularize=False,
+            monte_carlo=None, **solver_kw):
+    """-> (logpx (B,), z(T) (B, D), Δlogp(T) (B,), Solution); with
+    ``regularize`` the Solution carries ``lam1``, ``lam2`` (B,) (the
+    regulariser rows at T).  ``monte_carlo`` defaults to ``probe is not None``
+    (no probe: exact trace)."""
     B, D = x.shape
-    f = (hutchinson_rhs(mlp, p, probe, time_concat) if probe is not None
-         else exact_trace_rhs(mlp, p, D, time_concat))
-    y0 = np.concatenate([x, np.zeros((B, 1), dtype=x.dtype)], axis=1)
+    if monte_carlo is None:
+        monte_carlo = probe is not None
+    f = ffjord_rhs(mlp, p, D, probe, time_concat, regularize, monte_carlo)
+    extra = 3 if regularize else 1
+    y0 = np.concatenate([x, np.zeros((B, extra), dtype=x.dtype)], axis=1)
     sol = tsit5.solve(f, y0, t0, t1, save_start=False, save_end=True, **solver_kw)
     yT = np.asarray(sol.u[-1])
     z, dl = yT[:, :D], yT[:, D]
+    if regularize:
+        sol.lam1, sol.lam2 = yT[:, D + 1], yT[:, D + 2]
     logpz = -0.5 * np.sum(z * z, axis=1) - 0.5 * D * math.log(2 * math.pi)
     return logpz - dl, z, dl, sol
 
@@ -149,7 +200,83 @@ def trace_vjp(mlp, p, x, probe, lam_z, lam_d):
     return g, gp
 
 
-def backward(mlp, p, fwd, probe, dL_dz, dL_ddlogp, time_concat=False, **solver_kw):
+def bilinear_vjp(mlp, p, x, D, pairs, seed):
+    """VJP of ``Σ_b seed_b·f(x_b) + Σ_pairs Σ_b a_bᵀ J_b c_b`` (``J = ∂f/∂z``,
+    ``z`` the first D input columns) for per-sample vectors ``(a, c)``
+    (B, D): each pair runs one tangent chain seeded with ``c`` and its
+    reverse seeded with ``a`` (``trace_vjp`` is the pair ``(-λ_Δ e, e)``).
+    Returns (∂/∂x (B, in), ∂/∂p (P,))."""
+    from .mlp import act_fwd, act_deriv
+    layers = mlp.unflatten(p)
+    L = len(layers)
+    h, hs, As = x, [x], []
+    for (Wt, b), act in zip(layers, mlp.acts):
+        a = h @ Wt + b
+        h = act_fwd(act, a)
+        As.append(a)
+        hs.append(h)
+    gp = np.zeros(mlp.nparams, dtype=np.result_type(x, p, seed))
+    src = [np.zeros_like(a) for a in As]
+    for left, right in pairs:
+        t = np.zeros_like(x)
+        t[:, :D] = right
+        ts, adots = [t], []
+        for li, ((Wt, _), act) in enumerate(zip(layers, mlp.acts)):
+            ad = ts[-1] @ Wt
+            adots.append(ad)
+            ts.append(act_deriv(act, As[li], hs[li + 1]) * ad)
+        tbar = left
+        for li in range(L - 1, -1, -1):
+            Wt, _ = layers[li]
+            act = mlp.acts[li]
+            abar_t = tbar * act_deriv(act, As[li], hs[li + 1])
+            src[li] = src[li] + tbar * adots[li] * _act_d2(act, As[li])
+            w0, b0, _ = mlp.offsets[li]
+            gp[w0:b0] += (ts[li].T @ abar_t).ravel()
+            tbar = abar_t @ Wt.T
+    g = seed
+    for li in range(L - 1, -1, -1):
+        Wt, _ = layers[li]
+        act = mlp.acts[li]
+        abar = g * act_deriv(act, As[li], hs[li + 1]) + src[li]
+        w0, b0, e = mlp.offsets[li]
+        gp[w0:b0] += (hs[li].T @ abar).ravel()
+        gp[b0:e] += abar.sum(axis=0)
+        g = abar @ Wt.T
+    return g, gp
+
+
+def rhs_vjp(mlp, p, x, D, probe, lam_z, lam_d, lam_r1=None, lam_r2=None, monte_carlo=True):
+    """VJP of the augmented right-hand side ``ffjord_rhs`` at the model input
+    ``x`` with cotangents ``lam_z`` (B, D), ``lam_d`` (B,) and, with the
+    regularisers, ``lam_r1``, ``lam_r2`` (B,): the trace term is the pair
+    ``(-λ_Δ e, e)`` (Hutchinson) or ``(-λ_Δ e_d, e_d)`` for every unit probe
+    (exact); ``λ_r1 ‖f‖²`` adds ``2 λ_r1 f`` to the primal seed; ``λ_r2 ‖w‖``
+    with ``w = eᵀJ`` is the pair ``(e, λ_r2 w / ‖w‖)``."""
+    B = x.shape[0]
+    seed = np.array(lam_z, dtype=np.result_type(x, p, lam_z))
+    c = -np.asarray(lam_d)[:, None]
+    if monte_carlo:
+        pairs = [(c * probe, probe)]
+    else:
+        pairs = []
+        for d in range(D):
+            e = np.zeros((B, D), dtype=seed.dtype)
+            e[:, d] = 1.0
+            pairs.append((c * e, e))
+    if lam_r1 is not None and np.any(lam_r1 != 0):
+        seed = seed + 2.0 * np.asarray(lam_r1)[:, None] * mlp.forward(x, p)
+    if lam_r2 is not None and np.any(lam_r2 != 0):
+        ej, _ = mlp.vjp(x, p, probe.astype(seed.dtype))
+        w = ej[:, :D]
+        nw = np.linalg.norm(w, axis=1)
+        pairs.append((probe.astype(seed.dtype),
+                      np.asarray(lam_r2)[:, None] * w / np.where(nw > 0, nw, 1.0)[:, None]))
+    return bilinear_vjp(mlp, p, x, D, pairs, seed)
+
+
+def backward(mlp, p, fwd, probe, dL_dz, dL_ddlogp, time_concat=False, regularize=False,
+             monte_carlo=True, dL_dlam1=None, dL_dlam2=None, **solver_kw):
     """InterpolatingAdjoint of the FFJORD solve (the layer's default
     ``sensealg``, ``src/ffjord.jl:143``): the adjoint state
     ``[λ_z; λ_Δ; μ]`` solves, from T back to t0,
@@ -159,21 +286,35 @@ def backward(mlp, p, fwd, probe, dL_dz, dL_ddlogp, time_concat=False, **solver_k
     B, D = probe.shape
     P = mlp.nparams
     dtype = np.result_type(fwd.steps[0].y0, p)
+    nc = 3 if regularize else 1           # constant adjoint columns per sample
+    z0 = np.zeros(B, dtype)
+    l1 = np.asarray(dL_dlam1, dtype).reshape(-1) if dL_dlam1 is not None else z0
+    l2 = np.asarray(dL_dlam2, dtype).reshape(-1) if dL_dlam2 is not None else z0
 
     def f(y, t):
         lz = y[:B * D].reshape(B, D)
         ld = y[B * D:B * D + B]
         z = np.asarray(fwd.interp(t))[:, :D]
-        gx, gp = trace_vjp(mlp, p, model_input(z, t, time_concat), probe.astype(dtype), lz, ld)
-        return np.concatenate([-gx[:, :D].reshape(-1), np.zeros(B, dtype), -gp]).astype(dtype)
+        x = model_input(z, t, time_concat)
+        if regularize or not monte_carlo:
+            gx, gp = rhs_vjp(mlp, p, x, D, probe.astype(dtype), lz, ld,
+                             y[B * D + B:B * D + 2 * B] if regularize else None,
+                             y[B * D + 2 * B:B * D + 3 * B] if regularize else None,
+                             monte_carlo)
+        else:
+            gx, gp = trace_vjp(mlp, p, x, probe.astype(dtype), lz, ld)
+        return np.concatenate([-gx[:, :D].reshape(-1), np.zeros(nc * B, dtype),
+                               -gp]).astype(dtype)
 
-    y0 = np.concatenate([np.asarray(dL_dz, dtype).reshape(-1),
-                         np.asarray(dL_ddlogp, dtype).reshape(-1), np.zeros(P, dtype)])
+    cols = [np.asarray(dL_dz, dtype).reshape(-1), np.asarray(dL_ddlogp, dtype).reshape(-1)]
+    if regularize:
+        cols += [l1, l2]
+    y0 = np.concatenate(cols + [np.zeros(P, dtype)])
     kw = dict(solver_kw)
     adj = tsit5.solve(f, y0, fwd.t1, fwd.t0, saveat_interior=(), save_start=False,
                       save_end=True, save_everystep=False, record_dense=False, **kw)
     yT = adj.u[-1]
-    return yT[:B * D].reshape(B, D), yT[B * D + B:], adj
+    return yT[:B * D].reshape(B, D), yT[B * D + nc * B:], adj
 
 
 def nll_cotangents(z, B):

@@ -46,6 +46,18 @@ def teacher_rhs(D, seed):
     return lambda u, t=None: u @ A.T + np.tanh(u @ C.T)
 
 
+def teacher_targets(u0, ts, seed=1, abstol=1e-12, reltol=1e-12):
+    """Saved states of the teacher ``u' = A u + tanh(C u)`` (``teacher_rhs``
+    with the given seed) at the times ``ts`` from ``u0`` (B, D), solved by the
+    FP64 oracle Tsit5 at tight tolerance: (len(ts), B, D)."""
+    from . import tsit5
+    f = teacher_rhs(u0.shape[1], seed)
+    s = tsit5.solve(f, np.asarray(u0, np.float64), ts[0], ts[-1], saveat_interior=list(ts[1:-1]),
+                    save_start=True, save_end=True, abstol=abstol, reltol=reltol,
+                    record_dense=False)
+    return np.array(s.u)
+
+
 def mlp_problem(D, hidden, n_hidden, act, B, n_save, reltol, abstol,
                 dtype=np.float64, seed=0, u_seed=1):
     widths = [D] + [hidden] * n_hidden + [D]

@@ -182,14 +182,16 @@ def initdt(f, y0, f0, t, tdir, dtmax_abs, abstol, reltol, dtmin_opt):
 def solve(f, y0, t0, t1, *, abstol=1e-6, reltol=1e-3, adaptive=True, dt=None,
           dtmax=None, dtmin=None, maxiters=100000, saveat_interior=(),
           save_start=True, save_end=True, save_everystep=False, tstops=(),
-          callback_times=(), affect=None, record_dense=True):
+          callback_times=(), affect=None, record_dense=True, initdt_f=None):
     """Integrate ``y' = f(y, t)`` from t0 to t1 with Tsit5.
 
     ``f(y, t)`` must return an array shaped like ``y``.  ``affect(y, t)`` is a
     discrete callback applied when the integrator lands on one of
     ``callback_times`` (each is also a tstop); it returns the new state and
     marks u as modified (FSAL re-evaluated), as SciMLSensitivity's
-    IterativeCallback does for saveat cotangents.
+    IterativeCallback does for saveat cotangents.  ``initdt_f`` (default
+    ``f``) evaluates the initial-dt heuristic's probe ``f(y0 + dt0 f0, t0 +
+    dt0)``.
     """
     t0, t1 = float(t0), float(t1)
     tdir = 1.0 if t1 > t0 else -1.0
@@ -223,7 +225,7 @@ def solve(f, y0, t0, t1, *, abstol=1e-6, reltol=1e-3, adaptive=True, dt=None,
     fsalfirst = f(u, t)
     sol.nf += 1
     if adaptive and dt is None:
-        dt, extra = initdt(f, u, fsalfirst, t, tdir, abs(dtmax), abstol, reltol,
+        dt, extra = initdt(initdt_f or f, u, fsalfirst, t, tdir, abs(dtmax), abstol, reltol,
                            dtmin)
         sol.nf += 1 + extra   # reference: f(u0) once in initialize!, once here
     else:

@@ -206,44 +206,51 @@ REF_KWARGS = [  # test/BasicNeuralDE/neural_de_tests.jl:27-42
 ]
 
 
+@pytest.mark.parametrize("aug", [False, True])
 @pytest.mark.parametrize("vec", [True, False])
 @pytest.mark.parametrize("ki", range(len(REF_KWARGS)))
-def test_reference_kwargs_matrix(ki, vec):
-    """The reference's NeuralODE test matrix through the layer API and torch
-    autograd: grads non-zero (the reference's own assertion) and equal to the
-    oracle's (BacksolveAdjoint against the oracle's BacksolveAdjoint);
-    TrackerAdjoint, off the HIP path, raises NotImplementedError."""
+def test_reference_kwargs_matrix(ki, vec, aug):
+    """The reference's NeuralODE test matrix (test/BasicNeuralDE/
+    neural_de_tests.jl:26-55) through the layer API and torch autograd, for
+    NeuralODE(Chain(Dense(2, 50, tanh), Dense(50, 2))) and for
+    AugmentedNDELayer(NeuralODE(Chain(Dense(4, 50, tanh), Dense(50, 4))), 2):
+    grads non-zero (the reference's own assertion) and equal to the oracle's
+    on the same problem (BacksolveAdjoint against the oracle's
+    BacksolveAdjoint; the augmented layer's dL/dx = the first 2 rows of the
+    augmented problem's dL/du0); TrackerAdjoint, off the HIP path, raises
+    NotImplementedError.  The abstol = reltol = 1e-12 row solves in FP32 state
+    like the reference's Float32 problem: the error estimate's rounding noise
+    scales with dt, so the solve succeeds with very small steps (the FP32
+    oracle takes ~5e4 on the plain layer)."""
     import torch
     import diffeqflux_hip as dfh
     kw = dict(REF_KWARGS[ki])
     if "sensealg" in kw:
         kw["sensealg"] = getattr(dfh, kw["sensealg"])()
-    model = dfh.Chain(dfh.Dense(2, 50, "tanh"), dfh.Dense(50, 2))
+    D = 4 if aug else 2
+    model = dfh.Chain(dfh.Dense(D, 50, "tanh"), dfh.Dense(50, D))
     node = dfh.NeuralODE(model, (0.0, 1.0), dfh.Tsit5(), **kw)
-    ps, st = node.setup(np.random.default_rng(0))
+    layer = dfh.AugmentedNDELayer(node, 2) if aug else node
+    ps, st = layer.setup(np.random.default_rng(0))
     x = np.array([2.0, 0.0], np.float32) if vec else \
         np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]], np.float32)
-    print(f"\n[kwargs #{ki} vec={vec}]", flush=True)
-    if ki == 1:
-        pytest.skip("abstol=reltol=1e-12 is below the FP32 rounding floor of the error "
-                    "estimate: an FP32-state solve (the reference's Float32 one included) "
-                    "runs to MaxIters")
+    print(f"\n[kwargs #{ki} vec={vec} augmented={aug}]", flush=True)
     xt = torch.tensor(x, requires_grad=True)
     pt = torch.tensor(ps, requires_grad=True)
     if isinstance(kw.get("sensealg"), dfh.TrackerAdjoint):
         with pytest.raises(NotImplementedError):
-            node(xt, pt, st)
+            layer(xt, pt, st)
         return
     backsolve = isinstance(kw.get("sensealg"), dfh.BacksolveAdjoint)
-    sol, _ = node(xt, pt, st)
+    sol, _ = layer(xt, pt, st)
     loss = sol.array.sum()                      # sum o first o node
     loss.backward()
     assert xt.grad.abs().sum() > 0 and pt.grad.abs().sum() > 0
-    m = MLP([2, 50, 2], ["tanh", "identity"])
+    m = MLP([D, 50, D], ["tanh", "identity"])
     okw = {k: v for k, v in kw.items() if k != "sensealg"}
     B = 1 if vec else 3
-    fwd = OA.forward(m, ps.astype(np.float64), x.reshape(B, 2).astype(np.float64), 0.0, 1.0,
-                     **okw)
+    xa = np.concatenate([x.reshape(B, 2), np.zeros((B, D - 2), np.float32)], 1)
+    fwd = OA.forward(m, ps.astype(np.float64), xa.astype(np.float64), 0.0, 1.0, **okw)
     if okw:
         assert np.array_equal(np.array(fwd.t), sol.t)
     elif len(fwd.t) != len(sol.t) or not np.allclose(fwd.t, sol.t, rtol=0, atol=1e-5):
@@ -252,12 +259,16 @@ def test_reference_kwargs_matrix(ki, vec):
         return
     tol = {k: okw.pop(k) for k in ("abstol", "reltol") if k in okw}
     back = OA.backward_backsolve if backsolve else OA.backward
-    gu0, gp, _ = back(m, ps.astype(np.float64), fwd, [np.ones((B, 2))] * len(fwd.t), **tol)
-    eu = rel(xt.grad.numpy().reshape(B, 2), gu0)
+    gu0, gp, _ = back(m, ps.astype(np.float64), fwd, [np.ones((B, D))] * len(fwd.t), **tol)
+    eu = rel(xt.grad.numpy().reshape(B, 2), gu0[:, :2])
     ep = rel(pt.grad.numpy(), gp)
-    print(f"  retcode {sol.retcode} Nt {len(sol.t)} grad rel err du0 {eu:.2e} dp {ep:.2e}",
-          flush=True)
-    assert eu < 5e-3 and ep < 5e-3
+    print(f"  retcode {sol.retcode} Nt {len(sol.t)} steps {sol.stats['naccept']}+"
+          f"{sol.stats['nreject']} grad rel err du0 {eu:.2e} dp {ep:.2e}", flush=True)
+    assert sol.retcode == "Success"
+    # abstol = reltol = 1e-12: ~6e4 FP32 steps each way, whose rounding adds up
+    # to ~1e-2 of the gradient (the reference's Float32 solve carries the same)
+    bar = 1e-1 if ki == 1 else 5e-3
+    assert eu < bar and ep < bar
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])

@@ -39,3 +39,29 @@ def test_augment_appends_zero_rows():
     assert yv.shape == (4,)
     yv.sum().backward()
     assert torch.equal(xv.grad, torch.ones(2))
+
+
+def test_odesolution_consumers():
+    # the ODESolution consumers of the reference docs: Array(sol)
+    # (docs/src/examples/neural_ode.md:37), reduce(hcat, sol.u)
+    # (docs/src/examples/GPUs.md:107), sol.u[end] (mnist_neural_ode.md:43),
+    # last / sum (test/CUDA/cuda_tests.jl:52, test/BasicNeuralDE/neural_de_tests.jl:46)
+    from diffeqflux_hip import ODESolution
+    rng = np.random.default_rng(0)
+    arr = rng.standard_normal((5, 3, 2)).astype(np.float32)     # Julia 2 x 3 x 5
+    ts = np.array([0.0, 0.25, 0.5, 0.75, 1.0])
+    sol = ODESolution(arr, ts, 1, {"nf": 42, "naccept": 7, "nreject": 0})
+    assert sol.retcode == "Success" and sol.stats["nf"] == 42 and len(sol) == 5
+    assert np.array_equal(np.asarray(sol), arr)
+    assert np.array_equal(np.concatenate(sol.u, axis=0), arr.reshape(15, 2))   # reduce(hcat, u)
+    assert np.array_equal(sol.u[-1], arr[-1]) and np.array_equal(sol[2], arr[2])
+    assert np.isclose(sol.sum(), arr.sum())
+    # sol(t): the saved state at a saved time, linear in between
+    assert np.array_equal(sol(0.5), arr[2])
+    assert np.allclose(sol(0.6), arr[2] + 0.4 * (arr[3] - arr[2]), atol=1e-7)
+    assert np.array_equal(sol(1.0), arr[4])
+    import pytest
+    with pytest.raises(ValueError):
+        sol(1.5)
+    st = ODESolution(torch.tensor(arr), ts, 1, {})
+    assert torch.allclose(st(0.6), torch.tensor(arr[2] + 0.4 * (arr[3] - arr[2])), atol=1e-6)

@@ -197,3 +197,89 @@ def test_backsolve_without_u_integration_would_be_caught():
         m.forward = orig
     _, gref, _ = adj.backward_backsolve(m, p, fwd, [fwd.u[-1]], checkpointing=False, **tol)
     assert np.abs(gp - gref).max() > 1e-3
+
+
+# ---------------------------------------------------------------- checkpointing
+def test_checkpointed_linear_closed_forms():
+    # interior saves with zero cotangent only place checkpoints; L = |u(T)|^2 / 2
+    m, p, W, u0 = _linear_problem(seed=3)
+    T, D = 1.3, 3
+    tol = dict(abstol=1e-12, reltol=1e-12)
+    ts = [0.0, 0.3, 0.65, 1.0, T]
+    fwd = adj.forward(m, p, u0, 0.0, T, saveat=ts, **tol)
+    cot = [np.zeros_like(u0)] * (len(ts) - 1) + [fwd.u[-1]]
+    gu0, gp, a = adj.backward(m, p, fwd, cot, checkpointing=True, **tol)
+    assert a.ck_resolves == [3, 2, 1, 0]
+    E = sla.expm(W * T)
+    assert np.abs(gu0 - u0 @ (E.T @ E).T).max() < 1e-10
+    G = np.zeros((D, D))
+    for b in range(u0.shape[0]):
+        uT = E @ u0[b]
+        M = np.zeros((2 * D, 2 * D))
+        M[:D, :D], M[:D, D:], M[D:, D:] = W.T, np.outer(uT, u0[b]), W.T
+        G += sla.expm(M * T)[:D, D:]
+    assert np.abs(gp[:D * D].reshape(D, D).T - G).max() < 1e-9
+
+
+def test_checkpointed_saveat_jumps_closed_form():
+    m, p, W, u0 = _linear_problem(seed=1)
+    ts = [0.0, 0.4, 0.9, 1.2]
+    rng = np.random.default_rng(2)
+    cs = [rng.standard_normal(u0.shape) for _ in ts]
+    tol = dict(abstol=1e-12, reltol=1e-12)
+    fwd = adj.forward(m, p, u0, 0.0, 1.2, saveat=ts, **tol)
+    gu0, _, _ = adj.backward(m, p, fwd, cs, checkpointing=True, **tol)
+    expect = sum(c @ sla.expm(W * t) for c, t in zip(cs, ts))
+    assert np.abs(gu0 - expect).max() < 1e-10
+
+
+def test_checkpointed_fixed_dt_equals_dense():
+    # save times on step boundaries: each interval re-solve repeats the
+    # forward's own steps, so u(t) -- and the gradients -- are the dense ones
+    pr = spiral()
+    m, p, u0 = pr["model"], pr["p"], pr["u0"]
+    kw = dict(adaptive=False, dt=0.05)
+    ts = [0.15 * i for i in range(11)]
+    fwd = adj.forward(m, p, u0, 0.0, 1.5, saveat=ts, **kw)
+    cot = [2 * (u - 0.5) for u in fwd.u]
+    g0, gp0, _ = adj.backward(m, p, fwd, cot, **kw)
+    g1, gp1, a = adj.backward(m, p, fwd, cot, checkpointing=True, **kw)
+    assert a.ck_resolves == list(range(9, -1, -1))
+    assert np.abs(gp1 - gp0).max() < 1e-11 * np.abs(gp0).max()
+    assert np.abs(g1 - g0).max() < 1e-11 * np.abs(g0).max()
+
+
+def test_checkpointed_adaptive_matches_dense_and_fd():
+    pr = spiral()
+    m, p, u0 = pr["model"], pr["p"], pr["u0"]
+    tol = dict(abstol=1e-9, reltol=1e-9)
+    fwd = adj.forward(m, p, u0, *pr["tspan"], saveat=pr["saveat"], **tol)
+    target = np.array(fwd.u) * 0.9
+    cot = [2 * (u - t) for u, t in zip(fwd.u, target)]
+    _, gp0, _ = adj.backward(m, p, fwd, cot, **tol)
+    _, gp1, a = adj.backward(m, p, fwd, cot, checkpointing=True, **tol)
+    assert a.ck_nf > 0 and len(a.ck_resolves) == len(pr["saveat"]) - 1
+    assert np.abs(gp1 - gp0).max() < 1e-6 * np.abs(gp0).max()
+    idx = np.random.default_rng(5).choice(m.nparams, 6, replace=False)
+    fd = _fd_grad(m, p, u0, pr["tspan"], pr["saveat"], target, tol, idx)
+    assert np.abs(fd - gp1[idx]).max() < 1e-5 * max(1.0, np.abs(fd).max())
+
+
+def test_checkpointed_planted_wrong_checkpoint_fails(monkeypatch):
+    # re-solving each interval from the next checkpoint's state is caught
+    class Shifted(adj.Checkpoints):
+        def _resolve(self, k, dt):
+            self.u = self.u[1:] + self.u[-1:] if not hasattr(self, "_sh") else self.u
+            self._sh = True
+            return super()._resolve(k, dt)
+    monkeypatch.setattr(adj, "Checkpoints", Shifted)
+    m, p, W, u0 = _linear_problem(seed=1)
+    ts = [0.0, 0.4, 0.9, 1.2]
+    rng = np.random.default_rng(2)
+    cs = [rng.standard_normal(u0.shape) for _ in ts]
+    tol = dict(abstol=1e-12, reltol=1e-12)
+    fwd = adj.forward(m, p, u0, 0.0, 1.2, saveat=ts, **tol)
+    gu0, gp, _ = adj.backward(m, p, fwd, cs, checkpointing=True, **tol)
+    # dL/du0 of a linear ODE does not depend on u(t); dL/dW does
+    _, gp_ok, _ = adj.backward(m, p, fwd, cs, **tol)
+    assert np.abs(gp - gp_ok).max() > 1e-3 * np.abs(gp_ok).max()

@@ -178,3 +178,104 @@ def test_ffjord_adjoint_without_second_order_source_is_caught():
         FF._act_d2 = orig
     _, gp, _ = FF.backward(m, p, fwd, e, cz, cd, **tol)
     assert np.abs(gbad - gp).max() > 1e-4 * np.abs(gp).max()
+
+
+# ---- exact trace and regularisers (src/ffjord.jl:116-132, :149-169)
+@pytest.mark.parametrize("monte_carlo", [True, False])
+@pytest.mark.parametrize("regularize", [True, False])
+@pytest.mark.parametrize("tcat", [False, True])
+def test_rhs_vjp_matches_torch_double_backward(monte_carlo, regularize, tcat):
+    """The VJP of the whole augmented right-hand side [f; -tr; ‖f‖²; ‖eᵀJ‖]
+    (Hutchinson or exact trace, with and without the regularisers and with
+    non-zero regulariser cotangents) against torch FP64 autograd of the same
+    function built from the Jacobian torch computes itself."""
+    import torch
+    m, p, D, rng = _small_cnf(["tanh", "softplus", "identity"], tcat, seed=3)
+    B = 4
+    x = rng.standard_normal((B, m.widths[0]))
+    e = rng.choice([-1.0, 1.0], size=(B, D))
+    lz, ld = rng.standard_normal((B, D)), rng.standard_normal(B)
+    l1 = rng.standard_normal(B) if regularize else None
+    l2 = rng.standard_normal(B) if regularize else None
+    gx, gp = FF.rhs_vjp(m, p, x, D, e, lz, ld, l1, l2, monte_carlo)
+    xt = torch.tensor(x, requires_grad=True)
+    pt = torch.tensor(p, requires_grad=True)
+    acts_t = {1: torch.tanh, 2: torch.nn.functional.softplus, 0: lambda v: v}
+
+    def f(xx):
+        h = xx
+        for (i, j), (w0, b0, e_), a in zip(zip(m.widths[:-1], m.widths[1:]), m.offsets, m.acts):
+            h = acts_t[a](h @ pt[w0:b0].reshape(i, j) + pt[b0:e_])
+        return h
+    fx = f(xt)
+    rows = [torch.autograd.grad(fx[:, d].sum(), xt, create_graph=True)[0][:, :D]
+            for d in range(D)]                                   # J rows per sample
+    J = torch.stack(rows, 1)                                     # (B, D, D): J[b, d, :]
+    et = torch.tensor(e)
+    tr = torch.diagonal(J, dim1=1, dim2=2).sum(1) if not monte_carlo else \
+        torch.einsum("bi,bij,bj->b", et, J, et)
+    obj = (torch.tensor(lz) * fx).sum() - (torch.tensor(ld) * tr).sum()
+    if regularize:
+        w = torch.einsum("bi,bij->bj", et, J)
+        obj = obj + (torch.tensor(l1) * (fx * fx).sum(1)).sum() + \
+            (torch.tensor(l2) * torch.linalg.norm(w, dim=1)).sum()
+    tgx, tgp = torch.autograd.grad(obj, [xt, pt])
+    assert np.abs(gx - tgx.numpy()).max() < 1e-11 * max(1.0, np.abs(gx).max())
+    assert np.abs(gp - tgp.numpy()).max() < 1e-11 * max(1.0, np.abs(gp).max())
+    if not regularize and monte_carlo:       # the pair form is trace_vjp
+        hx, hp = FF.trace_vjp(m, p, x, e, lz, ld)
+        assert np.abs(hx - gx).max() < 1e-13 and np.abs(hp - gp).max() < 1e-13
+
+
+def test_regularisers_linear_flow_closed_forms():
+    # f = A z: ‖eᵀJ‖ = ‖Aᵀe‖ constant, so r2(T) = T ‖Aᵀe‖; r1(T) = ∫‖A e^{At} x‖² dt
+    from scipy.integrate import quad_vec
+    rng = np.random.default_rng(4)
+    D, B, T = 3, 4, 0.9
+    m, p, A = linear_model(D, rng)
+    x = rng.standard_normal((B, D))
+    e = rng.choice([-1.0, 1.0], size=(B, D))
+    tol = dict(abstol=1e-12, reltol=1e-12)
+    for mc in (True, False):
+        lp, z, dl, sol = FF.forward(m, p, x, 0.0, T, probe=e, regularize=True, monte_carlo=mc,
+                                    **tol)
+        assert np.abs(sol.lam2 - T * np.linalg.norm(e @ A, axis=1)).max() < 1e-10
+        r1, _ = quad_vec(lambda t: np.sum((x @ expm(A * t).T @ A.T) ** 2, axis=1), 0.0, T,
+                         epsabs=1e-13, epsrel=1e-13)
+        assert np.abs(sol.lam1 - r1).max() < 1e-9
+        tr = np.trace(A) if not mc else np.einsum("bi,ij,bj->b", e, A, e)
+        assert np.abs(dl + tr * T).max() < 1e-10
+        assert np.abs(z - x @ expm(A * T).T).max() < 1e-9
+
+
+@pytest.mark.parametrize("monte_carlo", [True, False])
+def test_regularised_exact_adjoint_vs_central_fd(monte_carlo):
+    """dL/dp and dL/dx of L = -mean(logpx) + 0.3 mean(λ₁) + 0.2 mean(λ₂)
+    through ``backward`` against central differences of ``forward``."""
+    m, p, D, rng = _small_cnf(["tanh", "tanh", "identity"], False, seed=6)
+    B = 3
+    x = rng.standard_normal((B, D))
+    e = rng.choice([-1.0, 1.0], size=(B, D))
+    tol = dict(abstol=1e-11, reltol=1e-11)
+    kw = dict(probe=e, regularize=True, monte_carlo=monte_carlo, **tol)
+
+    def loss(pp, xx):
+        lp, _, _, s = FF.forward(m, pp, xx, 0.0, 0.6, **kw)
+        return -float(np.mean(lp)) + 0.3 * float(np.mean(s.lam1)) + 0.2 * float(np.mean(s.lam2))
+    lp, z, dl, fwd = FF.forward(m, p, x, 0.0, 0.6, **kw)
+    cz, cd = FF.nll_cotangents(z, B)
+    gx, gp, _ = FF.backward(m, p, fwd, e, cz, cd, regularize=True, monte_carlo=monte_carlo,
+                            dL_dlam1=np.full(B, 0.3 / B), dL_dlam2=np.full(B, 0.2 / B), **tol)
+    h = 1e-6
+    idx = np.random.default_rng(7).choice(m.nparams, 10, replace=False)
+    fd = []
+    for j in idx:
+        d = np.zeros_like(p); d[j] = h
+        fd.append((loss(p + d, x) - loss(p - d, x)) / (2 * h))
+    assert np.abs(np.array(fd) - gp[idx]).max() < 1e-6 * max(1.0, np.abs(fd).max())
+    fx = []
+    for b in range(B):
+        for c in range(D):
+            d = np.zeros_like(x); d[b, c] = h
+            fx.append((loss(p, x + d) - loss(p, x - d)) / (2 * h))
+    assert np.abs(np.array(fx).reshape(B, D) - gx).max() < 1e-6 * max(1.0, np.abs(gx).max())
