@@ -2504,6 +2504,20 @@ void launch_prep_plain(const StateGeom& sg, const float* u, void* X0, void* X0T,
                                             u, (typename PP::T*)X0, (typename PP::T*)X0T, Ucur,
                                             ldx, ldxt, px, pxt));
 }
+// zero-fill of a 16-byte aligned region (memory a captured graph may clear:
+// a kernel, not hipMemsetAsync)
+__global__ void k_fill_zero(u32x4* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = u32x4{0u, 0u, 0u, 0u};
+}
+void launch_fill_zero(void* p, int64_t bytes, hipStream_t s) {
+  const int64_t n = bytes / 16;
+  if (n <= 0) return;
+  const int64_t nb = (n + EW_THREADS - 1) / EW_THREADS;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)(nb < 2048 ? nb : 2048)), dim3(EW_THREADS), 0, s,
+                     (u32x4*)p, n);
+}
 template <class PP>
 __global__ void k_prep_lambda(StateGeom sg, const float* lam, typename PP::T* L,
                               typename PP::T* LT, int ldl, int ldlt, int64_t pl, int64_t plt) {

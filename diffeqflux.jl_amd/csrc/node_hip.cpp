@@ -142,6 +142,8 @@ struct node_handle {
   double ck_last_dt = 0.0;    // last step of the previous re-solve (next one's initial dt)
   int64_t ck_nf = 0;          // RHS evaluations of the re-solves of the last backward
 
+  void* na_scratch = nullptr;  // this device's no-alloc array (node_rhs_device_noalloc)
+
   bool have_params = false, have_forward = false;
   node_solve_opts fwd_opts{};
   node_stats fwd_stats{};
@@ -1652,6 +1654,38 @@ int copy_grads(node_handle* h, float* dLdu0, double* dLdp, hipMemcpyKind kind) {
   return 0;
 }
 
+// ---------------------------------------------------------------- no-alloc RHS
+// The fixed device array of node_rhs_device_noalloc: the model input, two
+// ping-pong hidden-activation buffers and the output of one right-hand side
+// live here instead of in memory allocated per handle or per call.  Each GPU
+// holds its own copy until the process ends; host code uses the per-device
+// address from hipGetSymbolAddress (the array's name is a host-side stand-in).
+constexpr size_t NOALLOC_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex g_scratch_mtx;
+void* g_scratch_ptr[NOALLOC_MAX_DEV] = {};
+
+// Device address of the current device's array, resolved once per device
+// (node_create: the library's set-up, with the handle's device current).
+hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratch_mtx);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -1710,6 +1744,12 @@ int32_t node_create(const node_desc* desc, node_handle_t** out) {
     return NODE_ERR_HIP;
   }
   if (upload_tableau(h->stream) != hipSuccess) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return NODE_ERR_HIP;
+  }
+  if (hipError_t e = scratch_address(&h->na_scratch); e != hipSuccess) {
+    std::fprintf(stderr, "node_create: hipGetSymbolAddress(scratch): %s\n", hipGetErrorString(e));
     (void)hipStreamDestroy(h->stream);
     delete h;
     return NODE_ERR_HIP;
@@ -2243,6 +2283,73 @@ int32_t node_get_plan(node_handle_t* h, int32_t role, int32_t layer, int32_t* ti
   if (tile_rows) *tile_rows = gemm_cfg_tile_rows(cfg);
   if (tile_cols) *tile_cols = gemm_cfg_tile_cols(cfg);
   if (ksplit) *ksplit = ks;
+  return 0;
+}
+
+int32_t node_rhs_device_noalloc(node_handle_t* h, const float* d_u, float* d_du, int64_t n) {
+  if (!h || !d_u || !d_du || n < 1) return NODE_ERR_INVALID;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (h->cnf) return fail(h, NODE_ERR_UNSUPPORTED, "CNF handle: use node_cnf_forward");
+  if (!h->have_params) return fail(h, NODE_ERR_STATE, "parameters not set");
+  if (!h->na_scratch) return fail(h, NODE_ERR_STATE, "no-alloc array address not resolved");
+  // layout in the array: X0 [planes][Bp][Dp], two hidden buffers
+  // [planes][Bp][max outp], the FP32 output [Bp][D]; 256-byte aligned parts
+  const int64_t Bp = rup(n, BPAD);
+  int64_t hmax = 0;
+  for (int l = 0; l + 1 < h->L; ++l) hmax = std::max<int64_t>(hmax, h->layers[l].outp);
+  int64_t off = 0;
+  auto take = [&off](int64_t bytes) { const int64_t o = off; off = rup(off + bytes, 256); return o; };
+  const int64_t ox = take((int64_t)h->planes * Bp * h->Dp * h->esz);
+  const int64_t oh0 = take((int64_t)h->planes * Bp * hmax * h->esz);
+  const int64_t oh1 = take((int64_t)h->planes * Bp * hmax * h->esz);
+  const int64_t zero_bytes = off;
+  const int64_t oo = take(Bp * h->D * (int64_t)sizeof(float));
+  if (off > (int64_t)NOALLOC_BYTES)
+    return fail(h, NODE_ERR_CAPACITY,
+                "node_rhs_device_noalloc: " + std::to_string(n) + " trajectories need " +
+                    std::to_string(off) + " bytes of the " + std::to_string(NOALLOC_BYTES) +
+                    "-byte no-alloc array");
+  char* base = (char*)h->na_scratch;
+  // padded feature columns must read as zeros in the GEMM K loops; the array
+  // keeps whatever the last call (of any handle) left there
+  launch_fill_zero(base, zero_bytes, h->stream);
+  StateGeom sg = geom(h);
+  sg.B = (int)n;
+  sg.Bp = (int)Bp;
+  sg.sv = Bp * h->D;
+  sg.ent = 8 * sg.sv;
+  const int64_t px = Bp * h->Dp;
+  launch_prep_plain(sg, d_u, base + ox, nullptr, nullptr, h->Dp, 0, px, 0, h->stream);
+  const void* A = base + ox;
+  for (int l = 0; l + 1 < h->L; ++l) {
+    const Layer& ly = h->layers[l];
+    void* Y = base + ((l & 1) ? oh1 : oh0);
+    GemmArgs ga{};
+    ga.A = A; ga.lda = ly.inp; ga.B = ly.Wr; ga.ldb = ly.inp; ga.K = ly.inp;
+    ga.pa = Bp * ly.inp; ga.pb = (int64_t)ly.outp * ly.inp; ga.ksplit = 1;
+    const GemmPlan p = plan((int)Bp, ly.outp, 0);
+    ga.row_tiles = p.rt; ga.col_tiles = p.ct;
+    const EpiHidden e{ly.bias, ly.act, ly.out, ly.outp, 0, Y, nullptr, nullptr,
+                      Bp * ly.outp, 0};
+    launch_gemm_hidden(h->prec, p.cfg, ga, e, h->ctl, GATE_ALWAYS, h->stream);
+    A = Y;
+  }
+  const Layer& ly = h->layers[h->L - 1];
+  GemmArgs ga{};
+  ga.A = A; ga.lda = ly.inp; ga.B = ly.Wr; ga.ldb = ly.inp; ga.K = ly.inp;
+  ga.pa = Bp * ly.inp; ga.pb = (int64_t)ly.outp * ly.inp; ga.ksplit = 1;
+  const GemmPlan p = plan((int)Bp, ly.outp, 1);
+  ga.row_tiles = p.rt; ga.col_tiles = p.ct;
+  EpiFwdStage e{};
+  e.bias = ly.bias; e.act = ly.act; e.D = h->D; e.B = (int)n; e.stage = 0; e.mode = SM_PLAIN;
+  e.tape = nullptr; e.ent = sg.ent; e.sv = sg.sv;
+  e.X0 = nullptr; e.ldx = h->Dp; e.map = h->desc.input_map;
+  e.scratch = (float*)(base + oo); e.part = nullptr; e.px = px;
+  launch_gemm_fwdstage(h->prec, p.cfg, ga, e, h->ctl, GATE_ALWAYS, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(d_du, base + oo, (size_t)n * h->D * sizeof(float),
+                           hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 

@@ -281,6 +281,7 @@ void launch_adj_step_begin(const StateGeom& sg, const AdjBufs& ab, const Ctl* ct
 void launch_mu_norm(int mode, const AdjBufs& ab, const Ctl* ctl, double* part,
                     int nblk, int gate, hipStream_t s);
 int mu_blocks(int64_t P);
+void launch_fill_zero(void* p, int64_t bytes, hipStream_t s);
 void launch_adj_resume(Ctl* ctl, const Ctl* saved, int64_t n_fwd, double pause_t, int has_pause,
                        const double* fwd_times, hipStream_t s);
 void launch_adj_controller(Ctl* ctl, const double* part, int n_lam,

@@ -92,6 +92,8 @@ SIGNATURES = [
                                      ctypes.c_void_p, ctypes.POINTER(RaggedStats)]),
     ("node_cnf_regularizers", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("node_rhs", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("node_rhs_device_noalloc", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        c_i64]),
     ("node_vjp", c_i32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.c_void_p]),
     ("node_set_option", c_i32, [ctypes.c_void_p, c_i32, c_i64]),

@@ -209,6 +209,19 @@ int32_t node_rhs(node_handle_t* h, const float* u, float* du);
 int32_t node_vjp(node_handle_t* h, const float* u, const float* lambda,
                  float* du, double* dp);
 
+/* One right-hand side du = model(u, p) of n trajectories (d_u, d_du: D x n,
+ * device pointers) that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * result as node_rhs for those trajectories, any n (not only the handle's
+ * batch) whose model input and hidden activations fit the library's fixed
+ * no-alloc array: 64 MiB per GPU, held until the process ends (up to ~7,700
+ * trajectories at D = 64 with 1024-wide split-BF16 hidden layers); beyond it
+ * NODE_ERR_CAPACITY.
+ * Every call on a GPU shares that GPU's array and clears what it uses:
+ * calls that overlap on one GPU (from any handle) race.  Synchronous. */
+int32_t node_rhs_device_noalloc(node_handle_t* h, const float* d_u, float* d_du, int64_t n);
+
 /* ---- Per-trajectory ("ragged") solves ----------------------------------
  * n independent trajectories of the handle's model, each its own ODE problem
  * -- what n separate NeuralODE solves of batch 1 (src/neural_de.jl:47-61, one
