@@ -181,6 +181,89 @@ def measured_traffic():
     return out
 
 
+# Whole-solve counters of the headline step from the committed PMC summary:
+# "step_profile: <precision> hbm_bytes_per_solve <B> mfma_busy_cycles_per_solve <C>"
+# (C summed over the chip's SIMDs).
+def measured_step_profile():
+    out = {}
+    if not os.path.isdir(PROFILES):
+        return out
+    for fn in sorted(os.listdir(PROFILES)):
+        if not fn.startswith("pmc_"):
+            continue
+        with open(os.path.join(PROFILES, fn)) as f:
+            for line in f:
+                mt = re.match(r"\s*step_profile:\s*(\S+)\s+hbm_bytes_per_solve\s+(\d+)\s+"
+                              r"mfma_busy_cycles_per_solve\s+(\d+)", line)
+                if mt:
+                    out[mt.group(1)] = (int(mt.group(2)), int(mt.group(3)), fn)
+    return out
+
+
+def pmc_summary(root, timed=3):
+    """Per-solve totals of rocprofv3 --pmc runs of this bench's headline leg
+    (``--steps 3 --warmup 0``, csv output).  ``root`` holds one run per
+    counter set (SQ_VALU_MFMA_BUSY_CYCLES/, FETCH_SIZE/, WRITE_SIZE/) and
+    cFETCH_SIZE/, cWRITE_SIZE/ (a 256 MiB device copy, 3 dispatches).  The
+    dispatch stream is cut at every k_fwd_init (one per solve: the warm solve
+    with the autotuner, the timed solves, then the roofline passes); the
+    figures are the mean over the ``timed`` solves after the warm one."""
+    import csv
+    import glob
+
+    def kname(nm):
+        m = re.search(r"gemm_tn<.*?, (\d+), (\d+), (\d+), (\d+), \d+, nodehip::(\w+)>", nm)
+        if m:
+            return f"gemm_tn<{m.group(1)}x{m.group(2)},{m.group(3)}x{m.group(4)},{m.group(5)}>"
+        return re.sub(r"^void |<.*|\(.*", "", nm).split("::")[-1]
+
+    def solves(tag):
+        fs = glob.glob(os.path.join(root, tag, "*_counter_collection.csv"))
+        disp = {}
+        with open(fs[0]) as f:
+            for r in csv.DictReader(f):
+                d = disp.setdefault(int(r["Dispatch_Id"]), {"k": kname(r["Kernel_Name"])})
+                d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        segs, cur = [], None
+        for did in sorted(disp):
+            if disp[did]["k"] == "k_fwd_init":
+                cur = []
+                segs.append(cur)
+            if cur is not None:
+                cur.append(disp[did])
+        return segs[1:1 + timed]
+
+    def per_solve(tag, ctr):
+        segs = solves(tag)
+        out = {}
+        for seg in segs:
+            for d in seg:
+                out[d["k"]] = out.get(d["k"], 0.0) + d.get(ctr, 0.0) / len(segs)
+        return out
+
+    def copy_kib(tag, ctr):
+        fs = glob.glob(os.path.join(root, tag, "*_counter_collection.csv"))
+        with open(fs[0]) as f:
+            v = [float(r["Counter_Value"]) for r in csv.DictReader(f)
+                 if "copyBuffer" in r["Kernel_Name"] and r["Counter_Name"] == ctr]
+        return sum(v) / len(v)
+
+    fscale = 262144.0 / copy_kib("cFETCH_SIZE", "FETCH_SIZE")
+    wscale = 262144.0 / copy_kib("cWRITE_SIZE", "WRITE_SIZE")
+    fe, wr = per_solve("FETCH_SIZE", "FETCH_SIZE"), per_solve("WRITE_SIZE", "WRITE_SIZE")
+    mb = per_solve("SQ_VALU_MFMA_BUSY_CYCLES", "SQ_VALU_MFMA_BUSY_CYCLES")
+    mi = per_solve("SQ_VALU_MFMA_BUSY_CYCLES", "SQ_INSTS_MFMA")
+    ks = set(fe) | set(wr) | set(mb)
+    rows = [{"kernel": k, "fetch_bytes": fe.get(k, 0.0) * fscale * 1024,
+             "write_bytes": wr.get(k, 0.0) * wscale * 1024,
+             "mfma_busy_cycles": mb.get(k, 0.0), "mfma_insts": mi.get(k, 0.0)} for k in ks]
+    rows.sort(key=lambda r: -(r["fetch_bytes"] + r["write_bytes"]))
+    tot = {c: sum(r[c] for r in rows) for c in ("fetch_bytes", "write_bytes", "mfma_busy_cycles",
+                                                 "mfma_insts")}
+    return {"fetch_scale": fscale, "write_scale": wscale, "solves": timed, "per_kernel": rows,
+            "per_solve": tot}
+
+
 def profile_dominant(run):
     """One profiled step with every GEMM timed.  Launches are grouped by
     (role, GEMM shape) -- the layers of one shape run the same kernel -- and
@@ -762,7 +845,30 @@ def cpu_baseline_leg(p, Bs, gpu_grads):
 
 
 # ---------------------------------------------------------------- main
+# MI355X: 256 CUs x 4 SIMDs, peak engine clock 2.4 GHz
+N_SIMD, CLOCK_HZ = 1024, 2.4e9
+
+
+def solve_counters(precision, ms_per_solve):
+    """Whole-solve HBM rate and MFMA busy fraction: per-solve bytes and MFMA
+    busy cycles from the committed PMC summary of this command, over this
+    run's solve time (busy fraction at the peak clock)."""
+    sp = measured_step_profile().get(precision)
+    if sp is None:
+        return {}
+    nbytes, busy, fn = sp
+    t = ms_per_solve * 1e-3
+    return {"solve_hbm_gbs": round(nbytes / t / 1e9, 1),
+            "solve_hbm_frac": round(nbytes / t / 8e12, 4),
+            "solve_mfma_busy": round(busy / (N_SIMD * CLOCK_HZ * t), 4),
+            "solve_counters_source": f"profiles/{fn}: rocprofv3 --pmc per solve (FETCH_SIZE + "
+                                     "WRITE_SIZE copy-calibrated, SQ_VALU_MFMA_BUSY_CYCLES)"}
+
+
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--pmc-summary":
+        print(json.dumps(pmc_summary(sys.argv[2]), indent=1))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -912,6 +1018,7 @@ def main():
         # S = sum in*out) per second of the timed steps, against the peak
         "solve_tflops": round(solve_flops / dt / 1e12, 2),
         "solve_mfma_frac": round(solve_flops / dt / 1e12 / PEAK[args.precision], 4),
+        **solve_counters(args.precision, 1e3 * dt / args.steps),
         "nf_fwd": sf["nf"], "nf_adj": sa["nf"],
         "steps_fwd": f"{sf['naccept']}+{sf['nreject']}",
         "steps_adj": f"{sa['naccept']}+{sa['nreject']}",
