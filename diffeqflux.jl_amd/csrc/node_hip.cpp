@@ -1432,6 +1432,16 @@ int ragged_model(node_handle* h, RaggedModel& m, std::string& why) {
   m.sv = so;
   m.P = h->P;
   m.p = h->pflat;
+  int64_t no = 0;
+  for (int l = 0; l < m.L; ++l) {
+    m.nw0[l] = no;
+    no += (int64_t)m.ti[l] * m.to[l] * 256;
+  }
+  for (int l = 0; l < m.L; ++l) {
+    m.nb0[l] = no;
+    no += m.out[l];
+  }
+  m.Pn = (no + 63) / 64 * 64;
   return 0;
 }
 
@@ -1583,20 +1593,24 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   RaggedModel m;
   std::string why;
   if (ragged_model(h, m, why)) return fail(h, NODE_ERR_UNSUPPORTED, why);
-  const int64_t n = r.n, ns = r.ns, P = h->P;
+  const int64_t n = r.n, ns = r.ns, P = h->P, Pn = m.Pn;
   const int D = h->D, Dp = (D + 15) / 16 * 16;
-  const int grid = ragged_grid(h, n);
-  const int64_t slots = (int64_t)grid * 4 * 16;
+  // every CU gets a workgroup (the weights fill its LDS: one per CU) with
+  // just enough 16-slot waves for all trajectories to start at once
+  const int64_t waves_all = (n + RG_SLOTS - 1) / RG_SLOTS;
+  const int waves = (int)std::min<int64_t>(4, std::max<int64_t>(1, (waves_all + h->n_cu - 1) / h->n_cu));
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(h->n_cu, (waves_all + waves - 1) / waves));
+  const int64_t slots = (int64_t)grid * waves * RG_SLOTS;
   if (grow(h, &b.dLdout, std::max<int64_t>(ns * D, 1), b.dLdout_cap, std::max<int64_t>(ns * D, 64)) ||
       grow(h, &b.dLdu0, n * D, b.dLdu0_cap, n * D) ||
-      grow(h, &b.rows, n * P, b.rows_cap, n * P) ||
-      grow(h, &b.tmp, slots * P, b.tmp_cap, slots * P) ||
+      grow(h, &b.rows, n * Pn, b.rows_cap, n * Pn) ||
+      grow(h, &b.tmp, slots * Pn, b.tmp_cap, slots * Pn) ||
       grow(h, &b.sbuf, slots * 8 * m.sv, b.sbuf_cap, slots * 8 * m.sv) ||
       grow(h, &b.dp, P, b.dp_cap, P))
     return NODE_ERR_HIP;
   if (ns > 0)
     HIPCHK(h, hipMemcpyAsync(b.dLdout, dLdout, ns * D * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemsetAsync(b.rows, 0, (size_t)n * P * sizeof(float), h->stream));
+  HIPCHK(h, hipMemsetAsync(b.rows, 0, (size_t)n * Pn * sizeof(float), h->stream));
   HIPCHK(h, hipMemsetAsync(r.ctr, 0, 4 * sizeof(int), h->stream));
   HIPCHK(h, hipMemsetAsync(r.stats, 0, 4 * sizeof(unsigned long long), h->stream));
   RaggedAdjArgs a{};
@@ -1618,9 +1632,9 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   a.maxiters = o->maxiters > 0 ? o->maxiters : 100000;
   a.adaptive = o->adaptive ? 1 : 0;
   a.stats = r.stats;
-  launch_ragged_adj(a, grid, h->stream);
+  launch_ragged_adj(a, grid, waves, h->stream);
   HIPCHK(h, hipGetLastError());
-  launch_ragged_sum(b.rows, n, P, b.dp, h->stream);
+  launch_ragged_sum(b.rows, n, P, Pn, b.dp, h->stream);
   HIPCHK(h, hipGetLastError());
   std::vector<int> hret(n);
   unsigned long long hs[4];
@@ -1629,7 +1643,8 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   if (dLdu0) HIPCHK(h, hipMemcpyAsync(dLdu0, b.dLdu0, n * D * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   if (dLdp) HIPCHK(h, hipMemcpyAsync(dLdp, b.dp, P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   if (dLdp_traj)
-    HIPCHK(h, hipMemcpyAsync(dLdp_traj, b.rows, (size_t)n * P * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpy2DAsync(dLdp_traj, P * sizeof(float), b.rows, Pn * sizeof(float),
+                               P * sizeof(float), n, hipMemcpyDeviceToHost, h->stream));
   if (tstats) HIPCHK(h, hipMemcpyAsync(tstats, r.tstats, 4 * n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (ret) std::memcpy(ret, hret.data(), n * sizeof(int));

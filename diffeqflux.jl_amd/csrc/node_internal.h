@@ -303,6 +303,7 @@ void launch_gather(const StateGeom& sg, const float* src, int64_t src_stride_flo
 constexpr int RG_MAXL = 4;
 constexpr int RG_HT = 8;        // hidden width <= 128 (16-feature tiles)
 constexpr int RG_MAXD = 32;     // state dimension <= 32
+constexpr int RG_SLOTS = 16;    // trajectory slots per wave (MFMA columns)
 struct RaggedModel {
   int L, map;
   int in[RG_MAXL], out[RG_MAXL], act[RG_MAXL];
@@ -315,6 +316,13 @@ struct RaggedModel {
   int sv;                             // floats per stage block
   int64_t P;
   const float* p;                     // flat FP32 parameters (device)
+  // parameter-gradient rows while a trajectory integrates: each layer's
+  // weights tile-major (16x16 tile (it, ot), then lane, then 4 values: the
+  // MFMA accumulator order, one 16-byte vector per lane and tile), then each
+  // layer's bias; Pn floats per row (>= P, a multiple of 64).  A finished
+  // trajectory's row is rewritten in the flat parameter order (first P floats).
+  int64_t nw0[RG_MAXL], nb0[RG_MAXL];
+  int64_t Pn;
 };
 struct RaggedFwdArgs {
   RaggedModel m;
@@ -353,8 +361,8 @@ struct RaggedAdjArgs {
   const int* rec_prev;
   const float* rec_data;
   float* dLdu0;                       // [n][D]
-  float* dLdp_traj;                   // [n][P] per-trajectory parameter gradients (zeroed)
-  float* mu_tmp;                      // [slots][P]
+  float* dLdp_traj;                   // [n][Pn] per-trajectory parameter gradients (zeroed)
+  float* mu_tmp;                      // [slots][Pn]
   float* sbuf;                        // [slots][8][sv] stage blocks
   int* queue;
   int* retcode;                       // [n]
@@ -364,7 +372,8 @@ struct RaggedAdjArgs {
   int adaptive;
   unsigned long long* stats;
 };
-void launch_ragged_adj(const RaggedAdjArgs& a, int grid, hipStream_t s);
-void launch_ragged_sum(const float* rows, int64_t n, int64_t P, double* out, hipStream_t s);
+void launch_ragged_adj(const RaggedAdjArgs& a, int grid, int waves, hipStream_t s);
+void launch_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* out,
+                       hipStream_t s);
 
 }  // namespace nodehip

@@ -22,9 +22,15 @@
 // share the machine until the queue is empty.  Every trajectory's arithmetic
 // depends only on its own column, so results do not depend on scheduling.
 
-constexpr int RG_SLOTS = 16;
-
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Global-memory pointers.  The kernels get their buffers inside argument
+// structs, whose pointers the compiler leaves generic: their accesses become
+// FLAT instructions, which count against the LDS / scalar counter too, so a
+// wave can only wait for one of their results by draining every LDS and
+// scalar access in flight.  The hot loops go through these instead.
+template <class T> using gp = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ gp<T> glb(T* p) { return (gp<T>)p; }
 
 struct SlotCtl {
   double t, dt, tprev, dtpropose, qold, q11, eest, tdir, dtmax_abs, dtmin, t_end, dt_fixed;
@@ -476,7 +482,7 @@ void launch_ragged_fwd(const RaggedFwdArgs& a, int grid, hipStream_t s) {
 // g = W_0^T delta_0 (the input-map derivative is applied by the caller).
 __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
                                        const f32x4 (&x0)[RG_HT], const f32x4 (&lam)[RG_HT],
-                                       float* blk, f32x4 (&gout)[RG_HT], int lane) {
+                                       gp<float> blk, f32x4 (&gout)[RG_HT], int lane) {
   const int r = lane & 15, g4 = 4 * (lane >> 4);
   const int L = m.L;
   // one rolled loop body per sweep (the layer's input is copied into place
@@ -486,7 +492,7 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
 #pragma unroll
   for (int t = 0; t < RG_HT; ++t) {
     hin[t] = x0[t];
-    if (t < m.ti[0]) *(f32x4*)(blk + m.sh[0] + 16 * t + g4) = x0[t];
+    if (t < m.ti[0]) *(gp<f32x4>)(blk + m.sh[0] + 16 * t + g4) = x0[t];
   }
 #pragma unroll 1
   for (int l = 0; l < L; ++l) {
@@ -515,8 +521,8 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
           y[v] = act_f(act, acc[v]);
           sd[v] = act_d(act, acc[v], y[v]);
         }
-        *(f32x4*)(blk + m.sd[l] + 16 * ot + g4) = sd;
-        if (l < L - 1) *(f32x4*)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
+        *(gp<f32x4>)(blk + m.sd[l] + 16 * ot + g4) = sd;
+        if (l < L - 1) *(gp<f32x4>)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
         hout[ot] = y;
       }
     }
@@ -533,9 +539,9 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
 #pragma unroll
     for (int ot = 0; ot < RG_HT; ++ot) {
       if (ot < nto) {
-        float* pd = blk + m.sd[l] + 16 * ot + g4;
-        const f32x4 d = gi[ot] * *(const f32x4*)pd;
-        *(f32x4*)pd = d;
+        gp<float> pd = blk + m.sd[l] + 16 * ot + g4;
+        const f32x4 d = gi[ot] * *(gp<const f32x4>)pd;
+        *(gp<f32x4>)pd = d;
         gi[ot] = d;
       }
     }
@@ -564,24 +570,13 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
   for (int t = 0; t < RG_HT; ++t) gout[t] = gi[t];
 }
 
-// Parameter part of one trajectory's adjoint state, run by the whole wave for
-// slot block `sbs` (8 stage blocks of m.sv floats; logical stage j of the
-// attempt sits in block j, except j = 0 in block p0 and j = 6 in block 7-p0):
-//  MODE 0: stage sums kb = sum_j b_j g_j, ke = sum_j btilde_j g_j with
-//          g_j = -(delta_j h_j^T) (weights) and -delta_j (bias); writes
-//          mu_new = mu + dt kb; returns sum (dt ke / (atol + max|mu|,|mu_new| rtol))^2
-//  MODE 1: returns sum (g_0 / (atol + |mu| rtol))^2           (initial dt, d1)
-//  MODE 2: returns sum ((g_1 - g_0) / (atol + |mu| rtol))^2   (initial dt, d2;
-//          g_1 in block 7 - p0), *flag = 1 where g_1 != g_0
 template <int MODE>
-__device__ __forceinline__ void rg_mu_entry(int64_t idx, float kb, float ke, float dtf, float atf,
-                                            float rtf, const float* mu, float* mu_new, float m0,
-                                            float& acc, float& fl) {
+__device__ __forceinline__ float rg_mu_val(float kb, float ke, float dtf, float atf, float rtf,
+                                           float m0, float& mn, float& fl) {
   const float gb = -kb, ge = -ke;
   float e;
   if (MODE == 0) {
-    const float mn = m0 + dtf * gb;
-    mu_new[idx] = mn;
+    mn = m0 + dtf * gb;
     e = __fdividef(dtf * ge, atf + fmaxf(fabsf(m0), fabsf(mn)) * rtf);
   } else if (MODE == 1) {
     e = __fdividef(gb, atf + fabsf(m0) * rtf);
@@ -589,106 +584,139 @@ __device__ __forceinline__ void rg_mu_entry(int64_t idx, float kb, float ke, flo
     e = __fdividef(ge, atf + fabsf(m0) * rtf);
     if (ge != 0.f) fl = 1.f;
   }
-  acc += e * e;
+  return e;
 }
 
+// Parameter part of one trajectory's adjoint state, run by the whole wave for
+// slot block `sbs` (8 stage blocks of m.sv floats; logical stage j of the
+// attempt sits in block j, except j = 0 in block p0 and j = 6 in block 7-p0),
+// over a row in the tile-major order of RaggedModel (mu, mu_new: Pn floats):
+//  MODE 0: stage sums kb = sum_j b_j g_j, ke = sum_j btilde_j g_j with
+//          g_j = -(delta_j h_j^T) (weights) and -delta_j (bias); writes
+//          mu_new = mu + dt kb; returns sum (dt ke / (atol + max|mu|,|mu_new| rtol))^2
+//  MODE 1: returns sum (g_0 / (atol + |mu| rtol))^2           (initial dt, d1)
+//  MODE 2: returns sum ((g_1 - g_0) / (atol + |mu| rtol))^2   (initial dt, d2;
+//          g_1 in block 7 - p0), *flag = 1 where g_1 != g_0
+// The pass streams the row: every layer's bias entries (their loads issued
+// together), then each layer's weight tiles input block by input block, the
+// next block's loads issued before the current one is formed.  The stage sums
+// of a weight tile are outer products over the stages j:
+//   k_b[i][o] = sum_j cb_j delta_j[o] h_j[i],  k_e likewise with ce_j,
+// a GEMM with K = 7 stages (padded to 8): two v_mfma_f32_16x16x4f32 each,
+// A[i][j] = h_j[i], B[j][o] = c_j delta_j[o].  Lane (r, g) supplies stage
+// j = 4q + g of MFMA step q; the accumulator gives it output o = r and inputs
+// i = 4g + v, which the row stores as this lane's 16-byte vector of the tile
+// (padding entries outside the layer hold 0 and add no error).
 template <int MODE>
-__device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, double dt,
-                             double atol, double rtol, const float* __restrict__ mu,
-                             float* __restrict__ mu_new, double* flag, int lane) {
-  // The stage sums are outer products over the stages j:
-  //   k_b[i][o] = sum_j cb_j delta_j[o] h_j[i],  k_e likewise with ce_j,
-  // i.e. for every 16x16 block of (layer input i, output o) a GEMM with
-  // K = 7 stages (padded to 8): two v_mfma_f32_16x16x4f32 each, A[i][j] =
-  // h_j[i], B[j][o] = c_j delta_j[o].  Lane (r, g) supplies stage j = 4q + g
-  // of MFMA step q; the accumulator gives it output o = r and inputs
-  // i = 4g + v, i.e. flat entries W[o, i] at i*out + o (coalesced over r).
+__device__ double rg_mu_pass(const RaggedModel& m, gp<const float> sbs, int p0, double dt,
+                             double atol, double rtol, gp<const float> __restrict__ mu,
+                             gp<float> __restrict__ mu_new, double* flag, int lane) {
   const int r = lane & 15, g = lane >> 4;
-  int blk[2];
+  int boff[2];
   float cb[2], ce[2];
   bool use[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int j = 4 * q + g;
+    int blk;
     if (MODE == 0) {
       use[q] = j < 7;
-      blk[q] = j == 0 ? p0 : (j == 6 ? 7 - p0 : (j < 7 ? j : 0));
+      blk = j == 0 ? p0 : (j == 6 ? 7 - p0 : (j < 7 ? j : 0));
       cb[q] = j < 6 ? cA[6][j] : 0.f;
       ce[q] = j < 7 ? cBT[j] : 0.f;
     } else if (MODE == 1) {
       use[q] = j == 0;
-      blk[q] = p0;
+      blk = p0;
       cb[q] = 1.f;   // g_0
       ce[q] = 0.f;
     } else {
       use[q] = j < 2;
-      blk[q] = j == 0 ? p0 : 7 - p0;
+      blk = j == 0 ? p0 : 7 - p0;
       cb[q] = 0.f;
       ce[q] = j == 0 ? -1.f : 1.f;  // g_1 - g_0
     }
+    boff[q] = blk * m.sv;
   }
   const float dtf = (float)dt, atf = (float)atol, rtf = (float)rtol;
   float acc = 0.f, fl = 0.f;
-  for (int l = 0; l < m.L; ++l) {
-    const int in = m.in[l], out = m.out[l], nti = m.ti[l], nto = m.to[l];
-    const int64_t w0 = m.w0[l];
-    // bias entries (layer input 1): lanes over o, stages summed on the VALU
-    for (int o0 = 0; o0 < out; o0 += 64) {
-      const int o = o0 + lane;
-      if (o < out) {
-        float kb = 0.f, ke = 0.f;
+
+  // ---- biases: lanes over the outputs o of each layer (at most two 64-wide
+  // rounds: hidden widths <= 128, D <= 32); the loads of every layer's rounds
+  // are issued before any entry is formed
+  float bm[RG_MAXL][2], bkb[RG_MAXL][2], bke[RG_MAXL][2];
+#pragma unroll
+  for (int l = 0; l < RG_MAXL; ++l)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      bm[l][c] = bkb[l][c] = bke[l][c] = 0.f;
+      const int o = 64 * c + lane;
+      if (l < m.L && o < m.out[l]) {
+        const int off = m.sd[l] + o;
+        bm[l][c] = mu[m.nb0[l] + o];
         if (MODE == 0) {
 #pragma unroll
           for (int j = 0; j < 7; ++j) {
             const int bj = j == 0 ? p0 : (j == 6 ? 7 - p0 : j);
-            const float d = sbs[bj * m.sv + m.sd[l] + o];
-            kb += (j < 6 ? cA[6][j] : 0.f) * d;
-            ke += cBT[j] * d;
+            const float d = sbs[bj * m.sv + off];
+            bkb[l][c] += (j < 6 ? cA[6][j] : 0.f) * d;
+            bke[l][c] += cBT[j] * d;
           }
         } else if (MODE == 1) {
-          kb = sbs[p0 * m.sv + m.sd[l] + o];
+          bkb[l][c] = sbs[p0 * m.sv + off];
         } else {
-          ke = sbs[(7 - p0) * m.sv + m.sd[l] + o] - sbs[p0 * m.sv + m.sd[l] + o];
+          bke[l][c] = sbs[(7 - p0) * m.sv + off] - sbs[p0 * m.sv + off];
         }
-        const int64_t idx = m.b0[l] + o;
-        rg_mu_entry<MODE>(idx, kb, ke, dtf, atf, rtf, mu, mu_new, mu[idx], acc, fl);
       }
     }
-    // B operands of every output block: c_j delta_j[16 ot + r] for this lane's stages
+#pragma unroll
+  for (int l = 0; l < RG_MAXL; ++l)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int o = 64 * c + lane;
+      if (l < m.L && o < m.out[l]) {
+        float mn = 0.f;
+        const float ev = rg_mu_val<MODE>(bkb[l][c], bke[l][c], dtf, atf, rtf, bm[l][c], mn, fl);
+        if (MODE == 0) mu_new[m.nb0[l] + o] = mn;
+        acc += ev * ev;
+      }
+    }
+
+  // ---- weight tiles, layer by layer: the B operands c_j delta_j[16 ot + r]
+  // of every output block once per layer, then the input blocks it in order,
+  // the next block's mu vectors and A operands (h_j[16 it + r]) loaded before
+  // the current block is formed
+  for (int l = 0; l < m.L; ++l) {
+    const int in = m.in[l], out = m.out[l], nti = m.ti[l], nto = m.to[l];
     float db[RG_HT][2], de[RG_HT][2];
 #pragma unroll
     for (int ot = 0; ot < RG_HT; ++ot)
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         float d = 0.f;
-        if (ot < nto && use[q]) d = sbs[blk[q] * m.sv + m.sd[l] + 16 * ot + r];
+        if (ot < nto && use[q]) d = sbs[boff[q] + m.sd[l] + 16 * ot + r];
         db[ot][q] = cb[q] * d;
         de[ot][q] = ce[q] * d;
       }
-    for (int it = 0; it < nti; ++it) {
-      float ha[2];
+    const gp<const f32x4> mrow = (gp<const f32x4>)(mu + m.nw0[l]) + lane;
+    const gp<f32x4> nrow = (gp<f32x4>)(mu_new + m.nw0[l]) + lane;
+    auto fetch = [&](int it, float (&ha)[2], f32x4 (&mv)[RG_HT]) {
 #pragma unroll
       for (int q = 0; q < 2; ++q)
-        ha[q] = use[q] ? sbs[blk[q] * m.sv + m.sh[l] + 16 * it + r] : 0.f;
-      // every mu entry of this input block first (up to 4 RG_HT independent
-      // loads per lane in flight: the pass streams mu and is bound by how many
-      // loads wait at once), then the MFMAs and the entries
-      float m0a[RG_HT][4];
+        ha[q] = (it < nti && use[q]) ? sbs[boff[q] + m.sh[l] + 16 * it + r] : 0.f;
 #pragma unroll
-      for (int ot = 0; ot < RG_HT; ++ot) {
-        const int o = 16 * ot + r;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = 16 * it + 4 * g + v;
-          m0a[ot][v] = (ot < nto && o < out && i < in) ? mu[w0 + (int64_t)i * out + o] : 0.f;
-        }
-      }
+      for (int ot = 0; ot < RG_HT; ++ot)
+        mv[ot] = (it < nti && ot < nto) ? mrow[(it * nto + ot) * 64] : f32x4{0.f, 0.f, 0.f, 0.f};
+    };
+    float ha[2], han[2];
+    f32x4 mv[RG_HT], mvn[RG_HT];
+    fetch(0, ha, mv);
+    for (int it = 0; it < nti; ++it) {
+      fetch(it + 1, han, mvn);
 #pragma unroll
       for (int ot = 0; ot < RG_HT; ++ot) {
         if (ot < nto) {
           const int o = 16 * ot + r;
-          const float* m0v = m0a[ot];
-          f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb;
+          f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb, mn = kb;
           kb = rg_mfma(ha[0], db[ot][0], kb);
           kb = rg_mfma(ha[1], db[ot][1], kb);
           ke = rg_mfma(ha[0], de[ot][0], ke);
@@ -696,14 +724,23 @@ __device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, dou
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int i = 16 * it + 4 * g + v;
-            if (o < out && i < in)
-              rg_mu_entry<MODE>(w0 + (int64_t)i * out + o, kb[v], ke[v], dtf, atf, rtf, mu, mu_new,
-                                m0v[v], acc, fl);
+            if (o < out && i < in) {
+              float mvv = 0.f;
+              const float ev = rg_mu_val<MODE>(kb[v], ke[v], dtf, atf, rtf, mv[ot][v], mvv, fl);
+              mn[v] = mvv;
+              acc += ev * ev;
+            }
           }
+          if (MODE == 0) nrow[(it * nto + ot) * 64] = mn;
         }
       }
+      ha[0] = han[0];
+      ha[1] = han[1];
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) mv[ot] = mvn[ot];
     }
   }
+
   double sum = (double)acc, flg = (double)fl;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -712,6 +749,28 @@ __device__ double rg_mu_pass(const RaggedModel& m, const float* sbs, int p0, dou
   }
   if (flag) *flag = flg;
   return sum;
+}
+
+// A finished trajectory's row from the tile-major order to the flat
+// parameter order (its first P floats); `tmp` holds the tile-major row.
+__device__ void rg_row_to_flat(const RaggedModel& m, gp<const float> __restrict__ tmp,
+                               gp<float> __restrict__ row, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  for (int l = 0; l < m.L; ++l) {
+    const int in = m.in[l], out = m.out[l], nti = m.ti[l], nto = m.to[l];
+    const gp<const f32x4> src = (gp<const f32x4>)(tmp + m.nw0[l]) + lane;
+    for (int t = 0; t < nti * nto; ++t) {
+      const int it = t / nto, ot = t - it * nto;
+      const f32x4 v4 = src[t * 64];
+      const int o = 16 * ot + r;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * it + 4 * g + v;
+        if (o < out && i < in) row[m.w0[l] + (int64_t)i * out + o] = v4[v];
+      }
+    }
+    for (int o = lane; o < out; o += 64) row[m.b0[l] + o] = tmp[m.nb0[l] + o];
+  }
 }
 
 // u(tau) of the slot's trajectory from its forward tape: the stored state when
@@ -723,15 +782,17 @@ __device__ __forceinline__ void rg_tape_u(const RaggedAdjArgs& a, int cj, double
   const int g4 = 4 * (lane >> 4);
   int j = cj;
   double tj = tcj, tn = tnx;
-  while (tj > tau && a.rec_prev[j] >= 0) {
+  const gp<const int> rprev = glb(a.rec_prev);
+  const gp<const double> rt = glb(a.rec_t);
+  while (tj > tau && rprev[j] >= 0) {
     tn = tj;
-    j = a.rec_prev[j];
-    tj = a.rec_t[j];
+    j = rprev[j];
+    tj = rt[j];
   }
-  const float* rd = a.rec_data + (int64_t)j * 8 * a.Dp;
+  const gp<const float> rd = glb(a.rec_data) + (int64_t)j * 8 * a.Dp;
   if (tj == tau || !(tn > tj)) {
 #pragma unroll
-    for (int t = 0; t < DT; ++t) uu[t] = *(const f32x4*)(rd + 16 * t + g4);
+    for (int t = 0; t < DT; ++t) uu[t] = *(gp<const f32x4>)(rd + 16 * t + g4);
     return;
   }
   const double dts = tn - tj;
@@ -741,8 +802,8 @@ __device__ __forceinline__ void rg_tape_u(const RaggedAdjArgs& a, int cj, double
   for (int t = 0; t < DT; ++t) {
     f32x4 sm = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 7; ++i) sm += w[i] * *(const f32x4*)(rd + (1 + i) * a.Dp + 16 * t + g4);
-    uu[t] = *(const f32x4*)(rd + 16 * t + g4) + (float)dts * sm;
+    for (int i = 0; i < 7; ++i) sm += w[i] * *(gp<const f32x4>)(rd + (1 + i) * a.Dp + 16 * t + g4);
+    uu[t] = *(gp<const f32x4>)(rd + 16 * t + g4) + (float)dts * sm;
   }
 }
 
@@ -753,7 +814,7 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
   rg_load_weights(m, rg_lds);
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const int64_t wslot0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RG_SLOTS;
-  float* sb = a.sbuf + (wslot0 + r) * 8 * m.sv;
+  const gp<float> sb = glb(a.sbuf) + (wslot0 + r) * 8 * m.sv;
   const int D = a.D;
   const int64_t P = m.P;
   const double nel = (double)D + (double)P;
@@ -935,11 +996,11 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
       const int p0s = __shfl(p0, s);
       const int mfs = __shfl(muflip, s);
       const double dts = __shfl(c.dt, s);
-      const float* sbs = a.sbuf + (wslot0 + s) * 8 * m.sv;
-      float* row = a.dLdp_traj + (int64_t)trs * P;
-      float* tmp = a.mu_tmp + (wslot0 + s) * P;
-      const float* mu = mfs ? tmp : row;
-      float* mun = mfs ? row : tmp;
+      const gp<const float> sbs = glb(a.sbuf) + (wslot0 + s) * 8 * m.sv;
+      const gp<float> row = glb(a.dLdp_traj) + (int64_t)trs * m.Pn;
+      const gp<float> tmp = glb(a.mu_tmp) + (wslot0 + s) * m.Pn;
+      const gp<const float> mu = mfs ? tmp : row;
+      const gp<float> mun = mfs ? row : tmp;
       double fl = 0.0, sm;
       if (ms == 0) sm = rg_mu_pass<0>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, mun, nullptr, lane);
       else if (ms == 1) sm = rg_mu_pass<1>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, nullptr, lane);
@@ -1053,45 +1114,51 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
       }
       need_start = true;
     }
-    // the finished trajectory's parameter gradient must end in its own row
-    // (whole wave, converged)
-    const int fin_any = __any(fin && muflip);
-    if (fin_any) {
+    // the finished trajectory's parameter gradient ends in its own row in the
+    // flat parameter order (whole wave, converged): the tile-major row is
+    // first copied to the slot's row when it is not there already
+    if (__any(fin)) {
       for (int s = 0; s < RG_SLOTS; ++s) {
-        const int cp = __shfl((int)(fin && muflip), s);
-        if (!cp) continue;
+        if (!__shfl((int)fin, s)) continue;
         const int trs = __shfl(traj, s);
-        const float* tmp = a.mu_tmp + (wslot0 + s) * P;
-        float* row = a.dLdp_traj + (int64_t)trs * P;
-        for (int64_t i = lane; i < P; i += 64) row[i] = tmp[i];
+        const gp<float> tmp = glb(a.mu_tmp) + (wslot0 + s) * m.Pn;
+        const gp<float> row = glb(a.dLdp_traj) + (int64_t)trs * m.Pn;
+        if (!__shfl(muflip, s)) {
+          for (int64_t i = 4 * lane; i < m.Pn; i += 256) *(gp<f32x4>)(tmp + i) = *(gp<const f32x4>)(row + i);
+          __threadfence_block();
+        }
+        rg_row_to_flat(m, tmp, row, lane);
       }
     }
   }
 }
 
-void launch_ragged_adj(const RaggedAdjArgs& a, int grid, hipStream_t s) {
+// `waves` (1..4) waves of RG_SLOTS slots per workgroup
+void launch_ragged_adj(const RaggedAdjArgs& a, int grid, int waves, hipStream_t s) {
   const size_t lds = (size_t)a.m.lds_floats * sizeof(float);
   if (a.Dp <= 16) {
     (void)hipFuncSetAttribute((const void*)k_ragged_adj<1>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_ragged_adj<1>, dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(k_ragged_adj<1>, dim3(grid), dim3(64 * waves), lds, s, a);
   } else {
     (void)hipFuncSetAttribute((const void*)k_ragged_adj<2>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_ragged_adj<2>, dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(k_ragged_adj<2>, dim3(grid), dim3(64 * waves), lds, s, a);
   }
 }
 
-// dL/dp = sum over trajectories of their FP32 rows, in trajectory order, in FP64
-__global__ void k_ragged_sum(const float* rows, int64_t n, int64_t P, double* out) {
+// dL/dp = sum over trajectories of their FP32 rows (first P of every `stride`
+// floats), in trajectory order, in FP64
+__global__ void k_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P;
        i += (int64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (int64_t k = 0; k < n; ++k) s += (double)rows[k * P + i];
+    for (int64_t k = 0; k < n; ++k) s += (double)rows[k * stride + i];
     out[i] = s;
   }
 }
-void launch_ragged_sum(const float* rows, int64_t n, int64_t P, double* out, hipStream_t s) {
+void launch_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* out,
+                       hipStream_t s) {
   const int64_t nb = std::min<int64_t>((P + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_ragged_sum, dim3((int)nb), dim3(256), 0, s, rows, n, P, out);
+  hipLaunchKernelGGL(k_ragged_sum, dim3((int)nb), dim3(256), 0, s, rows, n, P, stride, out);
 }
