@@ -9,12 +9,12 @@ from .neural_ode import (NeuralODE, Chain, Dense, Cube, Tsit5, InterpolatingAdjo
                          GaussAdjoint, ODESolution, save_plan, AugmentedNDELayer, DimMover)
 from .handle import Handle
 from ._lib import load, NodeHipError
-from .ffjord import FFJORD
+from .ffjord import FFJORD, FFJORDDistribution
 from .multiple_shooting import (group_ranges, multiple_shoot, EnsembleProblem, EnsembleThreads,
                                 EnsembleSerial, EnsembleGPUArray)
 
 __all__ = ["NeuralODE", "Chain", "Dense", "Cube", "Tsit5", "InterpolatingAdjoint",
            "ZygoteVJP", "BacksolveAdjoint", "TrackerAdjoint", "QuadratureAdjoint",
            "GaussAdjoint", "ODESolution", "save_plan", "AugmentedNDELayer", "DimMover",
-           "Handle", "load", "NodeHipError", "FFJORD", "group_ranges", "multiple_shoot",
+           "Handle", "load", "NodeHipError", "FFJORD", "FFJORDDistribution", "group_ranges", "multiple_shoot",
            "EnsembleProblem", "EnsembleThreads", "EnsembleSerial", "EnsembleGPUArray"]

@@ -24,6 +24,9 @@ Differentiation: with ``x`` or ``ps`` a ``torch.Tensor`` the solve is a
 the augmented solve (the reference's ``sensealg``, ``:143``) with the same
 probe, including the second-order VJP of the Hutchinson term; ``logpx`` is
 then assembled from ``z(T)`` and ``Δlogp`` in torch.
+
+``FFJORDDistribution`` (``:216-246``) samples by the reversed-time solve and
+evaluates densities by the forward solve, on the same handles.
 """
 import numpy as np
 
@@ -88,10 +91,7 @@ class FFJORD:
             if hasattr(x_in, "detach"):
                 x_in = x_in.reshape(1, -1)
         B = x.shape[0]
-        probe = st.get("probe")
-        if probe is None:
-            rng = st.get("rng") or np.random.default_rng()
-            probe = rng.choice(np.array([-1.0, 1.0], np.float32), size=(B, self.D))
+        probe = _probe(st, B, self.D)
         h = self._handle(B, reg, mc)
         try:
             import torch
@@ -116,9 +116,10 @@ class FFJORD:
             return (lp.reshape(1, B), l1.reshape(1, B), l2.reshape(1, B)), st
         return (lp.reshape(1, B), dl, dl), st
 
-    def _solve(self, h, x, probe):
+    def _solve(self, h, x, probe, reverse=False):
         kw = self.kwargs
-        lp, z, dl, stats = h.cnf_forward(x, probe, self.tspan[0], self.tspan[1],
+        t0, t1 = self.tspan[::-1] if reverse else self.tspan
+        lp, z, dl, stats = h.cnf_forward(x, probe, t0, t1,
                                          abstol=kw.get("abstol", 1e-6), reltol=kw.get("reltol", 1e-3),
                                          adaptive=kw.get("adaptive", True), dt=kw.get("dt", 0.0),
                                          dtmax=kw.get("dtmax", 0.0), dtmin=kw.get("dtmin", 0.0),
@@ -129,6 +130,65 @@ class FFJORD:
         if getattr(h, "cnf_reg", False):
             l1, l2 = h.cnf_regularizers()
         return lp, z, dl, l1, l2, stats
+
+
+def _probe(st, B, D):
+    probe = st.get("probe")
+    if probe is None:
+        rng = st.get("rng") or np.random.default_rng()
+        probe = rng.choice(np.array([-1.0, 1.0], np.float32), size=(B, D))
+    return probe
+
+
+class FFJORDDistribution:
+    """``FFJORDDistribution(model, ps, st)`` (``src/ffjord.jl:216-246``): the
+    flow as a distribution over the data space.
+
+    * ``rand(n, rng)`` draws ``n`` standard-normal base samples and carries
+      them through the flow backwards in time (``__backward_ffjord``,
+      ``:183-214``: the same augmented solve over ``reverse(tspan)``), one
+      batched HIP solve; returns ``(n, D)`` (a Julia ``D x n`` matrix), or
+      ``(D,)`` when ``n`` is None.
+    * ``_logpdf(x)`` is the reference's ``first(first(__forward_ffjord(...)))``
+      (``:235-240``): the first sample's log-likelihood; ``logpdf(x)`` gives
+      every sample's (a vector for a ``(B, D)`` batch, as Distributions'
+      ``logpdf`` over the columns of a matrix), from one batched solve.
+    ``st`` as for ``FFJORD`` (``regularize``, ``monte_carlo``, ``rng`` /
+    ``probe``)."""
+
+    def __init__(self, model, ps, st=None):
+        self.model, self.ps = model, ps
+        self.st = dict(st or {"regularize": False, "monte_carlo": True})
+
+    def __len__(self):
+        return int(np.prod(self.model.input_dims))
+
+    @property
+    def dtype(self):
+        return np.asarray(self.ps).dtype
+
+    def logpdf(self, x):
+        x = np.asarray(x.detach().cpu().numpy() if hasattr(x, "detach") else x, np.float32)
+        (lp, _, _), _ = self.model(x, np.asarray(self.ps, np.float32), self.st)
+        lp = np.asarray(lp).reshape(-1)
+        return float(lp[0]) if x.ndim == 1 else lp
+
+    def _logpdf(self, x):
+        return float(np.asarray(self.logpdf(x)).reshape(-1)[0])
+
+    def rand(self, n=None, rng=None):
+        ffj = self.model
+        B = 1 if n is None else int(n)
+        rng = rng if rng is not None else (self.st.get("rng") or np.random.default_rng())
+        x = rng.standard_normal((B, ffj.D)).astype(np.float32)
+        reg = bool(self.st.get("regularize", False))
+        mc = bool(self.st.get("monte_carlo", True))
+        probe = _probe(self.st, B, ffj.D) if (mc or reg) else None
+        h = ffj._handle(B, reg, mc)
+        h.set_params(np.asarray(self.ps, np.float32))
+        _, z, _, _, _, stats = ffj._solve(h, x, probe, reverse=True)
+        self.st["stats"] = stats
+        return z[0] if n is None else z
 
 
 def _FFJORDFn_factory():

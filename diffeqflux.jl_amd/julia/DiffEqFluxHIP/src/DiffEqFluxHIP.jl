@@ -35,13 +35,19 @@ the exact trace, `NODE_MAP_CNF_EXACT_TRACE`; `regularize = true`: the two
 regulariser rows, `NODE_MAP_CNF_REGULARIZE`, returned as λ₁, λ₂ -- the rows,
 where the reference's `selectdim(pred, N, ...)` picks sample columns), its
 pullback to `node_cnf_backward` (the regularisers carry no gradient there: a
-non-zero cotangent for λ₁ / λ₂ raises).  Julia was not available where this package was written; the
-call contract it binds is exercised from Python through the same C ABI.
+non-zero cotangent for λ₁ / λ₂ raises).  `FFJORDDistribution` of such an
+`FFJORD` (src/ffjord.jl:216-246) samples through the same solve over the
+reversed time span (`rand`, as `__backward_ffjord`, :183-214) and evaluates
+`logpdf` through the forward solve.  Julia was not available where this
+package was written; the call contract it binds is exercised from Python
+through the same C ABI.
 """
 module DiffEqFluxHIP
 
 using ChainRulesCore: ChainRulesCore, NoTangent, ZeroTangent, unthunk
-using DiffEqFlux: DiffEqFlux, NeuralODE, FFJORD
+using DiffEqFlux: DiffEqFlux, NeuralODE, FFJORD, FFJORDDistribution
+using Distributions: Distributions
+using Random: AbstractRNG
 using Lux: Lux, Chain, Dense, WrappedFunction
 using OrdinaryDiffEq: Tsit5
 using SciMLBase: SciMLBase, ODEProblem, ODEFunction, ReturnCode
@@ -388,19 +394,55 @@ function make_opts(n::FFJORD)
                      Float64(get(kw, :dtmin, 0.0)), Int64(get(kw, :maxiters, 0)))
 end
 
+flat_params(ps) = ps isa AbstractVector ? Vector{Float32}(ps) :
+    reduce(vcat, [vcat(vec(l.weight), l.bias) for l in values(ps)
+                  if l isa NamedTuple && haskey(l, :weight)])
+
 # the drop-in for FFJORD: one Gaussian probe per call, held for the whole solve
 # (the reference redraws randn! inside every right-hand-side call, :96)
 function (n::FFJORD{<:Chain})(x::AbstractMatrix{Float32}, ps, st)
     supported(n, x, st) || return invoke(n, Tuple{Any, Any, Any}, x, ps, st)  # reference path
     D, B = size(x)
     h = ffjord_handle_for(n, B, st.regularize, st.monte_carlo)
-    pflat = ps isa AbstractVector ? Vector{Float32}(ps) :
-        reduce(vcat, [vcat(vec(l.weight), l.bias) for l in values(ps)
-                      if l isa NamedTuple && haskey(l, :weight)])
+    pflat = flat_params(ps)
     e = CRC.@ignore_derivatives randn(Float32, D, B)
     lp, _, dl, l1, l2 = hip_ffjord(Matrix{Float32}(x), pflat, e, h, make_opts(n), st.regularize)
     return (lp, l1, l2), (; model = st.model, regularize = st.regularize,
                           monte_carlo = st.monte_carlo)
+end
+
+# ---- FFJORDDistribution (src/ffjord.jl:216-246)
+const FFJORDDistHIP = FFJORDDistribution{<:FFJORD{<:Chain}}
+
+# base samples x (D x n) through the flow backwards in time: the augmented
+# solve of `__backward_ffjord` (:183-214) over reverse(tspan), z returned
+function hip_ffjord_sample(n::FFJORD, x::Matrix{Float32}, pflat, st)
+    D, B = size(x)
+    h = ffjord_handle_for(n, B, st.regularize, st.monte_carlo)
+    o = make_opts(n)
+    rev = SolveOpts(o.t1, o.t0, o.abstol, o.reltol, o.adaptive, o.save_start, o.save_end,
+                    o.save_everystep, o.dt, o.dtmax, o.dtmin, o.maxiters)
+    _, z, _, _, _ = hip_ffjord(x, pflat, randn(Float32, D, B), h, rev, st.regularize)
+    return z
+end
+
+function Distributions._rand!(rng::AbstractRNG, d::FFJORDDistHIP, x::AbstractArray{<:Real})
+    n = d.model
+    nsamp = size(x, ndims(x))
+    base = randn(rng, Float32, n.input_dims..., nsamp)             # :186-191
+    supported(n, base, d.st) ||                                    # reference path
+        return invoke(Distributions._rand!, Tuple{AbstractRNG, FFJORDDistribution,
+                                                  AbstractArray{<:Real}}, rng, d, x)
+    copyto!(x, hip_ffjord_sample(n, base, flat_params(d.ps), d.st))
+    return x
+end
+
+# first(first(...)) as the reference (:235-240): the first sample's logpx
+function Distributions._logpdf(d::FFJORDDistHIP, x::AbstractVector)
+    return first(first(d.model(reshape(Vector{Float32}(x), :, 1), d.ps, d.st)))
+end
+function Distributions._logpdf(d::FFJORDDistHIP, x::AbstractArray)
+    return first(first(d.model(Matrix{Float32}(reshape(x, size(x, 1), :)), d.ps, d.st)))
 end
 
 end # module

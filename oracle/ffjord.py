@@ -22,6 +22,8 @@ Follows ``src/ffjord.jl``:
     with ``selectdim(pred, N, ...)``, the batch dimension, which for a matrix
     input picks two sample columns rather than the two regulariser rows; the
     rows -- what the regularisers are -- are returned here.)
+  * ``sample`` (``__backward_ffjord``, ``:183-214``): the same augmented
+    solve with the time span reversed, from base samples to data space.
   * ``backward``: the InterpolatingAdjoint of that solve (``:143``) with the
     second-order VJP of the Hutchinson term (``trace_vjp``) or of the exact
     trace and the regularisers (``rhs_vjp``: bilinear forms ``aᵀ J b`` through
@@ -127,6 +129,26 @@ def forward(mlp, p, x, t0, t1, probe=None, time_concat=False, regularize=False,
         sol.lam1, sol.lam2 = yT[:, D + 1], yT[:, D + 2]
     logpz = -0.5 * np.sum(z * z, axis=1) - 0.5 * D * math.log(2 * math.pi)
     return logpz - dl, z, dl, sol
+
+
+def sample(mlp, p, x, t0, t1, probe=None, time_concat=False, regularize=False,
+           monte_carlo=None, **solver_kw):
+    """FFJORD sampling (``__backward_ffjord``, ``src/ffjord.jl:183-214``):
+    base samples ``x`` (B, D) (the reference draws them from ``basedist``,
+    standard normal by default, ``:186-191``) carried through the same
+    augmented solve with the time span reversed (``reverse(n.tspan)``,
+    ``:206``), from ``[x; 0]`` at ``t1`` to ``t0``; returns (z (B, D), Solution).
+    ``Δlogp`` (and the regulariser rows) are integrated as in ``forward`` and
+    dropped (``:211-213``).  Pinned by the linear flow ``z = e^{-AT} x`` and by
+    ``forward(sample(x)) = x`` on a nonlinear flow (tests/test_oracle_ffjord.py)."""
+    B, D = x.shape
+    if monte_carlo is None:
+        monte_carlo = probe is not None
+    f = ffjord_rhs(mlp, p, D, probe, time_concat, regularize, monte_carlo)
+    extra = 3 if regularize else 1
+    y0 = np.concatenate([x, np.zeros((B, extra), dtype=x.dtype)], axis=1)
+    sol = tsit5.solve(f, y0, t1, t0, save_start=False, save_end=True, **solver_kw)
+    return np.asarray(sol.u[-1])[:, :D], sol
 
 
 # ---------------------------------------------------------------- adjoint

@@ -6,7 +6,10 @@
 * Hutchinson: the mean of eᵀJe over all 2^D Rademacher probes is exactly
   tr(J) (E[e eᵀ] = I), J the brute-force finite-difference Jacobian of a
   nonlinear MLP;
-* time concatenation: the model sees [z; t] (a planted drop of t fails).
+* time concatenation: the model sees [z; t] (a planted drop of t fails);
+* sampling (reversed-time solve): z = e^{-AT} x on the linear flow, and
+  forward(sample(x)) = x on a nonlinear time-concatenated flow (a sample
+  solved forward in time instead, the plausible slip, fails both).
 """
 import math
 
@@ -279,3 +282,25 @@ def test_regularised_exact_adjoint_vs_central_fd(monte_carlo):
             d = np.zeros_like(x); d[b, c] = h
             fx.append((loss(p, x + d) - loss(p, x - d)) / (2 * h))
     assert np.abs(np.array(fx).reshape(B, D) - gx).max() < 1e-6 * max(1.0, np.abs(gx).max())
+
+
+def test_sampling_linear_flow_and_round_trip():
+    rng = np.random.default_rng(7)
+    D, B, T = 3, 6, 0.9
+    m, p, A = linear_model(D, rng)
+    x = rng.standard_normal((B, D))
+    z, sol = FF.sample(m, p, x, 0.0, T, abstol=1e-12, reltol=1e-12)
+    want = x @ expm(-A * T).T
+    assert np.abs(z - want).max() < 1e-9
+    assert sol.t[-1] == 0.0
+    # the slip of integrating forward in time lands on e^{+AT} x
+    assert np.abs(x @ expm(A * T).T - want).max() > 1e-2
+    # nonlinear time-concatenated flow: sampling inverts the forward map
+    mn = MLP([D + 1, 12, D], ["tanh", "identity"])
+    pn = mn.glorot_init(rng)
+    e = rng.choice([-1.0, 1.0], size=(B, D))
+    zs, _ = FF.sample(mn, pn, x, 0.0, T, probe=e, time_concat=True, abstol=1e-11, reltol=1e-11)
+    _, xr, _, _ = FF.forward(mn, pn, zs, 0.0, T, probe=e, time_concat=True,
+                             abstol=1e-11, reltol=1e-11)
+    assert np.abs(xr - x).max() < 1e-8
+    assert np.abs(zs - x).max() > 1e-2   # the flow moved the samples
