@@ -598,19 +598,36 @@ __device__ __forceinline__ float rg_mu_val(float kb, float ke, float dtf, float 
 //  MODE 2: returns sum ((g_1 - g_0) / (atol + |mu| rtol))^2   (initial dt, d2;
 //          g_1 in block 7 - p0), *flag = 1 where g_1 != g_0
 // The pass streams the row: every layer's bias entries (their loads issued
-// together), then each layer's weight tiles input block by input block, the
-// next block's loads issued before the current one is formed.  The stage sums
-// of a weight tile are outer products over the stages j:
+// together), then each layer's weight tiles in batches, the next batch's
+// loads issued before the current one is formed.  It is an outlined call:
+// inlined into the adjoint kernel, which runs at its register limit, its
+// address registers were spilled and every load waited for the one before.
+// The stage sums of a weight tile are outer products over the stages j:
 //   k_b[i][o] = sum_j cb_j delta_j[o] h_j[i],  k_e likewise with ce_j,
 // a GEMM with K = 7 stages (padded to 8): two v_mfma_f32_16x16x4f32 each,
 // A[i][j] = h_j[i], B[j][o] = c_j delta_j[o].  Lane (r, g) supplies stage
 // j = 4q + g of MFMA step q; the accumulator gives it output o = r and inputs
 // i = 4g + v, which the row stores as this lane's 16-byte vector of the tile
 // (padding entries outside the layer hold 0 and add no error).
+// The model struct in the kernel's argument segment (RaggedAdjArgs is the
+// launch's only argument and the model its first member), read through a
+// wave-uniform constant-address-space pointer: a device function gets its
+// arguments in vector registers, so a struct passed to it would be read with
+// per-lane FLAT loads and its layer sizes would be divergent branch
+// conditions.  `kargs` is the kernel's __builtin_amdgcn_kernarg_segment_ptr().
+typedef __attribute__((address_space(4))) const RaggedModel CRaggedModel;
+__device__ __forceinline__ CRaggedModel& rg_uniform(uint64_t kargs) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)kargs);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(kargs >> 32));
+  return *(CRaggedModel*)(((uint64_t)hi << 32) | lo);
+}
+
+constexpr int RG_NB = 8;   // weight tiles per batch
 template <int MODE>
-__device__ double rg_mu_pass(const RaggedModel& m, gp<const float> sbs, int p0, double dt,
+__device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const float> sbs, int p0, double dt,
                              double atol, double rtol, gp<const float> __restrict__ mu,
                              gp<float> __restrict__ mu_new, double* flag, int lane) {
+  CRaggedModel& mm = rg_uniform(kargs);
   const int r = lane & 15, g = lane >> 4;
   int boff[2];
   float cb[2], ce[2];
@@ -635,109 +652,119 @@ __device__ double rg_mu_pass(const RaggedModel& m, gp<const float> sbs, int p0, 
       cb[q] = 0.f;
       ce[q] = j == 0 ? -1.f : 1.f;  // g_1 - g_0
     }
-    boff[q] = blk * m.sv;
+    boff[q] = blk * mm.sv;
   }
   const float dtf = (float)dt, atf = (float)atol, rtf = (float)rtol;
   float acc = 0.f, fl = 0.f;
 
   // ---- biases: lanes over the outputs o of each layer (at most two 64-wide
-  // rounds: hidden widths <= 128, D <= 32); the loads of every layer's rounds
-  // are issued before any entry is formed
+  // rounds: hidden widths <= 128, D <= 32).  Every load is unconditional (an
+  // index out of range is clamped to a valid entry and its value dropped):
+  // a load under a lane-dependent branch makes the wave wait for everything
+  // in flight where the branch joins.
+  const int Lm1 = mm.L - 1;
   float bm[RG_MAXL][2], bkb[RG_MAXL][2], bke[RG_MAXL][2];
 #pragma unroll
   for (int l = 0; l < RG_MAXL; ++l)
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      bm[l][c] = bkb[l][c] = bke[l][c] = 0.f;
-      const int o = 64 * c + lane;
-      if (l < m.L && o < m.out[l]) {
-        const int off = m.sd[l] + o;
-        bm[l][c] = mu[m.nb0[l] + o];
-        if (MODE == 0) {
+      const int lc = l < mm.L ? l : Lm1, out = mm.out[lc];
+      const int o = 64 * c + lane, oc = o < out ? o : out - 1;
+      const float keep = (l < mm.L && o < out) ? 1.f : 0.f;
+      const int off = mm.sd[lc] + oc;
+      bm[l][c] = mu[mm.nb0[lc] + oc];
+      bkb[l][c] = bke[l][c] = 0.f;
+      if (MODE == 0) {
 #pragma unroll
-          for (int j = 0; j < 7; ++j) {
-            const int bj = j == 0 ? p0 : (j == 6 ? 7 - p0 : j);
-            const float d = sbs[bj * m.sv + off];
-            bkb[l][c] += (j < 6 ? cA[6][j] : 0.f) * d;
-            bke[l][c] += cBT[j] * d;
-          }
-        } else if (MODE == 1) {
-          bkb[l][c] = sbs[p0 * m.sv + off];
-        } else {
-          bke[l][c] = sbs[(7 - p0) * m.sv + off] - sbs[p0 * m.sv + off];
+        for (int j = 0; j < 7; ++j) {
+          const int bj = j == 0 ? p0 : (j == 6 ? 7 - p0 : j);
+          const float d = sbs[bj * mm.sv + off];
+          bkb[l][c] += (j < 6 ? cA[6][j] : 0.f) * d;
+          bke[l][c] += cBT[j] * d;
         }
+      } else if (MODE == 1) {
+        bkb[l][c] = sbs[p0 * mm.sv + off];
+      } else {
+        bke[l][c] = sbs[(7 - p0) * mm.sv + off] - sbs[p0 * mm.sv + off];
       }
+      bkb[l][c] *= keep;
+      bke[l][c] *= keep;
     }
 #pragma unroll
   for (int l = 0; l < RG_MAXL; ++l)
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int o = 64 * c + lane;
-      if (l < m.L && o < m.out[l]) {
+      if (l < mm.L && o < mm.out[l]) {
         float mn = 0.f;
         const float ev = rg_mu_val<MODE>(bkb[l][c], bke[l][c], dtf, atf, rtf, bm[l][c], mn, fl);
-        if (MODE == 0) mu_new[m.nb0[l] + o] = mn;
+        if (MODE == 0) mu_new[mm.nb0[l] + o] = mn;
         acc += ev * ev;
       }
     }
 
-  // ---- weight tiles, layer by layer: the B operands c_j delta_j[16 ot + r]
-  // of every output block once per layer, then the input blocks it in order,
-  // the next block's mu vectors and A operands (h_j[16 it + r]) loaded before
-  // the current block is formed
-  for (int l = 0; l < m.L; ++l) {
-    const int in = m.in[l], out = m.out[l], nti = m.ti[l], nto = m.to[l];
-    float db[RG_HT][2], de[RG_HT][2];
+  // ---- weight tiles, layer by layer: the stage operands of every block
+  // (delta_j[16 ot + r], h_j[16 it + r] of the lane's two stages) in one
+  // batch, then the layer's tiles T = it * to + ot in batches of RG_NB, the
+  // next batch's mu vectors loaded before the current batch is formed
+  for (int l = 0; l < mm.L; ++l) {
+    const int in = mm.in[l], out = mm.out[l], nti = mm.ti[l], nto = mm.to[l], ntl = nti * nto;
+    // (unconditional loads of clamped indices, unwanted values zeroed by a
+    // multiply -- a select here compiles to branches, see the bias part;
+    // every clamped index reads finite stage data)
+    float dl[RG_HT][2], hl[RG_HT][2];
 #pragma unroll
-    for (int ot = 0; ot < RG_HT; ++ot)
+    for (int t = 0; t < RG_HT; ++t)
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        float d = 0.f;
-        if (ot < nto && use[q]) d = sbs[boff[q] + m.sd[l] + 16 * ot + r];
-        db[ot][q] = cb[q] * d;
-        de[ot][q] = ce[q] * d;
+        const float wq = use[q] ? 1.f : 0.f;
+        dl[t][q] = (t < nto ? wq : 0.f) * sbs[boff[q] + mm.sd[l] + 16 * (t < nto ? t : nto - 1) + r];
+        hl[t][q] = (t < nti ? wq : 0.f) * sbs[boff[q] + mm.sh[l] + 16 * (t < nti ? t : nti - 1) + r];
       }
-    const gp<const f32x4> mrow = (gp<const f32x4>)(mu + m.nw0[l]) + lane;
-    const gp<f32x4> nrow = (gp<f32x4>)(mu_new + m.nw0[l]) + lane;
-    auto fetch = [&](int it, float (&ha)[2], f32x4 (&mv)[RG_HT]) {
+    const gp<const f32x4> mrow = (gp<const f32x4>)(mu + mm.nw0[l]) + lane;
+    const gp<f32x4> nrow = (gp<f32x4>)(mu_new + mm.nw0[l]) + lane;
+    f32x4 cur[RG_NB], nxt[RG_NB];
 #pragma unroll
-      for (int q = 0; q < 2; ++q)
-        ha[q] = (it < nti && use[q]) ? sbs[boff[q] + m.sh[l] + 16 * it + r] : 0.f;
+    for (int k = 0; k < RG_NB; ++k) cur[k] = mrow[(k < ntl ? k : ntl - 1) * 64];
+    for (int T0 = 0; T0 < ntl; T0 += RG_NB) {
 #pragma unroll
-      for (int ot = 0; ot < RG_HT; ++ot)
-        mv[ot] = (it < nti && ot < nto) ? mrow[(it * nto + ot) * 64] : f32x4{0.f, 0.f, 0.f, 0.f};
-    };
-    float ha[2], han[2];
-    f32x4 mv[RG_HT], mvn[RG_HT];
-    fetch(0, ha, mv);
-    for (int it = 0; it < nti; ++it) {
-      fetch(it + 1, han, mvn);
+      for (int k = 0; k < RG_NB; ++k) {
+        const int T = T0 + RG_NB + k;
+        nxt[k] = mrow[(T < ntl ? T : ntl - 1) * 64];
+      }
 #pragma unroll
-      for (int ot = 0; ot < RG_HT; ++ot) {
-        if (ot < nto) {
+      for (int k = 0; k < RG_NB; ++k) {
+        const int T = T0 + k;
+        if (T < ntl) {
+          const int it = T / nto, ot = T - it * nto;
+          // this block's operands out of the per-layer registers (uniform index)
+          float h0 = 0.f, h1 = 0.f, d0 = 0.f, d1 = 0.f;
+#pragma unroll
+          for (int t = 0; t < RG_HT; ++t) {
+            if (t == it) { h0 = hl[t][0]; h1 = hl[t][1]; }
+            if (t == ot) { d0 = dl[t][0]; d1 = dl[t][1]; }
+          }
           const int o = 16 * ot + r;
           f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb, mn = kb;
-          kb = rg_mfma(ha[0], db[ot][0], kb);
-          kb = rg_mfma(ha[1], db[ot][1], kb);
-          ke = rg_mfma(ha[0], de[ot][0], ke);
-          ke = rg_mfma(ha[1], de[ot][1], ke);
+          kb = rg_mfma(h0, cb[0] * d0, kb);
+          kb = rg_mfma(h1, cb[1] * d1, kb);
+          ke = rg_mfma(h0, ce[0] * d0, ke);
+          ke = rg_mfma(h1, ce[1] * d1, ke);
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int i = 16 * it + 4 * g + v;
             if (o < out && i < in) {
               float mvv = 0.f;
-              const float ev = rg_mu_val<MODE>(kb[v], ke[v], dtf, atf, rtf, mv[ot][v], mvv, fl);
+              const float ev = rg_mu_val<MODE>(kb[v], ke[v], dtf, atf, rtf, cur[k][v], mvv, fl);
               mn[v] = mvv;
               acc += ev * ev;
             }
           }
-          if (MODE == 0) nrow[(it * nto + ot) * 64] = mn;
+          if (MODE == 0) nrow[T * 64] = mn;
         }
       }
-      ha[0] = han[0];
-      ha[1] = han[1];
 #pragma unroll
-      for (int ot = 0; ot < RG_HT; ++ot) mv[ot] = mvn[ot];
+      for (int k = 0; k < RG_NB; ++k) cur[k] = nxt[k];
     }
   }
 
@@ -809,8 +836,10 @@ __device__ __forceinline__ void rg_tape_u(const RaggedAdjArgs& a, int cj, double
 
 template <int DT>
 __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
+  static_assert(offsetof(RaggedAdjArgs, m) == 0, "the model leads the argument struct");
   extern __shared__ __attribute__((aligned(16))) float rg_lds[];
   const RaggedModel& m = a.m;
+  const uint64_t kargs = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
   rg_load_weights(m, rg_lds);
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const int64_t wslot0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RG_SLOTS;
@@ -1002,9 +1031,9 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
       const gp<const float> mu = mfs ? tmp : row;
       const gp<float> mun = mfs ? row : tmp;
       double fl = 0.0, sm;
-      if (ms == 0) sm = rg_mu_pass<0>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, mun, nullptr, lane);
-      else if (ms == 1) sm = rg_mu_pass<1>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, nullptr, lane);
-      else sm = rg_mu_pass<2>(m, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, &fl, lane);
+      if (ms == 0) sm = rg_mu_pass<0>(kargs, sbs, p0s, dts, a.abstol, a.reltol, mu, mun, nullptr, lane);
+      else if (ms == 1) sm = rg_mu_pass<1>(kargs, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, nullptr, lane);
+      else sm = rg_mu_pass<2>(kargs, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, &fl, lane);
       if (r == s) { smu = sm; flmu = fl; }
     }
     // ---- consume, part 2: control
