@@ -1601,6 +1601,15 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   const int waves = (int)std::min<int64_t>(4, std::max<int64_t>(1, (waves_all + h->n_cu - 1) / h->n_cu));
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(h->n_cu, (waves_all + waves - 1) / waves));
   const int64_t slots = (int64_t)grid * waves * RG_SLOTS;
+  // act' of the VJPs in LDS beside the weights when every wave's area fits
+  int act_tiles = 0;
+  for (int l = 0; l < m.L; ++l) {
+    m.act_off[l] = act_tiles;
+    act_tiles += m.to[l];
+  }
+  m.act_stride = act_tiles * 256;
+  if (((int64_t)m.lds_floats + (int64_t)waves * m.act_stride) * (int64_t)sizeof(float) > 160 * 1024)
+    m.act_stride = 0;
   if (grow(h, &b.dLdout, std::max<int64_t>(ns * D, 1), b.dLdout_cap, std::max<int64_t>(ns * D, 64)) ||
       grow(h, &b.dLdu0, n * D, b.dLdu0_cap, n * D) ||
       grow(h, &b.rows, n * Pn, b.rows_cap, n * Pn) ||

@@ -323,6 +323,11 @@ struct RaggedModel {
   // trajectory's row is rewritten in the flat parameter order (first P floats).
   int64_t nw0[RG_MAXL], nb0[RG_MAXL];
   int64_t Pn;
+  // adjoint VJP: act' of the forward sweep parked in LDS after the weights,
+  // act_stride floats per wave (tile t of layer l at act_off[l] + t, 256
+  // floats per tile); 0: parked in the stage block in global memory
+  int act_off[RG_MAXL];
+  int act_stride;
 };
 struct RaggedFwdArgs {
   RaggedModel m;

@@ -478,9 +478,10 @@ void launch_ragged_fwd(const RaggedFwdArgs& a, int grid, hipStream_t s) {
 // One VJP for the 16 slot columns: x0 = map(u) (first ti[0] tiles), lam (first
 // to[L-1] tiles).  Stores the layer inputs h_l and the pre-activation
 // gradients delta_l of this evaluation in the slot's stage block `blk`
-// (act' is parked there during the forward sweep) and returns
+// (act' is parked in the wave's LDS area `act_lds` during the forward sweep,
+// or in the block's delta area when m.act_stride is 0) and returns
 // g = W_0^T delta_0 (the input-map derivative is applied by the caller).
-__device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
+__device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, float* act_lds,
                                        const f32x4 (&x0)[RG_HT], const f32x4 (&lam)[RG_HT],
                                        gp<float> blk, f32x4 (&gout)[RG_HT], int lane) {
   const int r = lane & 15, g4 = 4 * (lane >> 4);
@@ -521,7 +522,8 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
           y[v] = act_f(act, acc[v]);
           sd[v] = act_d(act, acc[v], y[v]);
         }
-        *(gp<f32x4>)(blk + m.sd[l] + 16 * ot + g4) = sd;
+        if (m.act_stride) *(f32x4*)(act_lds + (m.act_off[l] + ot) * 256 + 4 * lane) = sd;
+        else *(gp<f32x4>)(blk + m.sd[l] + 16 * ot + g4) = sd;
         if (l < L - 1) *(gp<f32x4>)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
         hout[ot] = y;
       }
@@ -540,7 +542,9 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds,
     for (int ot = 0; ot < RG_HT; ++ot) {
       if (ot < nto) {
         gp<float> pd = blk + m.sd[l] + 16 * ot + g4;
-        const f32x4 d = gi[ot] * *(gp<const f32x4>)pd;
+        const f32x4 ad = m.act_stride ? *(const f32x4*)(act_lds + (m.act_off[l] + ot) * 256 + 4 * lane)
+                                      : *(gp<const f32x4>)pd;
+        const f32x4 d = gi[ot] * ad;
         *(gp<f32x4>)pd = d;
         gi[ot] = d;
       }
@@ -945,7 +949,8 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
         lg[t] = ls[t];
       }
     }
-    rg_vjp(m, rg_lds, x0, lg, sb + blkid * m.sv, gv, lane);
+    rg_vjp(m, rg_lds, rg_lds + m.lds_floats + (threadIdx.x >> 6) * m.act_stride, x0, lg,
+           sb + blkid * m.sv, gv, lane);
     // k = -(lambda^T df/du): input-map derivative 3u^2 for x.^3
     f32x4 kn[DT];
 #pragma unroll
@@ -1164,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
 
 // `waves` (1..4) waves of RG_SLOTS slots per workgroup
 void launch_ragged_adj(const RaggedAdjArgs& a, int grid, int waves, hipStream_t s) {
-  const size_t lds = (size_t)a.m.lds_floats * sizeof(float);
+  const size_t lds = ((size_t)a.m.lds_floats + (size_t)waves * a.m.act_stride) * sizeof(float);
   if (a.Dp <= 16) {
     (void)hipFuncSetAttribute((const void*)k_ragged_adj<1>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
