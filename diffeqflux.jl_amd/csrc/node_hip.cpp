@@ -1434,9 +1434,12 @@ int ragged_model(node_handle* h, RaggedModel& m, std::string& why) {
   m.p = h->pflat;
   int64_t no = 0;
   for (int l = 0; l < m.L; ++l) {
+    m.rowout[l] = m.to[l] >= m.ti[l];
+    m.nrow[l] = m.rowout[l] ? m.ti[l] : m.to[l];
     m.nw0[l] = no;
-    no += (int64_t)m.ti[l] * m.to[l] * 256;
+    no += (int64_t)m.nrow[l] * RG_HT * 256;
   }
+  m.ntile = (int)(no / 256);
   for (int l = 0; l < m.L; ++l) {
     m.nb0[l] = no;
     no += m.out[l];

@@ -574,45 +574,61 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
   for (int t = 0; t < RG_HT; ++t) gout[t] = gi[t];
 }
 
+// One entry of the parameter part from its pre-scaled stage sums (MODE as in
+// rg_mu_pass): kb = -dt sum_j b_j g~_j (MODE 0) or the g_0 sum (MODE 1), ke
+// the error sum.  The error's denominator is kept at least FLT_MIN, so padded
+// entries (all zero) add 0 also for atol = 0; v_rcp_f32 (1 ulp) stands in for
+// the divide, whose IEEE expansion cost a dozen instructions per entry.
 template <int MODE>
-__device__ __forceinline__ float rg_mu_val(float kb, float ke, float dtf, float atf, float rtf,
-                                           float m0, float& mn, float& fl) {
-  const float gb = -kb, ge = -ke;
-  float e;
+__device__ __forceinline__ float rg_mu_err(float kb, float ke, float atf, float rtf, float m0,
+                                           float& mn, float& fl) {
+  float num, mx;
   if (MODE == 0) {
-    mn = m0 + dtf * gb;
-    e = __fdividef(dtf * ge, atf + fmaxf(fabsf(m0), fabsf(mn)) * rtf);
+    mn = m0 + kb;
+    mx = fmaxf(fabsf(m0), fabsf(mn));
+    num = ke;
   } else if (MODE == 1) {
-    e = __fdividef(gb, atf + fabsf(m0) * rtf);
+    mx = fabsf(m0);
+    num = kb;
   } else {
-    e = __fdividef(ge, atf + fabsf(m0) * rtf);
-    if (ge != 0.f) fl = 1.f;
+    mx = fabsf(m0);
+    num = ke;
+    fl = fmaxf(fl, ke != 0.f ? 1.f : 0.f);
   }
-  return e;
+  return num * __builtin_amdgcn_rcpf(fmaxf(fmaf(mx, rtf, atf), 1.17549435e-38f));
 }
 
 // Parameter part of one trajectory's adjoint state, run by the whole wave for
 // slot block `sbs` (8 stage blocks of m.sv floats; logical stage j of the
 // attempt sits in block j, except j = 0 in block p0 and j = 6 in block 7-p0),
-// over a row in the tile-major order of RaggedModel (mu, mu_new: Pn floats):
+// over a row in the tile-row order of RaggedModel (mu, mu_new: Pn floats):
 //  MODE 0: stage sums kb = sum_j b_j g_j, ke = sum_j btilde_j g_j with
 //          g_j = -(delta_j h_j^T) (weights) and -delta_j (bias); writes
 //          mu_new = mu + dt kb; returns sum (dt ke / (atol + max|mu|,|mu_new| rtol))^2
 //  MODE 1: returns sum (g_0 / (atol + |mu| rtol))^2           (initial dt, d1)
 //  MODE 2: returns sum ((g_1 - g_0) / (atol + |mu| rtol))^2   (initial dt, d2;
 //          g_1 in block 7 - p0), *flag = 1 where g_1 != g_0
+// The stage weights carry the factor of their use (-dt b_j for the update,
+// dt btilde_j for the error in MODE 0, -1 in MODES 1 and 2), so an entry is
+// finished by rg_mu_err alone.
 // The pass streams the row: every layer's bias entries (their loads issued
-// together), then each layer's weight tiles in batches, the next batch's
-// loads issued before the current one is formed.  It is an outlined call:
-// inlined into the adjoint kernel, which runs at its register limit, its
-// address registers were spilled and every load waited for the one before.
+// together), then the weight tiles in memory order, each tile's successor
+// one row on loaded as soon as the tile is read.  It is an outlined call: inlined
+// into the adjoint kernel, which runs at its register limit, its address
+// registers were spilled and every load waited for the one before.
 // The stage sums of a weight tile are outer products over the stages j:
 //   k_b[i][o] = sum_j cb_j delta_j[o] h_j[i],  k_e likewise with ce_j,
 // a GEMM with K = 7 stages (padded to 8): two v_mfma_f32_16x16x4f32 each,
 // A[i][j] = h_j[i], B[j][o] = c_j delta_j[o].  Lane (r, g) supplies stage
 // j = 4q + g of MFMA step q; the accumulator gives it output o = r and inputs
-// i = 4g + v, which the row stores as this lane's 16-byte vector of the tile
-// (padding entries outside the layer hold 0 and add no error).
+// i = 4g + v, which the row stores as this lane's 16-byte vector of the tile.
+// Operand rows beyond the layer's inputs / outputs are zeroed once per layer,
+// so a padded entry's (and a padding tile's) stage sums are exactly 0 and its
+// row entry stays 0 (rows start zeroed and every write stores the whole tile).
+// About one wave runs per SIMD, so the pass is bound by instruction issue:
+// a layer's tile rows run along its longer tile dimension, the operand of the
+// row index is picked once per row and the column index is the unrolled loop
+// counter (no per-tile index arithmetic, selects or branches).
 // The model struct in the kernel's argument segment (RaggedAdjArgs is the
 // launch's only argument and the model its first member), read through a
 // wave-uniform constant-address-space pointer: a device function gets its
@@ -626,39 +642,37 @@ __device__ __forceinline__ CRaggedModel& rg_uniform(uint64_t kargs) {
   return *(CRaggedModel*)(((uint64_t)hi << 32) | lo);
 }
 
-constexpr int RG_NB = 8;   // weight tiles per batch
 template <int MODE>
 __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const float> sbs, int p0, double dt,
                              double atol, double rtol, gp<const float> __restrict__ mu,
                              gp<float> __restrict__ mu_new, double* flag, int lane) {
   CRaggedModel& mm = rg_uniform(kargs);
   const int r = lane & 15, g = lane >> 4;
+  const float dtf = (float)dt, atf = (float)atol, rtf = (float)rtol;
+  const float sb = MODE == 0 ? -dtf : -1.f, se = MODE == 0 ? dtf : -1.f;
   int boff[2];
   float cb[2], ce[2];
-  bool use[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int j = 4 * q + g;
     int blk;
     if (MODE == 0) {
-      use[q] = j < 7;
       blk = j == 0 ? p0 : (j == 6 ? 7 - p0 : (j < 7 ? j : 0));
       cb[q] = j < 6 ? cA[6][j] : 0.f;
       ce[q] = j < 7 ? cBT[j] : 0.f;
     } else if (MODE == 1) {
-      use[q] = j == 0;
       blk = p0;
-      cb[q] = 1.f;   // g_0
+      cb[q] = j == 0 ? 1.f : 0.f;   // g_0
       ce[q] = 0.f;
     } else {
-      use[q] = j < 2;
       blk = j == 0 ? p0 : 7 - p0;
       cb[q] = 0.f;
-      ce[q] = j == 0 ? -1.f : 1.f;  // g_1 - g_0
+      ce[q] = j == 0 ? -1.f : (j == 1 ? 1.f : 0.f);  // g_1 - g_0
     }
     boff[q] = blk * mm.sv;
+    cb[q] *= sb;
+    ce[q] *= se;
   }
-  const float dtf = (float)dt, atf = (float)atol, rtf = (float)rtol;
   float acc = 0.f, fl = 0.f;
 
   // ---- biases: lanes over the outputs o of each layer (at most two 64-wide
@@ -677,22 +691,24 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
       const float keep = (l < mm.L && o < out) ? 1.f : 0.f;
       const int off = mm.sd[lc] + oc;
       bm[l][c] = mu[mm.nb0[lc] + oc];
-      bkb[l][c] = bke[l][c] = 0.f;
+      float kb = 0.f, ke = 0.f;
       if (MODE == 0) {
 #pragma unroll
         for (int j = 0; j < 7; ++j) {
           const int bj = j == 0 ? p0 : (j == 6 ? 7 - p0 : j);
           const float d = sbs[bj * mm.sv + off];
-          bkb[l][c] += (j < 6 ? cA[6][j] : 0.f) * d;
-          bke[l][c] += cBT[j] * d;
+          kb += (j < 6 ? cA[6][j] : 0.f) * d;
+          ke += cBT[j] * d;
         }
+        kb *= sb;
+        ke *= se;
       } else if (MODE == 1) {
-        bkb[l][c] = sbs[p0 * mm.sv + off];
+        kb = sb * sbs[p0 * mm.sv + off];
       } else {
-        bke[l][c] = sbs[(7 - p0) * mm.sv + off] - sbs[p0 * mm.sv + off];
+        ke = se * (sbs[(7 - p0) * mm.sv + off] - sbs[p0 * mm.sv + off]);
       }
-      bkb[l][c] *= keep;
-      bke[l][c] *= keep;
+      bkb[l][c] = keep * kb;
+      bke[l][c] = keep * ke;
     }
 #pragma unroll
   for (int l = 0; l < RG_MAXL; ++l)
@@ -701,74 +717,83 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
       const int o = 64 * c + lane;
       if (l < mm.L && o < mm.out[l]) {
         float mn = 0.f;
-        const float ev = rg_mu_val<MODE>(bkb[l][c], bke[l][c], dtf, atf, rtf, bm[l][c], mn, fl);
+        const float ev = rg_mu_err<MODE>(bkb[l][c], bke[l][c], atf, rtf, bm[l][c], mn, fl);
         if (MODE == 0) mu_new[mm.nb0[l] + o] = mn;
         acc += ev * ev;
       }
     }
 
-  // ---- weight tiles, layer by layer: the stage operands of every block
-  // (delta_j[16 ot + r], h_j[16 it + r] of the lane's two stages) in one
-  // batch, then the layer's tiles T = it * to + ot in batches of RG_NB, the
-  // next batch's mu vectors loaded before the current batch is formed
-  for (int l = 0; l < mm.L; ++l) {
-    const int in = mm.in[l], out = mm.out[l], nti = mm.ti[l], nto = mm.to[l], ntl = nti * nto;
-    // (unconditional loads of clamped indices, unwanted values zeroed by a
-    // multiply -- a select here compiles to branches, see the bias part;
-    // every clamped index reads finite stage data)
-    float dl[RG_HT][2], hl[RG_HT][2];
+  // ---- weight tiles: the rows of RG_HT tiles in memory order, all layers.
+  // One row buffer, refilled tile by tile: as soon as a tile's mu vector has
+  // been used, its register is loaded with the tile RG_HT further on (the next
+  // row, across layer ends; clamped at the last tile), so a row of loads
+  // stays in flight with no copies or branches (the pass runs near its
+  // register limit, and a load under a branch is waited for where it joins).
+  const gp<const f32x4> wsrc = (gp<const f32x4>)mu + lane;
+  const gp<f32x4> wdst = (gp<f32x4>)mu_new + lane;
+  const int tlast = mm.ntile - 1;
+  f32x4 buf[RG_HT];
 #pragma unroll
-    for (int t = 0; t < RG_HT; ++t)
+  for (int k = 0; k < RG_HT; ++k) buf[k] = wsrc[k * 64];
+  for (int l = 0; l < mm.L; ++l) {
+    const int in = mm.in[l], out = mm.out[l];
+    const bool ro = mm.rowout[l];
+    // The lane's operands of every tile index t: h_j[16 t + r] and the
+    // scaled c_j delta_j[16 t + r] of its two stages, zero beyond the layer's
+    // inputs / outputs (unconditional loads of clamped indices, unwanted
+    // values zeroed by a multiply -- a select here compiles to branches, see
+    // the bias part; every clamped index reads finite stage data).  They are
+    // sorted into the operands of the row index (tB, tE) and of the column
+    // index (cB, cE) of the update and error sums; rows along the inputs
+    // (ro = 0) store their tiles transposed, so that both orders run the same
+    // code: the update sum is MFMA(row operand, column operand).
+    float tB[RG_HT][2], tE[RG_HT][2], cB[RG_HT][2], cE[RG_HT][2];
+#pragma unroll
+    for (int t = 0; t < RG_HT; ++t) {
+      const int i = 16 * t + r;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        const float wq = use[q] ? 1.f : 0.f;
-        dl[t][q] = (t < nto ? wq : 0.f) * sbs[boff[q] + mm.sd[l] + 16 * (t < nto ? t : nto - 1) + r];
-        hl[t][q] = (t < nti ? wq : 0.f) * sbs[boff[q] + mm.sh[l] + 16 * (t < nti ? t : nti - 1) + r];
+        const float hv = sbs[boff[q] + mm.sh[l] + (i < in ? i : in - 1)];
+        const float dv = sbs[boff[q] + mm.sd[l] + (i < out ? i : out - 1)];
+        const float h = (i < in ? 1.f : 0.f) * hv;
+        const float dm = (i < out ? 1.f : 0.f) * dv;
+        const float bq = cb[q] * dm, eq = ce[q] * dm;
+        tB[t][q] = ro ? h : bq;
+        tE[t][q] = ro ? h : eq;
+        cB[t][q] = ro ? bq : h;
+        cE[t][q] = ro ? eq : h;
       }
-    const gp<const f32x4> mrow = (gp<const f32x4>)(mu + mm.nw0[l]) + lane;
-    const gp<f32x4> nrow = (gp<f32x4>)(mu_new + mm.nw0[l]) + lane;
-    f32x4 cur[RG_NB], nxt[RG_NB];
+    }
+    const int T0 = (int)(mm.nw0[l] >> 8), nr = mm.nrow[l];
+    for (int a = 0; a < nr; ++a) {
+      float rb0 = 0.f, rb1 = 0.f, re0 = 0.f, re1 = 0.f;
 #pragma unroll
-    for (int k = 0; k < RG_NB; ++k) cur[k] = mrow[(k < ntl ? k : ntl - 1) * 64];
-    for (int T0 = 0; T0 < ntl; T0 += RG_NB) {
+      for (int t = 0; t < RG_HT; ++t)
+        if (t == a) { rb0 = tB[t][0]; rb1 = tB[t][1]; re0 = tE[t][0]; re1 = tE[t][1]; }
 #pragma unroll
-      for (int k = 0; k < RG_NB; ++k) {
-        const int T = T0 + RG_NB + k;
-        nxt[k] = mrow[(T < ntl ? T : ntl - 1) * 64];
-      }
-#pragma unroll
-      for (int k = 0; k < RG_NB; ++k) {
-        const int T = T0 + k;
-        if (T < ntl) {
-          const int it = T / nto, ot = T - it * nto;
-          // this block's operands out of the per-layer registers (uniform index)
-          float h0 = 0.f, h1 = 0.f, d0 = 0.f, d1 = 0.f;
-#pragma unroll
-          for (int t = 0; t < RG_HT; ++t) {
-            if (t == it) { h0 = hl[t][0]; h1 = hl[t][1]; }
-            if (t == ot) { d0 = dl[t][0]; d1 = dl[t][1]; }
-          }
-          const int o = 16 * ot + r;
-          f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb, mn = kb;
-          kb = rg_mfma(h0, cb[0] * d0, kb);
-          kb = rg_mfma(h1, cb[1] * d1, kb);
-          ke = rg_mfma(h0, ce[0] * d0, ke);
-          ke = rg_mfma(h1, ce[1] * d1, ke);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = 16 * it + 4 * g + v;
-            if (o < out && i < in) {
-              float mvv = 0.f;
-              const float ev = rg_mu_val<MODE>(kb[v], ke[v], dtf, atf, rtf, cur[k][v], mvv, fl);
-              mn[v] = mvv;
-              acc += ev * ev;
-            }
-          }
-          if (MODE == 0) nrow[T * 64] = mn;
+      for (int k = 0; k < RG_HT; ++k) {
+        // tile T: stage sums, entries, store; then the refill
+        const int T = T0 + a * RG_HT + k;
+        f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb, mn = kb;
+        if (MODE != 2) {
+          kb = rg_mfma(rb0, cB[k][0], kb);
+          kb = rg_mfma(rb1, cB[k][1], kb);
         }
-      }
+        if (MODE != 1) {
+          ke = rg_mfma(re0, cE[k][0], ke);
+          ke = rg_mfma(re1, cE[k][1], ke);
+        }
 #pragma unroll
-      for (int k = 0; k < RG_NB; ++k) cur[k] = nxt[k];
+        for (int v = 0; v < 4; ++v) {
+          float mv = 0.f;
+          const float ev = rg_mu_err<MODE>(kb[v], ke[v], atf, rtf, buf[k][v], mv, fl);
+          mn[v] = mv;
+          acc = fmaf(ev, ev, acc);
+        }
+        if (MODE == 0) wdst[T * 64] = mn;
+        const int Tn = T + RG_HT;
+        buf[k] = wsrc[(Tn < tlast ? Tn : tlast) * 64];
+      }
     }
   }
 
@@ -782,7 +807,7 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
   return sum;
 }
 
-// A finished trajectory's row from the tile-major order to the flat
+// A finished trajectory's row from the tile-row order to the flat
 // parameter order (its first P floats); `tmp` holds the tile-major row.
 __device__ void rg_row_to_flat(const RaggedModel& m, gp<const float> __restrict__ tmp,
                                gp<float> __restrict__ row, int lane) {
@@ -790,13 +815,15 @@ __device__ void rg_row_to_flat(const RaggedModel& m, gp<const float> __restrict_
   for (int l = 0; l < m.L; ++l) {
     const int in = m.in[l], out = m.out[l], nti = m.ti[l], nto = m.to[l];
     const gp<const f32x4> src = (gp<const f32x4>)(tmp + m.nw0[l]) + lane;
-    for (int t = 0; t < nti * nto; ++t) {
-      const int it = t / nto, ot = t - it * nto;
+    for (int t = 0; t < m.nrow[l] * RG_HT; ++t) {
+      const int a = t / RG_HT, k = t - a * RG_HT;
+      const int it = m.rowout[l] ? a : k, ot = m.rowout[l] ? k : a;
+      if (it >= nti || ot >= nto) continue;   // padding tile
       const f32x4 v4 = src[t * 64];
-      const int o = 16 * ot + r;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 16 * it + 4 * g + v;
+      for (int v = 0; v < 4; ++v) {   // (tiles of rows along the inputs are transposed)
+        const int o = 16 * ot + (m.rowout[l] ? r : 4 * g + v);
+        const int i = 16 * it + (m.rowout[l] ? 4 * g + v : r);
         if (o < out && i < in) row[m.w0[l] + (int64_t)i * out + o] = v4[v];
       }
     }

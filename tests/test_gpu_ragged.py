@@ -107,16 +107,22 @@ def test_ragged_rejects_unsupported_and_invalid():
         h2.ragged_forward(np.zeros((1, 16)), [1.0], [[0.5, 0.2]])
 
 
-@pytest.mark.parametrize("case", ["config4", "spiral"])
+@pytest.mark.parametrize("case", ["config4", "spiral", "padded_rows"])
 def test_ragged_adjoint_matches_per_trajectory_oracle(case):
     """dL/du0 and each trajectory's dL/dp against the oracle's
     InterpolatingAdjoint of that trajectory alone; dL/dp summed over
-    trajectories in FP64."""
+    trajectories in FP64.  The cases cover the parameter rows' tile orders:
+    rows along the outputs (16-128-128-16 first layers), transposed rows
+    along the inputs (its last layer), padding tiles in both (2-50-2), and
+    partial tiles whose padding features carry nonzero activations
+    (3-40-40-3 with sigmoid: sigmoid(0) = 1/2 sits in the padding rows)."""
     from diffeqflux_hip import Handle
     if case == "config4":
         widths, acts, imap, n, tol = [16, 128, 128, 16], ["tanh", "tanh", "identity"], 0, 10, 1e-5
-    else:
+    elif case == "spiral":
         widths, acts, imap, n, tol = [2, 50, 2], ["tanh", "identity"], MAP_CUBE, 8, 1e-6
+    else:
+        widths, acts, imap, n, tol = [3, 40, 40, 3], ["sigmoid", "tanh", "identity"], 0, 6, 1e-6
     m, p32, u0, t1, saves = ragged_case(widths, acts, imap, n, seed=7, nsave=(2, 9))
     h = Handle(widths, acts, imap, "fp32", 1)
     h.set_params(p32)
