@@ -475,6 +475,28 @@ void launch_ragged_fwd(const RaggedFwdArgs& a, int grid, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------- adjoint
+// y = act(z) and act'(z) of a 4-entry vector, the activation chosen once per
+// vector (a per-entry switch puts a chain of branches around every entry)
+__device__ __forceinline__ void rg_act4(int act, const f32x4& z, f32x4& y, f32x4& sd) {
+#define RG_ACT_CASE(A)                                                   \
+  case A:                                                                \
+    _Pragma("unroll") for (int v = 0; v < 4; ++v) {                      \
+      y[v] = act_f(A, z[v]);                                             \
+      sd[v] = act_d(A, z[v], y[v]);                                      \
+    }                                                                    \
+    break;
+  switch (act) {
+    RG_ACT_CASE(1)
+    RG_ACT_CASE(2)
+    RG_ACT_CASE(3)
+    RG_ACT_CASE(4)
+    default:
+      y = z;
+      sd = f32x4{1.f, 1.f, 1.f, 1.f};
+  }
+#undef RG_ACT_CASE
+}
+
 // One VJP for the 16 slot columns: x0 = map(u) (first ti[0] tiles), lam (first
 // to[L-1] tiles).  Stores the layer inputs h_l and the pre-activation
 // gradients delta_l of this evaluation in the slot's stage block `blk`
@@ -506,22 +528,22 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
       if (ot < nto) {
         f32x4 acc = *(const f32x4*)(bias + 16 * ot + g4);
         const float* wrow = W + (16 * ot + r) * pitch + g4;
+        // the tile's weight vectors read in one batch (clamped indices, so
+        // the reads are unconditional): one LDS wait per tile, not per k-tile
+        f32x4 a[RG_HT];
+#pragma unroll
+        for (int it = 0; it < RG_HT; ++it) a[it] = *(const f32x4*)(wrow + 16 * (it < nti ? it : nti - 1));
 #pragma unroll
         for (int it = 0; it < RG_HT; ++it) {
           if (it < nti) {
-            const f32x4 a = *(const f32x4*)(wrow + 16 * it);
-            acc = rg_mfma(a.x, hin[it].x, acc);
-            acc = rg_mfma(a.y, hin[it].y, acc);
-            acc = rg_mfma(a.z, hin[it].z, acc);
-            acc = rg_mfma(a.w, hin[it].w, acc);
+            acc = rg_mfma(a[it].x, hin[it].x, acc);
+            acc = rg_mfma(a[it].y, hin[it].y, acc);
+            acc = rg_mfma(a[it].z, hin[it].z, acc);
+            acc = rg_mfma(a[it].w, hin[it].w, acc);
           }
         }
         f32x4 y, sd;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          y[v] = act_f(act, acc[v]);
-          sd[v] = act_d(act, acc[v], y[v]);
-        }
+        rg_act4(act, acc, y, sd);
         if (m.act_stride) *(f32x4*)(act_lds + (m.act_off[l] + ot) * 256 + 4 * lane) = sd;
         else *(gp<f32x4>)(blk + m.sd[l] + 16 * ot + g4) = sd;
         if (l < L - 1) *(gp<f32x4>)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
@@ -554,14 +576,21 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
     for (int it = 0; it < RG_HT; ++it) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       if (it < nti) {
+        // the tile's W^T entries read in one batch (clamped indices)
+        float w[RG_HT][4];
+#pragma unroll
+        for (int ot = 0; ot < RG_HT; ++ot) {
+          const float* wc = W + (16 * (ot < nto ? ot : nto - 1) + g4) * pitch + 16 * it + r;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) w[ot][v] = wc[v * pitch];
+        }
 #pragma unroll
         for (int ot = 0; ot < RG_HT; ++ot) {
           if (ot < nto) {
-            const float* wc = W + (16 * ot + g4) * pitch + 16 * it + r;
-            acc = rg_mfma(wc[0], gi[ot].x, acc);
-            acc = rg_mfma(wc[pitch], gi[ot].y, acc);
-            acc = rg_mfma(wc[2 * pitch], gi[ot].z, acc);
-            acc = rg_mfma(wc[3 * pitch], gi[ot].w, acc);
+            acc = rg_mfma(w[ot][0], gi[ot].x, acc);
+            acc = rg_mfma(w[ot][1], gi[ot].y, acc);
+            acc = rg_mfma(w[ot][2], gi[ot].z, acc);
+            acc = rg_mfma(w[ot][3], gi[ot].w, acc);
           }
         }
       }
