@@ -208,6 +208,7 @@ struct node_handle {
     float* tmp = nullptr; int64_t tmp_cap = 0;
     float* sbuf = nullptr; int64_t sbuf_cap = 0;
     double* dp = nullptr; int64_t dp_cap = 0;
+    int* order = nullptr; int64_t order_cap = 0;
   } ra;
 
   // profiling (node_prof_enable)
@@ -1613,7 +1614,35 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   m.act_stride = act_tiles * 256;
   if (((int64_t)m.lds_floats + (int64_t)waves * m.act_stride) * (int64_t)sizeof(float) > 160 * 1024)
     m.act_stride = 0;
-  if (grow(h, &b.dLdout, std::max<int64_t>(ns * D, 1), b.dLdout_cap, std::max<int64_t>(ns * D, 64)) ||
+  // start order (a schedule only: each trajectory's arithmetic is the same
+  // in any slot).  A wave runs until its last slot's trajectory is done and
+  // does one parameter pass per step of every slot, so the waves' sums of
+  // steps should match.  A trajectory's adjoint steps are estimated as the
+  // steps of the solve before it (the forward) plus its save times (the
+  // adjoint stops at each of them); the trajectories, costliest first, are
+  // dealt over the waves back and forth (wave w: ranks w, 2W-1-w, 2W+w, ...);
+  // any beyond the launch's slots follow from the queue in the same order.
+  const int64_t W = slots / RG_SLOTS, norder = std::max<int64_t>(n, slots);
+  std::vector<int> hts(4 * n), ord(norder, -1);
+  std::vector<int64_t> hso(n + 1), rank(n), cost(n);
+  HIPCHK(h, hipMemcpyAsync(hts.data(), r.tstats, 4 * n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(hso.data(), r.soff, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int64_t i = 0; i < n; ++i) {
+    rank[i] = i;
+    cost[i] = (int64_t)hts[4 * i + 1] + hts[4 * i + 2] + (hso[i + 1] - hso[i]);
+  }
+  std::stable_sort(rank.begin(), rank.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t pos = k;
+    if (k < slots) {
+      const int64_t lap = k / W, w = (lap & 1) ? W - 1 - k % W : k % W;
+      pos = w * RG_SLOTS + lap;
+    }
+    ord[pos] = (int)rank[k];
+  }
+  if (grow(h, &b.order, norder, b.order_cap, norder) ||
+      grow(h, &b.dLdout, std::max<int64_t>(ns * D, 1), b.dLdout_cap, std::max<int64_t>(ns * D, 64)) ||
       grow(h, &b.dLdu0, n * D, b.dLdu0_cap, n * D) ||
       grow(h, &b.rows, n * Pn, b.rows_cap, n * Pn) ||
       grow(h, &b.tmp, slots * Pn, b.tmp_cap, slots * Pn) ||
@@ -1622,6 +1651,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
     return NODE_ERR_HIP;
   if (ns > 0)
     HIPCHK(h, hipMemcpyAsync(b.dLdout, dLdout, ns * D * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(b.order, ord.data(), norder * sizeof(int), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemsetAsync(b.rows, 0, (size_t)n * Pn * sizeof(float), h->stream));
   HIPCHK(h, hipMemsetAsync(r.ctr, 0, 4 * sizeof(int), h->stream));
   HIPCHK(h, hipMemsetAsync(r.stats, 0, 4 * sizeof(unsigned long long), h->stream));
@@ -1634,6 +1664,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   a.last_rec = r.last; a.rec_t = r.rec_t; a.rec_prev = r.rec_prev; a.rec_data = r.rec_data;
   a.dLdu0 = b.dLdu0; a.dLdp_traj = b.rows; a.mu_tmp = b.tmp; a.sbuf = b.sbuf;
   a.queue = r.ctr; a.retcode = r.ret; a.traj_stats = r.tstats;
+  a.order = b.order; a.norder = norder; a.nslots = slots;
   const node_solve_opts* o = &r.opts;
   a.t0 = o->t0;
   a.abstol = o->abstol > 0 ? o->abstol : 1e-6;

@@ -376,6 +376,11 @@ struct RaggedAdjArgs {
   float* mu_tmp;                      // [slots][Pn]
   float* sbuf;                        // [slots][8][sv] stage blocks
   int* queue;
+  // start order: the trajectory at each queue position (-1: none).  Slot s of
+  // the launch starts at position s; later starts take positions nslots,
+  // nslots + 1, ... from the queue counter
+  const int* order;                   // [norder]
+  int64_t norder, nslots;
   int* retcode;                       // [n]
   int* traj_stats;                    // [n][4]
   double t0, abstol, reltol, dtmax, dtmin, dt_init;

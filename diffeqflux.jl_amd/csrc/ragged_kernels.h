@@ -910,7 +910,7 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
   const float atol = (float)a.abstol, rtol = (float)a.reltol;
 
   int traj = -1, phase = 0, stage = 0, p0 = 0, pad = 0, fsal_re = 0, muflip = 0, cj = 0;
-  bool need_start = true;
+  bool need_start = true, first_start = true;
   SlotCtl c{};
   f32x4 lam[DT], lnew[DT], kl[7][DT], uc[DT];
 #pragma unroll
@@ -925,10 +925,15 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
   for (;;) {
     if (need_start) {
       need_start = false;
-      int next = 0;
-      if (g == 0) next = atomicAdd(a.queue, 1);
-      next = rg_bcast(next, lane);
-      traj = next < a.n ? next : -1;
+      // the slot's own position first, then positions from the queue
+      int64_t pos = wslot0 + r;
+      if (!first_start) {
+        int q = 0;
+        if (g == 0) q = atomicAdd(a.queue, 1);
+        pos = a.nslots + rg_bcast(q, lane);
+      }
+      first_start = false;
+      traj = pos < a.norder ? a.order[pos] : -1;
       if (traj >= 0) {
         c = SlotCtl{};
         c.t = a.t1[traj];
