@@ -1599,12 +1599,19 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   if (ragged_model(h, m, why)) return fail(h, NODE_ERR_UNSUPPORTED, why);
   const int64_t n = r.n, ns = r.ns, P = h->P, Pn = m.Pn;
   const int D = h->D, Dp = (D + 15) / 16 * 16;
-  // every CU gets a workgroup (the weights fill its LDS: one per CU) with
-  // just enough 16-slot waves for all trajectories to start at once
-  const int64_t waves_all = (n + RG_SLOTS - 1) / RG_SLOTS;
+  // A workgroup per CU (the weights fill its LDS) of up to 4 waves, one per
+  // SIMD.  A wave's time is its VJP rounds (as many as its longest slot
+  // needs) plus one parameter pass per step of every slot, each pass bound
+  // by the wave's own memory latency: so the trajectories are spread over as
+  // many waves as the SIMDs hold (spw slots of 16 active per wave), and all
+  // of them start at once when they fit.
+  const int64_t max_waves = 4 * (int64_t)h->n_cu;
+  const int spw = (int)std::min<int64_t>(RG_SLOTS, std::max<int64_t>(1, (n + max_waves - 1) / max_waves));
+  const int64_t waves_all = (n + spw - 1) / spw;
   const int waves = (int)std::min<int64_t>(4, std::max<int64_t>(1, (waves_all + h->n_cu - 1) / h->n_cu));
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(h->n_cu, (waves_all + waves - 1) / waves));
-  const int64_t slots = (int64_t)grid * waves * RG_SLOTS;
+  const int64_t slots = (int64_t)grid * waves * RG_SLOTS;      // stage blocks, row buffers
+  const int64_t nslots = (int64_t)grid * waves * spw;          // active slots
   // act' of the VJPs in LDS beside the weights when every wave's area fits
   int act_tiles = 0;
   for (int l = 0; l < m.L; ++l) {
@@ -1622,7 +1629,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   // adjoint stops at each of them); the trajectories, costliest first, are
   // dealt over the waves back and forth (wave w: ranks w, 2W-1-w, 2W+w, ...);
   // any beyond the launch's slots follow from the queue in the same order.
-  const int64_t W = slots / RG_SLOTS, norder = std::max<int64_t>(n, slots);
+  const int64_t W = (int64_t)grid * waves, norder = std::max<int64_t>(n, nslots);
   std::vector<int> hts(4 * n), ord(norder, -1);
   std::vector<int64_t> hso(n + 1), rank(n), cost(n);
   HIPCHK(h, hipMemcpyAsync(hts.data(), r.tstats, 4 * n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -1635,9 +1642,9 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   std::stable_sort(rank.begin(), rank.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
   for (int64_t k = 0; k < n; ++k) {
     int64_t pos = k;
-    if (k < slots) {
+    if (k < nslots) {
       const int64_t lap = k / W, w = (lap & 1) ? W - 1 - k % W : k % W;
-      pos = w * RG_SLOTS + lap;
+      pos = w * spw + lap;
     }
     ord[pos] = (int)rank[k];
   }
@@ -1664,7 +1671,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   a.last_rec = r.last; a.rec_t = r.rec_t; a.rec_prev = r.rec_prev; a.rec_data = r.rec_data;
   a.dLdu0 = b.dLdu0; a.dLdp_traj = b.rows; a.mu_tmp = b.tmp; a.sbuf = b.sbuf;
   a.queue = r.ctr; a.retcode = r.ret; a.traj_stats = r.tstats;
-  a.order = b.order; a.norder = norder; a.nslots = slots;
+  a.order = b.order; a.norder = norder; a.nslots = nslots; a.spw = spw;
   const node_solve_opts* o = &r.opts;
   a.t0 = o->t0;
   a.abstol = o->abstol > 0 ? o->abstol : 1e-6;

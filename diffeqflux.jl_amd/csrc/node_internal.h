@@ -376,11 +376,14 @@ struct RaggedAdjArgs {
   float* mu_tmp;                      // [slots][Pn]
   float* sbuf;                        // [slots][8][sv] stage blocks
   int* queue;
-  // start order: the trajectory at each queue position (-1: none).  Slot s of
-  // the launch starts at position s; later starts take positions nslots,
-  // nslots + 1, ... from the queue counter
+  // start order: the trajectory at each queue position (-1: none).  A wave
+  // runs trajectories in its first spw slots only (spw <= 16: more waves, each
+  // with fewer parameter passes, when there are SIMDs to spare); slot s < spw
+  // of wave w starts at position w * spw + s, later starts take positions
+  // nslots, nslots + 1, ... from the queue counter
   const int* order;                   // [norder]
   int64_t norder, nslots;
+  int spw;
   int* retcode;                       // [n]
   int* traj_stats;                    // [n][4]
   double t0, abstol, reltol, dtmax, dtmin, dt_init;

@@ -925,8 +925,9 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
   for (;;) {
     if (need_start) {
       need_start = false;
-      // the slot's own position first, then positions from the queue
-      int64_t pos = wslot0 + r;
+      // the slot's own position first (slots >= spw stay idle), then
+      // positions from the queue
+      int64_t pos = r < a.spw ? wslot0 / RG_SLOTS * a.spw + r : a.norder;
       if (!first_start) {
         int q = 0;
         if (g == 0) q = atomicAdd(a.queue, 1);
