@@ -1421,6 +1421,10 @@ int ragged_model(node_handle* h, RaggedModel& m, std::string& why) {
     m.w0[l] = h->layers[l].w0;
     m.b0[l] = h->layers[l].b0;
   }
+  // 128 zero floats after the last layer: a sweep reads whole 8-tile rows
+  // (k-tiles past a layer's own are read but not used), which may run up to
+  // 128 floats past the last layer's weights
+  off += 128;
   m.lds_floats = off;
   if ((size_t)off * sizeof(float) > 160 * 1024) { why = "model weights exceed the 160 KB LDS of a workgroup"; return -1; }
   int so = 0;

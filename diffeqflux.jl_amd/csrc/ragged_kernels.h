@@ -528,11 +528,12 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
       if (ot < nto) {
         f32x4 acc = *(const f32x4*)(bias + 16 * ot + g4);
         const float* wrow = W + (16 * ot + r) * pitch + g4;
-        // the tile's weight vectors read in one batch (clamped indices, so
-        // the reads are unconditional): one LDS wait per tile, not per k-tile
+        // the tile's weight vectors read in one batch at constant offsets:
+        // one LDS wait per tile, not per k-tile (k-tiles past nti read the
+        // next rows or the zero pad after the weights, and go unused)
         f32x4 a[RG_HT];
 #pragma unroll
-        for (int it = 0; it < RG_HT; ++it) a[it] = *(const f32x4*)(wrow + 16 * (it < nti ? it : nti - 1));
+        for (int it = 0; it < RG_HT; ++it) a[it] = *(const f32x4*)(wrow + 16 * it);
 #pragma unroll
         for (int it = 0; it < RG_HT; ++it) {
           if (it < nti) {
@@ -572,29 +573,30 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
       }
     }
     const float* W = lds + m.lw[l];
+    // output tile by output tile, every input tile's sum at once: the four
+    // W^T row bases of an output tile serve all input tiles at constant
+    // offsets (k-tiles past nti read the next rows or the zero pad, unused)
 #pragma unroll
-    for (int it = 0; it < RG_HT; ++it) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      if (it < nti) {
-        // the tile's W^T entries read in one batch (clamped indices)
+    for (int it = 0; it < RG_HT; ++it) go[it] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ot = 0; ot < RG_HT; ++ot) {
+      if (ot < nto) {
+        const float* wc = W + (16 * ot + g4) * pitch + r;
         float w[RG_HT][4];
 #pragma unroll
-        for (int ot = 0; ot < RG_HT; ++ot) {
-          const float* wc = W + (16 * (ot < nto ? ot : nto - 1) + g4) * pitch + 16 * it + r;
+        for (int v = 0; v < 4; ++v)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) w[ot][v] = wc[v * pitch];
-        }
+          for (int it = 0; it < RG_HT; ++it) w[it][v] = wc[v * pitch + 16 * it];
 #pragma unroll
-        for (int ot = 0; ot < RG_HT; ++ot) {
-          if (ot < nto) {
-            acc = rg_mfma(w[ot][0], gi[ot].x, acc);
-            acc = rg_mfma(w[ot][1], gi[ot].y, acc);
-            acc = rg_mfma(w[ot][2], gi[ot].z, acc);
-            acc = rg_mfma(w[ot][3], gi[ot].w, acc);
+        for (int it = 0; it < RG_HT; ++it) {
+          if (it < nti) {
+            go[it] = rg_mfma(w[it][0], gi[ot].x, go[it]);
+            go[it] = rg_mfma(w[it][1], gi[ot].y, go[it]);
+            go[it] = rg_mfma(w[it][2], gi[ot].z, go[it]);
+            go[it] = rg_mfma(w[it][3], gi[ot].w, go[it]);
           }
         }
       }
-      go[it] = acc;
     }
 #pragma unroll
     for (int t = 0; t < RG_HT; ++t) gi[t] = go[t];
