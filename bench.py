@@ -360,6 +360,15 @@ def ragged_leg(n=10000, reps=3, cpu_sample=8):
         rec = (t2 - t0 - (t1c - t1f), t1f - t0, t2 - t1c, tot, atot)
         if best is None or rec[0] < best[0]:
             best = rec
+    # the BF16-operand VJP option on the same forward and cotangents
+    # (best of two: the first also loads that kernel)
+    h.set_option("ragged_vjp_precision", "bf16")
+    tb = float("inf")
+    for _ in range(2):
+        t0 = time.perf_counter()
+        du0b, dpb, _, aretb, _, atotb = h.ragged_backward(cots)
+        tb = min(tb, time.perf_counter() - t0)
+    h.set_option("ragged_vjp_precision", "fp32")
     wall, tf, ta, tot, atot = best
     evals = tot["nf"] + atot["nf"]
     res = {"workload": "10,000 trajectories, D=16, MLP 16-128-128-16 tanh, t1~U(0.1,5), "
@@ -370,7 +379,12 @@ def ragged_leg(n=10000, reps=3, cpu_sample=8):
            "fwd_steps": tot["naccept"], "fwd_rejects": tot["nreject"],
            "adj_steps": atot["naccept"], "adj_rejects": atot["nreject"],
            "all_success": bool(np.all(ret == 1) and np.all(aret == 1)),
-           "timing": "host entry points incl. PCIe copies of inputs/outputs"}
+           "timing": "host entry points incl. PCIe copies of inputs/outputs",
+           "bf16_vjp_option": {
+               "adj_ms": round(1e3 * tb, 2), "adj_steps": atotb["naccept"],
+               "adj_rejects": atotb["nreject"], "all_success": bool(np.all(aretb == 1)),
+               "dLdp_rel_err_vs_fp32": float(np.abs(dpb - dp).max() / np.abs(dp).max()),
+               "dLdu0_rel_err_vs_fp32": float(np.abs(du0b - du0).max() / np.abs(du0).max())}}
     # CPU baseline: the oracle (FP32) on the first cpu_sample trajectories
     from oracle import adjoint as OA
     tc = time.perf_counter()

@@ -133,6 +133,7 @@ struct node_handle {
   bool ckpt = false;          // NODE_OPT_CHECKPOINTING
   bool fwd_ring = false;      // the last forward kept checkpoints only
   int64_t tape_budget = 0;    // NODE_OPT_TAPE_BUDGET: bytes of tape (0: unlimited)
+  int rg_vjp_prec = NODE_PREC_FP32;  // NODE_OPT_RAGGED_VJP_PRECISION
   float* ck_u0 = nullptr;     // u(t0), FP32 [B][D]
   float* ck_uT = nullptr;     // u(t1), FP32 [B][D]
   Ctl* ctl_save = nullptr;    // the paused adjoint's control block
@@ -1676,6 +1677,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   a.dLdu0 = b.dLdu0; a.dLdp_traj = b.rows; a.mu_tmp = b.tmp; a.sbuf = b.sbuf;
   a.queue = r.ctr; a.retcode = r.ret; a.traj_stats = r.tstats;
   a.order = b.order; a.norder = norder; a.nslots = nslots; a.spw = spw;
+  a.vjp_bf16 = h->rg_vjp_prec == NODE_PREC_BF16;
   const node_solve_opts* o = &r.opts;
   a.t0 = o->t0;
   a.abstol = o->abstol > 0 ? o->abstol : 1e-6;
@@ -2325,6 +2327,11 @@ int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value) {
     if (value < 0) return fail(h, NODE_ERR_INVALID, "tape budget must be >= 0 bytes");
     h->tape_budget = value;
     return 0;
+  } else if (key == NODE_OPT_RAGGED_VJP_PRECISION) {
+    if (value != NODE_PREC_FP32 && value != NODE_PREC_BF16)
+      return fail(h, NODE_ERR_INVALID, "per-trajectory VJP precision must be NODE_PREC_FP32 or NODE_PREC_BF16");
+    h->rg_vjp_prec = (int)value;
+    return 0;  // read by the next node_ragged_backward
   } else {
     return fail(h, NODE_ERR_INVALID, "unknown option");
   }

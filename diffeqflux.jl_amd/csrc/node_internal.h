@@ -384,6 +384,7 @@ struct RaggedAdjArgs {
   const int* order;                   // [norder]
   int64_t norder, nslots;
   int spw;
+  int vjp_bf16;                       // VJP products on BF16 operands (NODE_OPT_RAGGED_VJP_PRECISION)
   int* retcode;                       // [n]
   int* traj_stats;                    // [n][4]
   double t0, abstol, reltol, dtmax, dtmin, dt_init;

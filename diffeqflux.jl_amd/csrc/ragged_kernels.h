@@ -42,6 +42,19 @@ __device__ __forceinline__ f32x4 rg_mfma(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// BF16 operands for v_mfma_f32_16x16x16_bf16: a lane's 4 values of a 16-wide
+// k-tile (k = 4 (l / 16) + v) are that MFMA's k-slice for the lane, the same
+// layout as the four FP32 steps above, so one instruction replaces four
+typedef short rg_s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 rg_b4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ rg_s4 rg_bf4(const f32x4& v) {
+  const rg_b4 b = {(__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+  return __builtin_bit_cast(rg_s4, b);
+}
+__device__ __forceinline__ f32x4 rg_mfma_bf(rg_s4 a, rg_s4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
 // z[ot] = W x for output tiles ot < nto (no bias); x: nti input tiles
 template <int NTI, int NTO>
 __device__ __forceinline__ void rg_fwd_gemm(const float* W, int pitch, const f32x4 (&x)[NTI],
@@ -503,6 +516,8 @@ __device__ __forceinline__ void rg_act4(int act, const f32x4& z, f32x4& y, f32x4
 // (act' is parked in the wave's LDS area `act_lds` during the forward sweep,
 // or in the block's delta area when m.act_stride is 0) and returns
 // g = W_0^T delta_0 (the input-map derivative is applied by the caller).
+// BF: the products on BF16 operands (rounded to nearest), FP32 accumulation.
+template <bool BF>
 __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, float* act_lds,
                                        const f32x4 (&x0)[RG_HT], const f32x4 (&lam)[RG_HT],
                                        gp<float> blk, f32x4 (&gout)[RG_HT], int lane) {
@@ -522,6 +537,11 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
     const float* W = lds + m.lw[l];
     const float* bias = lds + m.lb[l];
     const int pitch = m.pitch[l], nti = m.ti[l], nto = m.to[l], act = m.act[l];
+    rg_s4 hb[RG_HT];
+    if (BF) {
+#pragma unroll
+      for (int t = 0; t < RG_HT; ++t) hb[t] = rg_bf4(hin[t]);
+    }
 #pragma unroll
     for (int ot = 0; ot < RG_HT; ++ot) {
       hout[ot] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -537,10 +557,14 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
 #pragma unroll
         for (int it = 0; it < RG_HT; ++it) {
           if (it < nti) {
-            acc = rg_mfma(a[it].x, hin[it].x, acc);
-            acc = rg_mfma(a[it].y, hin[it].y, acc);
-            acc = rg_mfma(a[it].z, hin[it].z, acc);
-            acc = rg_mfma(a[it].w, hin[it].w, acc);
+            if (BF) {
+              acc = rg_mfma_bf(rg_bf4(a[it]), hb[it], acc);
+            } else {
+              acc = rg_mfma(a[it].x, hin[it].x, acc);
+              acc = rg_mfma(a[it].y, hin[it].y, acc);
+              acc = rg_mfma(a[it].z, hin[it].z, acc);
+              acc = rg_mfma(a[it].w, hin[it].w, acc);
+            }
           }
         }
         f32x4 y, sd;
@@ -587,13 +611,18 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
         for (int v = 0; v < 4; ++v)
 #pragma unroll
           for (int it = 0; it < RG_HT; ++it) w[it][v] = wc[v * pitch + 16 * it];
+        const rg_s4 gb = rg_bf4(gi[ot]);
 #pragma unroll
         for (int it = 0; it < RG_HT; ++it) {
           if (it < nti) {
-            go[it] = rg_mfma(w[it][0], gi[ot].x, go[it]);
-            go[it] = rg_mfma(w[it][1], gi[ot].y, go[it]);
-            go[it] = rg_mfma(w[it][2], gi[ot].z, go[it]);
-            go[it] = rg_mfma(w[it][3], gi[ot].w, go[it]);
+            if (BF) {
+              go[it] = rg_mfma_bf(rg_bf4(f32x4{w[it][0], w[it][1], w[it][2], w[it][3]}), gb, go[it]);
+            } else {
+              go[it] = rg_mfma(w[it][0], gi[ot].x, go[it]);
+              go[it] = rg_mfma(w[it][1], gi[ot].y, go[it]);
+              go[it] = rg_mfma(w[it][2], gi[ot].z, go[it]);
+              go[it] = rg_mfma(w[it][3], gi[ot].w, go[it]);
+            }
           }
         }
       }
@@ -896,7 +925,7 @@ __device__ __forceinline__ void rg_tape_u(const RaggedAdjArgs& a, int cj, double
   }
 }
 
-template <int DT>
+template <int DT, bool BF>
 __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
   static_assert(offsetof(RaggedAdjArgs, m) == 0, "the model leads the argument struct");
   extern __shared__ __attribute__((aligned(16))) float rg_lds[];
@@ -1013,7 +1042,7 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
         lg[t] = ls[t];
       }
     }
-    rg_vjp(m, rg_lds, rg_lds + m.lds_floats + (threadIdx.x >> 6) * m.act_stride, x0, lg,
+    rg_vjp<BF>(m, rg_lds, rg_lds + m.lds_floats + (threadIdx.x >> 6) * m.act_stride, x0, lg,
            sb + blkid * m.sv, gv, lane);
     // k = -(lambda^T df/du): input-map derivative 3u^2 for x.^3
     f32x4 kn[DT];
@@ -1234,14 +1263,16 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
 // `waves` (1..4) waves of RG_SLOTS slots per workgroup
 void launch_ragged_adj(const RaggedAdjArgs& a, int grid, int waves, hipStream_t s) {
   const size_t lds = ((size_t)a.m.lds_floats + (size_t)waves * a.m.act_stride) * sizeof(float);
+  auto go = [&](auto kern) {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), lds, s, a);
+  };
   if (a.Dp <= 16) {
-    (void)hipFuncSetAttribute((const void*)k_ragged_adj<1>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_ragged_adj<1>, dim3(grid), dim3(64 * waves), lds, s, a);
+    if (a.vjp_bf16) go(k_ragged_adj<1, true>);
+    else go(k_ragged_adj<1, false>);
   } else {
-    (void)hipFuncSetAttribute((const void*)k_ragged_adj<2>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_ragged_adj<2>, dim3(grid), dim3(64 * waves), lds, s, a);
+    if (a.vjp_bf16) go(k_ragged_adj<2, true>);
+    else go(k_ragged_adj<2, false>);
   }
 }
 

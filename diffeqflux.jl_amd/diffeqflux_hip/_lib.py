@@ -14,7 +14,7 @@ ACT = {"identity": 0, "tanh": 1, "softplus": 2, "relu": 3, "sigmoid": 4}
 MAP_IDENTITY, MAP_CUBE, MAP_CNF, MAP_CNF_TIME = 0, 1, 2, 3
 MAP_CNF_REGULARIZE, MAP_CNF_EXACT_TRACE = 0x10, 0x20     # flags on MAP_CNF / MAP_CNF_TIME
 OPT_GRAPHS, OPT_GRAPH_ATTEMPTS, OPT_AUTOTUNE, OPT_WGRAD_PRECISION, OPT_SENSEALG = 1, 2, 3, 4, 5
-OPT_CHECKPOINTING, OPT_TAPE_BUDGET = 6, 7
+OPT_CHECKPOINTING, OPT_TAPE_BUDGET, OPT_RAGGED_VJP_PRECISION = 6, 7, 8
 SENSE = {"interpolating": 0, "backsolve": 1, "backsolve_nockpt": 2}
 PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 RET_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN",

@@ -271,13 +271,16 @@ class Handle:
         'wgrad_precision' ('fp32' / 'bf16' / 'bf16x3') or 'sensealg'
         ('interpolating' / 'backsolve' / 'backsolve_nockpt'), 'checkpointing'
         (bool: InterpolatingAdjoint(checkpointing = true), the forward keeps
-        checkpoint states only and the adjoint re-solves each interval) or
-        'tape_budget' (bytes of forward tape, 0 = unlimited)."""
+        checkpoint states only and the adjoint re-solves each interval),
+        'tape_budget' (bytes of forward tape, 0 = unlimited) or
+        'ragged_vjp_precision' ('fp32' / 'bf16': the per-trajectory adjoint's
+        VJP products on BF16 operands)."""
         k = {"graphs": _lib.OPT_GRAPHS, "graph_attempts": _lib.OPT_GRAPH_ATTEMPTS,
              "autotune": _lib.OPT_AUTOTUNE, "wgrad_precision": _lib.OPT_WGRAD_PRECISION,
              "sensealg": _lib.OPT_SENSEALG, "checkpointing": _lib.OPT_CHECKPOINTING,
-             "tape_budget": _lib.OPT_TAPE_BUDGET}[key]
-        if key == "wgrad_precision" and isinstance(value, str):
+             "tape_budget": _lib.OPT_TAPE_BUDGET,
+             "ragged_vjp_precision": _lib.OPT_RAGGED_VJP_PRECISION}[key]
+        if key in ("wgrad_precision", "ragged_vjp_precision") and isinstance(value, str):
             value = _lib.PREC[value]
         if key == "sensealg" and isinstance(value, str):
             value = _lib.SENSE[value]

@@ -242,7 +242,8 @@ int32_t node_rhs_device_noalloc(node_handle_t* h, const float* d_u, float* d_du,
  *             RHS evaluations, accepted steps, rejected steps, tape records.
  * Host pointers.  Computes in FP32 (MFMA 16x16x4 FP32) whatever the handle's
  * precision: at the tolerances these solves are run with, BF16 operands push
- * the error estimate onto their rounding floor.  Model limits: D <= 32, hidden
+ * the error estimate onto their rounding floor (the adjoint's VJPs can opt in
+ * to BF16 operands: NODE_OPT_RAGGED_VJP_PRECISION).  Model limits: D <= 32, hidden
  * widths <= 128, at most 4 Dense layers. */
 typedef struct node_ragged_stats {
     int64_t nf, naccept, nreject;   /* summed over trajectories */
@@ -309,9 +310,14 @@ int32_t node_ragged_backward(node_handle_t* h, const float* dLdout, float* dLdu0
  *                            default).  A solve whose tape would exceed it fails with
  *                            NODE_ERR_CAPACITY (with checkpointing, only an interval
  *                            re-solve can need more than a few entries). */
+/*   NODE_OPT_RAGGED_VJP_PRECISION  NODE_PREC_FP32 (default) or NODE_PREC_BF16: the
+ *                            products of node_ragged_backward's VJPs on BF16 operands
+ *                            (one v_mfma_f32_16x16x16_bf16 per 16x16x16 block instead of
+ *                            four FP32 ones; FP32 accumulation).  The forward, the
+ *                            parameter-gradient sums and all state stay FP32. */
 enum { NODE_OPT_GRAPHS = 1, NODE_OPT_GRAPH_ATTEMPTS = 2, NODE_OPT_AUTOTUNE = 3,
        NODE_OPT_WGRAD_PRECISION = 4, NODE_OPT_SENSEALG = 5, NODE_OPT_CHECKPOINTING = 6,
-       NODE_OPT_TAPE_BUDGET = 7 };
+       NODE_OPT_TAPE_BUDGET = 7, NODE_OPT_RAGGED_VJP_PRECISION = 8 };
 enum { NODE_SENSE_INTERPOLATING = 0, NODE_SENSE_BACKSOLVE = 1, NODE_SENSE_BACKSOLVE_NOCKPT = 2 };
 int32_t node_set_option(node_handle_t* h, int32_t key, int64_t value);
 /* Tile plan chosen for a GEMM role (roles as for node_prof_enable below) and

@@ -153,3 +153,45 @@ def test_ragged_adjoint_matches_per_trajectory_oracle(case):
     print(f"  worst du0 {worst_u:.2e} dp {worst_p:.2e}; summed dL/dp rel err {e:.2e}", flush=True)
     assert e < 50 * tol + 1e-5
     assert np.allclose(dp, rows.astype(np.float64).sum(0), rtol=1e-6, atol=1e-9)
+
+
+def test_ragged_adjoint_bf16_vjp_option():
+    """NODE_OPT_RAGGED_VJP_PRECISION = BF16: the adjoint's VJP products on
+    BF16 operands (FP32 accumulation, everything else FP32) on the
+    16-128-128-16 case, against the per-trajectory FP64 oracle at a BF16 bar
+    (operand rounding 2^-9 relative, through three layers and the adjoint's
+    error control); the FP32 setting restored gives the FP32 results again."""
+    from diffeqflux_hip import Handle, NodeHipError
+    widths, acts, tol, n = [16, 128, 128, 16], ["tanh", "tanh", "identity"], 1e-5, 10
+    m, p32, u0, t1, saves = ragged_case(widths, acts, 0, n, seed=7, nsave=(2, 9))
+    h = Handle(widths, acts, 0, "fp32", 1)
+    h.set_params(p32)
+    outs, ret, _, _ = h.ragged_forward(u0, t1, saves, abstol=tol, reltol=tol)
+    assert np.all(ret == 1)
+    rng = np.random.default_rng(11)
+    cots = [rng.standard_normal(o.shape).astype(np.float32) for o in outs]
+    du0_32, dp_32, _, _, _, tot32 = h.ragged_backward(cots)
+    with pytest.raises(NodeHipError):
+        h.set_option("ragged_vjp_precision", "bf16x3")
+    h.set_option("ragged_vjp_precision", "bf16")
+    du0, dp, rows, aret, ats, tot = h.ragged_backward(cots, per_trajectory=True)
+    assert np.all(aret == 1), aret
+    p = p32.astype(np.float64)
+    gp_sum, gu = np.zeros_like(p), []
+    for i in range(n):
+        s = list(saves[i])
+        ref = OA.forward(m, p, u0[i:i + 1].astype(np.float64), 0.0, float(t1[i]), saveat=s,
+                         abstol=tol, reltol=tol)
+        c = [cots[i][k:k + 1].astype(np.float64) for k in range(len(s))]
+        gu0, gp, _ = OA.backward(m, p, ref, c, abstol=tol, reltol=tol)
+        gp_sum += gp
+        gu.append(gu0[0])
+    gu = np.array(gu)
+    eu = float(np.abs(du0 - gu).max() / np.abs(gu).max())
+    ep = float(np.abs(dp - gp_sum).max() / np.abs(gp_sum).max())
+    print(f"\n[ragged adjoint, BF16 VJP operands] adjoint steps {tot['naccept']}+{tot['nreject']} "
+          f"(FP32 {tot32['naccept']}+{tot32['nreject']}); rel err du0 {eu:.2e} dL/dp {ep:.2e}", flush=True)
+    assert eu < 2e-2 and ep < 2e-2
+    h.set_option("ragged_vjp_precision", "fp32")
+    du0_b, dp_b, _, _, _, _ = h.ragged_backward(cots)
+    assert np.array_equal(dp_b, dp_32) and np.array_equal(du0_b, du0_32)
