@@ -735,6 +735,20 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
   }
   float acc = 0.f, fl = 0.f;
 
+  // One row buffer for the weight tiles, refilled tile by tile: as soon as a
+  // tile's mu vector has been used, its register is loaded with the tile
+  // RG_HT further on (the next row, across layer ends; clamped at the last
+  // tile), so a row of loads stays in flight with no copies or branches (the
+  // pass runs near its register limit, and a load under a branch is waited
+  // for where it joins).  The first row is requested here, before the bias
+  // part, whose own loads it then overlaps.
+  const gp<const f32x4> wsrc = (gp<const f32x4>)mu + lane;
+  const gp<f32x4> wdst = (gp<f32x4>)mu_new + lane;
+  const int tlast = mm.ntile - 1;
+  f32x4 buf[RG_HT];
+#pragma unroll
+  for (int k = 0; k < RG_HT; ++k) buf[k] = wsrc[(k < tlast ? k : tlast) * 64];
+
   // ---- biases: lanes over the outputs o of each layer (at most two 64-wide
   // rounds: hidden widths <= 128, D <= 32).  Every load is unconditional (an
   // index out of range is clamped to a valid entry and its value dropped):
@@ -783,18 +797,8 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
       }
     }
 
-  // ---- weight tiles: the rows of RG_HT tiles in memory order, all layers.
-  // One row buffer, refilled tile by tile: as soon as a tile's mu vector has
-  // been used, its register is loaded with the tile RG_HT further on (the next
-  // row, across layer ends; clamped at the last tile), so a row of loads
-  // stays in flight with no copies or branches (the pass runs near its
-  // register limit, and a load under a branch is waited for where it joins).
-  const gp<const f32x4> wsrc = (gp<const f32x4>)mu + lane;
-  const gp<f32x4> wdst = (gp<f32x4>)mu_new + lane;
-  const int tlast = mm.ntile - 1;
-  f32x4 buf[RG_HT];
-#pragma unroll
-  for (int k = 0; k < RG_HT; ++k) buf[k] = wsrc[k * 64];
+  // ---- weight tiles: the rows of RG_HT tiles in memory order, all layers
+  // (the buffer was filled with the first row before the bias part).
   for (int l = 0; l < mm.L; ++l) {
     const int in = mm.in[l], out = mm.out[l];
     const bool ro = mm.rowout[l];
