@@ -195,3 +195,45 @@ def test_ragged_adjoint_bf16_vjp_option():
     h.set_option("ragged_vjp_precision", "fp32")
     du0_b, dp_b, _, _, _, _ = h.ragged_backward(cots)
     assert np.array_equal(dp_b, dp_32) and np.array_equal(du0_b, du0_32)
+
+
+def test_ragged_many_trajectories_take_queue_positions():
+    """More trajectories than the launch's slots (16 per wave, 4 waves per CU:
+    16,384 on 256 CUs): the rest start from the queue as slots free up.  10
+    distinct trajectories repeated 2,000 times: every copy's saved states,
+    dL/du0 and own dL/dp equal its first copy's bit for bit (whichever slot
+    or queue position ran it), and the 10 against the FP64 oracle."""
+    from diffeqflux_hip import Handle
+    widths, acts, tol, k, reps = [2, 16, 2], ["tanh", "identity"], 1e-6, 10, 2000
+    m, p32, u0s, t1s, saves_s = ragged_case(widths, acts, 0, k, seed=13, nsave=(2, 4))
+    n = k * reps
+    u0 = np.tile(u0s, (reps, 1))
+    t1 = np.tile(t1s, reps)
+    saves = [saves_s[i % k] for i in range(n)]
+    h = Handle(widths, acts, 0, "fp32", 1)
+    h.set_params(p32)
+    outs, ret, _, tot = h.ragged_forward(u0, t1, saves, abstol=tol, reltol=tol)
+    assert np.all(ret == 1)
+    rng = np.random.default_rng(17)
+    cots_s = [rng.standard_normal(o.shape).astype(np.float32) for o in outs[:k]]
+    cots = [cots_s[i % k] for i in range(n)]
+    du0, dp, rows, aret, _, atot = h.ragged_backward(cots, per_trajectory=True)
+    print(f"\n[ragged n={n}] forward {tot['naccept']} / adjoint {atot['naccept']} steps", flush=True)
+    assert np.all(aret == 1)
+    for i in range(k, n):
+        j = i % k
+        assert np.array_equal(outs[i], outs[j]), i
+    du0r, rowsr = du0.reshape(reps, k, -1), rows.reshape(reps, k, -1)
+    assert np.array_equal(du0r, np.broadcast_to(du0r[:1], du0r.shape))
+    assert np.array_equal(rowsr, np.broadcast_to(rowsr[:1], rowsr.shape))
+    p = p32.astype(np.float64)
+    for i in range(k):
+        s = list(saves[i])
+        ref = OA.forward(m, p, u0[i:i + 1].astype(np.float64), 0.0, float(t1[i]), saveat=s,
+                         abstol=tol, reltol=tol)
+        R = np.array(ref.u)[:, 0, :]
+        assert np.all(np.abs(outs[i] - R) <= 20 * (tol + tol * np.abs(R))), i
+        c = [cots[i][q:q + 1].astype(np.float64) for q in range(len(s))]
+        gu0, gp, _ = OA.backward(m, p, ref, c, abstol=tol, reltol=tol)
+        assert np.abs(du0[i] - gu0[0]).max() <= 50 * tol * max(np.abs(gu0).max(), 1e-3), i
+        assert np.abs(rows[i] - gp).max() <= 50 * tol * max(np.abs(gp).max(), 1e-3), i
