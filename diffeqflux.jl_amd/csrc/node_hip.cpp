@@ -1658,7 +1658,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
       grow(h, &b.dLdu0, n * D, b.dLdu0_cap, n * D) ||
       grow(h, &b.rows, n * Pn, b.rows_cap, n * Pn) ||
       grow(h, &b.tmp, slots * Pn, b.tmp_cap, slots * Pn) ||
-      grow(h, &b.sbuf, slots * 8 * m.sv, b.sbuf_cap, slots * 8 * m.sv) ||
+      grow(h, &b.sbuf, (slots + 1) * 8 * m.sv, b.sbuf_cap, (slots + 1) * 8 * m.sv) ||
       grow(h, &b.dp, P, b.dp_cap, P))
     return NODE_ERR_HIP;
   if (ns > 0)
@@ -1676,7 +1676,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   a.last_rec = r.last; a.rec_t = r.rec_t; a.rec_prev = r.rec_prev; a.rec_data = r.rec_data;
   a.dLdu0 = b.dLdu0; a.dLdp_traj = b.rows; a.mu_tmp = b.tmp; a.sbuf = b.sbuf;
   a.queue = r.ctr; a.retcode = r.ret; a.traj_stats = r.tstats;
-  a.order = b.order; a.norder = norder; a.nslots = nslots; a.spw = spw;
+  a.order = b.order; a.norder = norder; a.nslots = nslots; a.spw = spw; a.nslots_all = slots;
   a.vjp_bf16 = h->rg_vjp_prec == NODE_PREC_BF16;
   const node_solve_opts* o = &r.opts;
   a.t0 = o->t0;

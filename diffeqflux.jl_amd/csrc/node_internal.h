@@ -374,7 +374,8 @@ struct RaggedAdjArgs {
   float* dLdu0;                       // [n][D]
   float* dLdp_traj;                   // [n][Pn] per-trajectory parameter gradients (zeroed)
   float* mu_tmp;                      // [slots][Pn]
-  float* sbuf;                        // [slots][8][sv] stage blocks
+  float* sbuf;                        // [slots + 1][8][sv] stage blocks (the last: sink of idle columns)
+  int64_t nslots_all;                 // slots of the launch (16 per wave)
   int* queue;
   // start order: the trajectory at each queue position (-1: none).  A wave
   // runs trajectories in its first spw slots only (spw <= 16: more waves, each

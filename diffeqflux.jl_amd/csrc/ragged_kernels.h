@@ -938,7 +938,11 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
   rg_load_weights(m, rg_lds);
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const int64_t wslot0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RG_SLOTS;
-  const gp<float> sb = glb(a.sbuf) + (wslot0 + r) * 8 * m.sv;
+  // stage blocks of the slot's VJPs; the columns past spw never hold a
+  // trajectory, and their VJP writes (garbage that stays in its own MFMA
+  // column) go to one sink block shared by all of them, which stays in L2
+  const gp<float> sb = r < a.spw ? glb(a.sbuf) + (wslot0 + r) * 8 * m.sv
+                                 : glb(a.sbuf) + a.nslots_all * 8 * m.sv;
   const int D = a.D;
   const int64_t P = m.P;
   const double nel = (double)D + (double)P;
