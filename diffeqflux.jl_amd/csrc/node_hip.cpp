@@ -210,6 +210,7 @@ struct node_handle {
     float* sbuf = nullptr; int64_t sbuf_cap = 0;
     double* dp = nullptr; int64_t dp_cap = 0;
     int* order = nullptr; int64_t order_cap = 0;
+    double* part = nullptr; int64_t part_cap = 0;
   } ra;
 
   // profiling (node_prof_enable)
@@ -1659,7 +1660,8 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
       grow(h, &b.rows, n * Pn, b.rows_cap, n * Pn) ||
       grow(h, &b.tmp, slots * Pn, b.tmp_cap, slots * Pn) ||
       grow(h, &b.sbuf, (slots + 1) * 8 * m.sv, b.sbuf_cap, (slots + 1) * 8 * m.sv) ||
-      grow(h, &b.dp, P, b.dp_cap, P))
+      grow(h, &b.dp, P, b.dp_cap, P) ||
+      grow(h, &b.part, RG_SUM_CHUNKS * P, b.part_cap, RG_SUM_CHUNKS * P))
     return NODE_ERR_HIP;
   if (ns > 0)
     HIPCHK(h, hipMemcpyAsync(b.dLdout, dLdout, ns * D * sizeof(float), hipMemcpyHostToDevice, h->stream));
@@ -1690,7 +1692,7 @@ int ragged_backward_impl(node_handle* h, const float* dLdout, float* dLdu0, doub
   a.stats = r.stats;
   launch_ragged_adj(a, grid, waves, h->stream);
   HIPCHK(h, hipGetLastError());
-  launch_ragged_sum(b.rows, n, P, Pn, b.dp, h->stream);
+  launch_ragged_sum(b.rows, n, P, Pn, b.part, b.dp, h->stream);
   HIPCHK(h, hipGetLastError());
   std::vector<int> hret(n);
   unsigned long long hs[4];

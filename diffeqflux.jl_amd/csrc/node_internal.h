@@ -304,6 +304,7 @@ constexpr int RG_MAXL = 4;
 constexpr int RG_HT = 8;        // hidden width <= 128 (16-feature tiles)
 constexpr int RG_MAXD = 32;     // state dimension <= 32
 constexpr int RG_SLOTS = 16;    // trajectory slots per wave (MFMA columns)
+constexpr int RG_SUM_CHUNKS = 64;  // trajectory ranges of the dL/dp sum
 struct RaggedModel {
   int L, map;
   int in[RG_MAXL], out[RG_MAXL], act[RG_MAXL];
@@ -394,7 +395,8 @@ struct RaggedAdjArgs {
   unsigned long long* stats;
 };
 void launch_ragged_adj(const RaggedAdjArgs& a, int grid, int waves, hipStream_t s);
-void launch_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* out,
-                       hipStream_t s);
+// part: [RG_SUM_CHUNKS][P] FP64 scratch (the per-range sums)
+void launch_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* part,
+                       double* out, hipStream_t s);
 
 }  // namespace nodehip

@@ -1285,17 +1285,31 @@ void launch_ragged_adj(const RaggedAdjArgs& a, int grid, int waves, hipStream_t 
 }
 
 // dL/dp = sum over trajectories of their FP32 rows (first P of every `stride`
-// floats), in trajectory order, in FP64
-__global__ void k_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* out) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int64_t k = 0; k < n; ++k) s += (double)rows[k * stride + i];
-    out[i] = s;
-  }
+// floats) in FP64, in a fixed order: RG_SUM_CHUNKS consecutive ranges of
+// trajectories, each summed in trajectory order (blockIdx.y: the range), then
+// the ranges' sums in range order.  One thread per parameter and range keeps
+// thousands of blocks streaming the rows (a thread per parameter over all
+// trajectories left most CUs idle and every thread one long dependent chain).
+__global__ void k_ragged_sum_part(const float* rows, int64_t n, int64_t P, int64_t stride,
+                                  double* part) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const int64_t c = blockIdx.y, k0 = n * c / RG_SUM_CHUNKS, k1 = n * (c + 1) / RG_SUM_CHUNKS;
+  double s = 0.0;
+  for (int64_t k = k0; k < k1; ++k) s += (double)rows[k * stride + i];
+  part[c * P + i] = s;
 }
-void launch_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* out,
-                       hipStream_t s) {
-  const int64_t nb = std::min<int64_t>((P + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_ragged_sum, dim3((int)nb), dim3(256), 0, s, rows, n, P, stride, out);
+__global__ void k_ragged_sum_fold(const double* part, int64_t P, double* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  double s = 0.0;
+  for (int c = 0; c < RG_SUM_CHUNKS; ++c) s += part[c * P + i];
+  out[i] = s;
+}
+void launch_ragged_sum(const float* rows, int64_t n, int64_t P, int64_t stride, double* part,
+                       double* out, hipStream_t s) {
+  const int64_t nb = (P + 255) / 256;
+  hipLaunchKernelGGL(k_ragged_sum_part, dim3((unsigned)nb, RG_SUM_CHUNKS), dim3(256), 0, s, rows, n,
+                     P, stride, part);
+  hipLaunchKernelGGL(k_ragged_sum_fold, dim3((unsigned)nb), dim3(256), 0, s, part, P, out);
 }

@@ -260,7 +260,8 @@ int32_t node_ragged_forward(node_handle_t* h, const node_solve_opts* opts, int64
  * D + P entries, steps landing on its save times with the cotangent jumps),
  * in one persistent launch.  dLdout: D x save_offsets[n] cotangents of the
  * saved states.  Writes dL/du0 (D x n), dL/dp summed over trajectories (P,
- * FP64, summed in trajectory order) and optionally each trajectory's own
+ * FP64, summed in a fixed order: 64 consecutive ranges of trajectories each in
+ * trajectory order, then the ranges in order) and optionally each trajectory's own
  * dL/dp (P x n, FP32, the adjoint state mu of that trajectory).  All forward
  * retcodes must be Success.  Host pointers.
  * Replaces: per trajectory, the pullback of src/neural_de.jl:55-58 with
