@@ -201,20 +201,29 @@ class Handle:
         return dx, dp, st.as_dict()
 
     def ragged_forward(self, u0, t1, save_times, t0=0.0, abstol=1e-6, reltol=1e-3, adaptive=True,
-                       dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=0):
+                       dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=0, flat_out=False):
         """Per-trajectory solves.  u0: (n, D); t1: (n,); save_times: list of n
-        increasing sequences.  Returns (list of (len_i, D) saved states,
+        increasing sequences, or a pair (times, offsets) of the concatenated
+        save times and the n + 1 offsets of each trajectory's run (the C
+        entry point's own layout).  Returns (list of (len_i, D) saved states
+        -- or, with flat_out, the (offsets[n], D) array they are views of --,
         retcodes (n,), per-trajectory stats (n, 4) [nf, naccept, nreject,
         records], totals dict)."""
         self.rg_gen += 1
         u0 = _f32(u0).reshape(-1, self.D)
         n = u0.shape[0]
         t1 = np.ascontiguousarray(np.asarray(t1, dtype=np.float64).reshape(n))
-        lens = [len(s) for s in save_times]
-        off = np.zeros(n + 1, np.int64)
-        off[1:] = np.cumsum(lens)
-        st = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float64) for s in save_times])
-                                  if off[-1] else np.zeros(1))
+        if isinstance(save_times, tuple):
+            st, off = save_times
+            off = np.ascontiguousarray(np.asarray(off, np.int64).reshape(n + 1))
+            st = np.ascontiguousarray(np.asarray(st, np.float64).reshape(-1)
+                                      if off[-1] else np.zeros(1))
+        else:
+            lens = [len(s) for s in save_times]
+            off = np.zeros(n + 1, np.int64)
+            off[1:] = np.cumsum(lens)
+            st = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float64) for s in save_times])
+                                      if off[-1] else np.zeros(1))
         out = np.zeros((max(int(off[-1]), 1), self.D), np.float32)
         ret = np.zeros(n, np.int32)
         ts = np.zeros((n, 4), np.int32)
@@ -226,6 +235,8 @@ class Handle:
                                                  _ptr(ts), ctypes.byref(rs)),
                     "node_ragged_forward")
         self.ragged_offsets = off
+        if flat_out:
+            return out[:off[-1]], ret, ts, rs.as_dict()
         return [out[off[i]:off[i + 1]] for i in range(n)], ret, ts, rs.as_dict()
 
     def ragged_backward(self, dLdout, per_trajectory=False):

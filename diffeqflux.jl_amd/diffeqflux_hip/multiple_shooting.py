@@ -169,17 +169,17 @@ def multiple_shoot(p, ode_data, tsteps, prob, loss_function, *args, continuity_t
 
 def _segments(data, t, ranges):
     """Per (group, trajectory) solve set-up, group-major: initial states,
-    shifted end times and shifted save times."""
+    shifted end times and the shifted save times as (concatenated times,
+    offsets), the per-trajectory entry point's layout."""
     ntraj = data.shape[0]
-    u0, t1, saves = [], [], []
-    for rg in ranges:
-        ta = t[rg.start]
-        ts = t[rg.start:rg.stop] - ta
-        for j in range(ntraj):
-            u0.append(data[j, rg.start])
-            t1.append(ts[-1])
-            saves.append(ts)
-    return np.asarray(u0, np.float32), np.asarray(t1, np.float64), saves
+    u0 = np.concatenate([data[:, rg.start] for rg in ranges]).astype(np.float32)
+    ts = [t[rg.start:rg.stop] - t[rg.start] for rg in ranges]
+    t1 = np.concatenate([np.full(ntraj, g[-1]) for g in ts])
+    times = np.concatenate([np.tile(g, ntraj) for g in ts])
+    lens = np.concatenate([np.full(ntraj, len(g), np.int64) for g in ts])
+    off = np.zeros(lens.size + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    return u0, t1, (times, off)
 
 
 def _solve_groups(node, p, data, t, ranges, kw):
@@ -224,8 +224,8 @@ def _solve_ragged(node, h, p, data, t, ranges, kw):
         flat, ok = _RaggedFn.apply(p, h, u0, t1, saves, rk)
         return _split(flat, data, ranges), ok
     h.set_params(np.asarray(p, np.float32))
-    outs, ret, _, _ = h.ragged_forward(u0, t1, saves, **rk)
-    return _split(np.concatenate(outs), data, ranges), bool(np.all(ret == _SUCCESS))
+    flat, ret, _, _ = h.ragged_forward(u0, t1, saves, flat_out=True, **rk)
+    return _split(flat, data, ranges), bool(np.all(ret == _SUCCESS))
 
 
 def _ragged_fn_factory():
@@ -238,10 +238,10 @@ def _ragged_fn_factory():
         @staticmethod
         def forward(ctx, p, h, u0, t1, saves, rk):
             h.set_params(p.detach().to(torch.float32).cpu().numpy())
-            outs, ret, _, _ = h.ragged_forward(u0, t1, saves, **rk)
+            out, ret, _, _ = h.ragged_forward(u0, t1, saves, flat_out=True, **rk)
             ctx.h, ctx.pdev, ctx.pdtype, ctx.gen = h, p.device, p.dtype, h.rg_gen
             ctx.ok = bool(np.all(ret == _SUCCESS))
-            flat = torch.from_numpy(np.ascontiguousarray(np.concatenate(outs))).to(p.device)
+            flat = torch.from_numpy(out).to(p.device)
             return flat, ctx.ok
 
         @staticmethod

@@ -47,14 +47,17 @@ def test_segments_shift_each_group_to_zero():
     data = rng.standard_normal((3, 11, 2)).astype(np.float32)   # (ntraj, T, D)
     t = np.linspace(0.5, 3.0, 11)
     rs = group_ranges(11, 4)
-    u0, t1, saves = MS._segments(data, t, rs)
-    assert u0.shape == (3 * len(rs), 2)
+    u0, t1, (times, off) = MS._segments(data, t, rs)
+    assert u0.shape == (3 * len(rs), 2) and u0.dtype == np.float32
+    assert off.shape == (3 * len(rs) + 1,) and off[0] == 0 and off[-1] == times.size
     for gi, rg in enumerate(rs):
         for j in range(3):
             k = gi * 3 + j                                       # group-major
+            sk = times[off[k]:off[k + 1]]
             assert np.array_equal(u0[k], data[j, rg.start])
-            assert saves[k][0] == 0.0 and saves[k][-1] == t1[k]
-            assert np.allclose(saves[k] + t[rg.start], t[rg.start:rg.stop], rtol=0, atol=1e-15)
+            assert len(sk) == len(rg)
+            assert sk[0] == 0.0 and sk[-1] == t1[k]
+            assert np.allclose(sk + t[rg.start], t[rg.start:rg.stop], rtol=0, atol=1e-15)
 
 
 def test_split_restores_group_shapes():
