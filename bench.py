@@ -676,6 +676,29 @@ def spiral_leg(reps=20):
                            "ms_per_loss_grad": round(1e3 * cs, 2), "cores": 1, "kind": "port",
                            "sample": f"oracle numpy FP32, the whole problem, mean of {n} runs "
                                      "after one warm-up"}
+    # the same problem through the per-trajectory entry points (n = 1: one
+    # persistent launch per direction instead of a launch sequence per step)
+    from diffeqflux_hip import Handle, _lib
+    hr = Handle(widths, acts, _lib.MAP_CUBE, "fp32", 1)
+    hr.set_params(p)
+    off = np.array([0, ts.size], np.int64)
+
+    def rg_step():
+        flat, _, _, tot = hr.ragged_forward(u0, [1.5], (ts, off), flat_out=True, **tol)
+        cot = ((2.0 / flat.size) * (flat - tgt.reshape(flat.shape))).astype(np.float32)
+        _, dpr, _, _, _, atot = hr.ragged_backward(cot)
+        return dpr, tot, atot
+    for _ in range(2):
+        rg_step()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dpr, tot, atot = rg_step()
+    sr = (time.perf_counter() - t0) / reps
+    res["per_trajectory"] = {
+        "ms_per_loss_grad": round(1e3 * sr, 3),
+        "steps_fwd_adj": f"{tot['naccept']}+{tot['nreject']}/{atot['naccept']}+{atot['nreject']}",
+        "grad_rel_err": float(f"{np.linalg.norm(dpr - gp64) / np.linalg.norm(gp64):.3e}"),
+        "timing": "node_ragged_forward / node_ragged_backward with n = 1, host pointers"}
     return res
 
 
