@@ -237,3 +237,26 @@ def test_ragged_many_trajectories_take_queue_positions():
         gu0, gp, _ = OA.backward(m, p, ref, c, abstol=tol, reltol=tol)
         assert np.abs(du0[i] - gu0[0]).max() <= 50 * tol * max(np.abs(gu0).max(), 1e-3), i
         assert np.abs(rows[i] - gp).max() <= 50 * tol * max(np.abs(gp).max(), 1e-3), i
+
+
+def test_ragged_flat_layout_forms_match_lists():
+    """save_times as (times, offsets) and flat_out=True give exactly the
+    per-trajectory list forms' results (the entry point's own layout)."""
+    from diffeqflux_hip import Handle
+    widths, acts, tol, n = [2, 50, 2], ["tanh", "identity"], 1e-6, 12
+    m, p32, u0, t1, saves = ragged_case(widths, acts, MAP_CUBE, n, seed=21)
+    h = Handle(widths, acts, MAP_CUBE, "fp32", 1)
+    h.set_params(p32)
+    outs, ret, ts, _ = h.ragged_forward(u0, t1, saves, abstol=tol, reltol=tol)
+    cots = [np.full(o.shape, 0.5, np.float32) for o in outs]
+    du0, dp, _, _, _, _ = h.ragged_backward(cots)
+    lens = np.array([len(s) for s in saves], np.int64)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    flat, ret2, ts2, _ = h.ragged_forward(u0, t1, (np.concatenate(saves), off), flat_out=True,
+                                          abstol=tol, reltol=tol)
+    assert flat.shape == (off[-1], 2)
+    assert np.array_equal(flat, np.concatenate(outs))
+    assert np.array_equal(ret, ret2) and np.array_equal(ts, ts2)
+    du0f, dpf, _, _, _, _ = h.ragged_backward(np.full(flat.shape, 0.5, np.float32))
+    assert np.array_equal(du0, du0f) and np.array_equal(dp, dpf)
