@@ -19,8 +19,10 @@
 // 2..7 of an attempt); every round evaluates the right-hand side once for all
 // 16 slots, whatever phase each is in.  A slot whose trajectory has finished
 // takes the next one from a global counter, so long and short trajectories
-// share the machine until the queue is empty.  Every trajectory's arithmetic
-// depends only on its own column, so results do not depend on scheduling.
+// share the machine until the queue is empty (the adjoint starts each slot
+// from a host-computed, cost-balanced order and may run fewer than 16 slots
+// per wave: RaggedAdjArgs).  Every trajectory's arithmetic depends only on
+// its own column, so results do not depend on scheduling.
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
