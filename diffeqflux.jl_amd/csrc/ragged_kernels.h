@@ -544,6 +544,39 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
 #pragma unroll
       for (int t = 0; t < RG_HT; ++t) hb[t] = rg_bf4(hin[t]);
     }
+    // a hidden tanh layer of 8 x 8 tiles with act' parked in LDS (the usual
+    // shape): one straight-line block, so the compiler can run a tile's
+    // activation and stores under the next tile's reads and MFMAs
+    if (nti == RG_HT && nto == RG_HT && act == 1 && m.act_stride && l < L - 1) {
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        f32x4 acc = *(const f32x4*)(bias + 16 * ot + g4);
+        const float* wrow = W + (16 * ot + r) * pitch + g4;
+        f32x4 a[RG_HT];
+#pragma unroll
+        for (int it = 0; it < RG_HT; ++it) a[it] = *(const f32x4*)(wrow + 16 * it);
+#pragma unroll
+        for (int it = 0; it < RG_HT; ++it) {
+          if (BF) {
+            acc = rg_mfma_bf(rg_bf4(a[it]), hb[it], acc);
+          } else {
+            acc = rg_mfma(a[it].x, hin[it].x, acc);
+            acc = rg_mfma(a[it].y, hin[it].y, acc);
+            acc = rg_mfma(a[it].z, hin[it].z, acc);
+            acc = rg_mfma(a[it].w, hin[it].w, acc);
+          }
+        }
+        f32x4 y, sd;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          y[v] = act_f(1, acc[v]);
+          sd[v] = act_d(1, acc[v], y[v]);
+        }
+        *(f32x4*)(act_lds + (m.act_off[l] + ot) * 256 + 4 * lane) = sd;
+        *(gp<f32x4>)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
+        hout[ot] = y;
+      }
+    } else {
 #pragma unroll
     for (int ot = 0; ot < RG_HT; ++ot) {
       hout[ot] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -577,6 +610,7 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
         hout[ot] = y;
       }
     }
+    }
 #pragma unroll
     for (int t = 0; t < RG_HT; ++t) hin[t] = hout[t];
   }
@@ -587,6 +621,43 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
 #pragma unroll 1
   for (int l = L - 1; l >= 0; --l) {
     const int nti = m.ti[l], nto = m.to[l], pitch = m.pitch[l];
+    const float* W = lds + m.lw[l];
+    if (nti == RG_HT && nto == RG_HT && m.act_stride) {
+      // 8 x 8 tiles, act' in LDS: the same straight-line form as the forward
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        const f32x4 ad = *(const f32x4*)(act_lds + (m.act_off[l] + ot) * 256 + 4 * lane);
+        const f32x4 d = gi[ot] * ad;
+        *(gp<f32x4>)(blk + m.sd[l] + 16 * ot + g4) = d;
+        gi[ot] = d;
+      }
+#pragma unroll
+      for (int it = 0; it < RG_HT; ++it) go[it] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        const float* wc = W + (16 * ot + g4) * pitch + r;
+        float w[RG_HT][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int it = 0; it < RG_HT; ++it) w[it][v] = wc[v * pitch + 16 * it];
+        const rg_s4 gb = rg_bf4(gi[ot]);
+#pragma unroll
+        for (int it = 0; it < RG_HT; ++it) {
+          if (BF) {
+            go[it] = rg_mfma_bf(rg_bf4(f32x4{w[it][0], w[it][1], w[it][2], w[it][3]}), gb, go[it]);
+          } else {
+            go[it] = rg_mfma(w[it][0], gi[ot].x, go[it]);
+            go[it] = rg_mfma(w[it][1], gi[ot].y, go[it]);
+            go[it] = rg_mfma(w[it][2], gi[ot].z, go[it]);
+            go[it] = rg_mfma(w[it][3], gi[ot].w, go[it]);
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < RG_HT; ++t) gi[t] = go[t];
+      continue;
+    }
 #pragma unroll
     for (int ot = 0; ot < RG_HT; ++ot) {
       if (ot < nto) {
@@ -598,7 +669,6 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
         gi[ot] = d;
       }
     }
-    const float* W = lds + m.lw[l];
     // output tile by output tile, every input tile's sum at once: the four
     // W^T row bases of an output tile serve all input tiles at constant
     // offsets (k-tiles past nti read the next rows or the zero pad, unused)
