@@ -576,6 +576,30 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
         *(gp<f32x4>)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
         hout[ot] = y;
       }
+    } else if (nti == 1 && nto == RG_HT && act == 1 && m.act_stride && l < L - 1) {
+      // the same for a first layer of 1 x 8 tiles (inputs <= 16)
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        f32x4 acc = *(const f32x4*)(bias + 16 * ot + g4);
+        const f32x4 a = *(const f32x4*)(W + (16 * ot + r) * pitch + g4);
+        if (BF) {
+          acc = rg_mfma_bf(rg_bf4(a), hb[0], acc);
+        } else {
+          acc = rg_mfma(a.x, hin[0].x, acc);
+          acc = rg_mfma(a.y, hin[0].y, acc);
+          acc = rg_mfma(a.z, hin[0].z, acc);
+          acc = rg_mfma(a.w, hin[0].w, acc);
+        }
+        f32x4 y, sd;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          y[v] = act_f(1, acc[v]);
+          sd[v] = act_d(1, acc[v], y[v]);
+        }
+        *(f32x4*)(act_lds + (m.act_off[l] + ot) * 256 + 4 * lane) = sd;
+        *(gp<f32x4>)(blk + m.sh[l + 1] + 16 * ot + g4) = y;
+        hout[ot] = y;
+      }
     } else {
 #pragma unroll
     for (int ot = 0; ot < RG_HT; ++ot) {
@@ -651,6 +675,60 @@ __device__ __forceinline__ void rg_vjp(const RaggedModel& m, const float* lds, f
             go[it] = rg_mfma(w[it][1], gi[ot].y, go[it]);
             go[it] = rg_mfma(w[it][2], gi[ot].z, go[it]);
             go[it] = rg_mfma(w[it][3], gi[ot].w, go[it]);
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < RG_HT; ++t) gi[t] = go[t];
+      continue;
+    }
+    if (m.act_stride && ((nti == RG_HT && nto == 1) || (nti == 1 && nto == RG_HT))) {
+      // the last layer's 8 x 1 and the first layer's 1 x 8 tiles, likewise:
+      // NTO output tiles by NTI input tiles, one of them 1
+      const bool last = nto == 1;
+#pragma unroll
+      for (int ot = 0; ot < RG_HT; ++ot) {
+        if (ot > 0 && last) break;
+        const f32x4 d = gi[ot] * *(const f32x4*)(act_lds + (m.act_off[l] + ot) * 256 + 4 * lane);
+        *(gp<f32x4>)(blk + m.sd[l] + 16 * ot + g4) = d;
+        gi[ot] = d;
+      }
+#pragma unroll
+      for (int it = 0; it < RG_HT; ++it) go[it] = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (last) {
+        const float* wc = W + g4 * pitch + r;
+        float w[RG_HT][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int it = 0; it < RG_HT; ++it) w[it][v] = wc[v * pitch + 16 * it];
+        const rg_s4 gb = rg_bf4(gi[0]);
+#pragma unroll
+        for (int it = 0; it < RG_HT; ++it) {
+          if (BF) {
+            go[it] = rg_mfma_bf(rg_bf4(f32x4{w[it][0], w[it][1], w[it][2], w[it][3]}), gb, go[it]);
+          } else {
+            go[it] = rg_mfma(w[it][0], gi[0].x, go[it]);
+            go[it] = rg_mfma(w[it][1], gi[0].y, go[it]);
+            go[it] = rg_mfma(w[it][2], gi[0].z, go[it]);
+            go[it] = rg_mfma(w[it][3], gi[0].w, go[it]);
+          }
+        }
+      } else {
+        float w[RG_HT][4];
+#pragma unroll
+        for (int ot = 0; ot < RG_HT; ++ot)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) w[ot][v] = W[(16 * ot + g4 + v) * pitch + r];
+#pragma unroll
+        for (int ot = 0; ot < RG_HT; ++ot) {
+          if (BF) {
+            go[0] = rg_mfma_bf(rg_bf4(f32x4{w[ot][0], w[ot][1], w[ot][2], w[ot][3]}), rg_bf4(gi[ot]), go[0]);
+          } else {
+            go[0] = rg_mfma(w[ot][0], gi[ot].x, go[0]);
+            go[0] = rg_mfma(w[ot][1], gi[ot].y, go[0]);
+            go[0] = rg_mfma(w[ot][2], gi[ot].z, go[0]);
+            go[0] = rg_mfma(w[ot][3], gi[ot].w, go[0]);
           }
         }
       }
