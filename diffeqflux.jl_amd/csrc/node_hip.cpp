@@ -1439,12 +1439,17 @@ int ragged_model(node_handle* h, RaggedModel& m, std::string& why) {
   m.sv = so;
   m.P = h->P;
   m.p = h->pflat;
+  // rows of rw tiles: the longest tile dimension of any layer, rounded up to
+  // a power of two (small models stream no 8-tile padding)
+  int tmax = 1;
+  for (int l = 0; l < m.L; ++l) tmax = std::max(tmax, std::max(m.ti[l], m.to[l]));
+  m.rw = tmax <= 1 ? 1 : tmax <= 2 ? 2 : tmax <= 4 ? 4 : RG_HT;
   int64_t no = 0;
   for (int l = 0; l < m.L; ++l) {
     m.rowout[l] = m.to[l] >= m.ti[l];
     m.nrow[l] = m.rowout[l] ? m.ti[l] : m.to[l];
     m.nw0[l] = no;
-    no += (int64_t)m.nrow[l] * RG_HT * 256;
+    no += (int64_t)m.nrow[l] * m.rw * 256;
   }
   m.ntile = (int)(no / 256);
   for (int l = 0; l < m.L; ++l) {

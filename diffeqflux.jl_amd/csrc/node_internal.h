@@ -318,16 +318,17 @@ struct RaggedModel {
   int64_t P;
   const float* p;                     // flat FP32 parameters (device)
   // parameter-gradient rows while a trajectory integrates: each layer's
-  // weights as rows of RG_HT 16x16 tiles (a tile: lane, then 4 values, the
+  // weights as rows of rw 16x16 tiles (a tile: lane, then 4 values, the
   // MFMA accumulator order, one 16-byte vector per lane), then each layer's
   // bias; Pn floats per row (>= P, a multiple of 64).  A layer's tile rows run
   // along its longer tile dimension: rowout[l] = 1: row it holds tiles
-  // (it, ot = 0..RG_HT-1), else row ot holds tiles (it = 0..RG_HT-1, ot);
+  // (it, ot = 0..rw-1), else row ot holds tiles (it = 0..rw-1, ot);
   // nrow[l] rows, tiles past the layer's shorter dimension are padding (0).
   // A finished trajectory's row is rewritten in the flat parameter order
   // (first P floats).
   int64_t nw0[RG_MAXL], nb0[RG_MAXL];
   int nrow[RG_MAXL], rowout[RG_MAXL];
+  int rw;                             // tiles per row: the longest tile dimension, rounded up to 1, 2, 4 or 8
   int ntile;                          // weight tiles of a row, all layers
   int64_t Pn;
   // adjoint VJP: act' of the forward sweep parked in LDS after the weights,

@@ -852,7 +852,7 @@ __device__ __forceinline__ CRaggedModel& rg_uniform(uint64_t kargs) {
   return *(CRaggedModel*)(((uint64_t)hi << 32) | lo);
 }
 
-template <int MODE>
+template <int MODE, int RW>
 __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const float> sbs, int p0, double dt,
                              double atol, double rtol, gp<const float> __restrict__ mu,
                              gp<float> __restrict__ mu_new, double* flag, int lane) {
@@ -887,7 +887,7 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
 
   // One row buffer for the weight tiles, refilled tile by tile: as soon as a
   // tile's mu vector has been used, its register is loaded with the tile
-  // RG_HT further on (the next row, across layer ends; clamped at the last
+  // RW further on (the next row, across layer ends; clamped at the last
   // tile), so a row of loads stays in flight with no copies or branches (the
   // pass runs near its register limit, and a load under a branch is waited
   // for where it joins).  The first row is requested here, before the bias
@@ -895,9 +895,9 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
   const gp<const f32x4> wsrc = (gp<const f32x4>)mu + lane;
   const gp<f32x4> wdst = (gp<f32x4>)mu_new + lane;
   const int tlast = mm.ntile - 1;
-  f32x4 buf[RG_HT];
+  f32x4 buf[RW];
 #pragma unroll
-  for (int k = 0; k < RG_HT; ++k) buf[k] = wsrc[(k < tlast ? k : tlast) * 64];
+  for (int k = 0; k < RW; ++k) buf[k] = wsrc[(k < tlast ? k : tlast) * 64];
 
   // ---- biases: lanes over the outputs o of each layer (at most two 64-wide
   // rounds: hidden widths <= 128, D <= 32).  Every load is unconditional (an
@@ -947,7 +947,7 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
       }
     }
 
-  // ---- weight tiles: the rows of RG_HT tiles in memory order, all layers
+  // ---- weight tiles: the rows of RW tiles in memory order, all layers
   // (the buffer was filled with the first row before the bias part).
   for (int l = 0; l < mm.L; ++l) {
     const int in = mm.in[l], out = mm.out[l];
@@ -961,9 +961,9 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
     // index (cB, cE) of the update and error sums; rows along the inputs
     // (ro = 0) store their tiles transposed, so that both orders run the same
     // code: the update sum is MFMA(row operand, column operand).
-    float tB[RG_HT][2], tE[RG_HT][2], cB[RG_HT][2], cE[RG_HT][2];
+    float tB[RW][2], tE[RW][2], cB[RW][2], cE[RW][2];
 #pragma unroll
-    for (int t = 0; t < RG_HT; ++t) {
+    for (int t = 0; t < RW; ++t) {
       const int i = 16 * t + r;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
@@ -982,12 +982,12 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
     for (int a = 0; a < nr; ++a) {
       float rb0 = 0.f, rb1 = 0.f, re0 = 0.f, re1 = 0.f;
 #pragma unroll
-      for (int t = 0; t < RG_HT; ++t)
+      for (int t = 0; t < RW; ++t)
         if (t == a) { rb0 = tB[t][0]; rb1 = tB[t][1]; re0 = tE[t][0]; re1 = tE[t][1]; }
 #pragma unroll
-      for (int k = 0; k < RG_HT; ++k) {
+      for (int k = 0; k < RW; ++k) {
         // tile T: stage sums, entries, store; then the refill
-        const int T = T0 + a * RG_HT + k;
+        const int T = T0 + a * RW + k;
         f32x4 kb = {0.f, 0.f, 0.f, 0.f}, ke = kb, mn = kb;
         if (MODE != 2) {
           kb = rg_mfma(rb0, cB[k][0], kb);
@@ -1005,7 +1005,7 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
           acc = fmaf(ev, ev, acc);
         }
         if (MODE == 0) wdst[T * 64] = mn;
-        const int Tn = T + RG_HT;
+        const int Tn = T + RW;
         buf[k] = wsrc[(Tn < tlast ? Tn : tlast) * 64];
       }
     }
@@ -1021,6 +1021,25 @@ __device__ __attribute__((noinline)) double rg_mu_pass(uint64_t kargs, gp<const 
   return sum;
 }
 
+// The pass for the model's row width (1, 2, 4 or 8 tiles) and mode.
+__device__ __forceinline__ double rg_mu_call(uint64_t kargs, int rw, int mode, gp<const float> sbs,
+                                             int p0, double dt, double atol, double rtol,
+                                             gp<const float> mu, gp<float> mun, double* fl, int lane) {
+#define RG_MU_RW(R)                                                                          \
+  if (rw == R) {                                                                            \
+    if (mode == 0) return rg_mu_pass<0, R>(kargs, sbs, p0, dt, atol, rtol, mu, mun, nullptr, lane); \
+    if (mode == 1) return rg_mu_pass<1, R>(kargs, sbs, p0, dt, atol, rtol, mu, nullptr, nullptr, lane); \
+    return rg_mu_pass<2, R>(kargs, sbs, p0, dt, atol, rtol, mu, nullptr, fl, lane);                \
+  }
+  RG_MU_RW(1)
+  RG_MU_RW(2)
+  RG_MU_RW(4)
+#undef RG_MU_RW
+  if (mode == 0) return rg_mu_pass<0, RG_HT>(kargs, sbs, p0, dt, atol, rtol, mu, mun, nullptr, lane);
+  if (mode == 1) return rg_mu_pass<1, RG_HT>(kargs, sbs, p0, dt, atol, rtol, mu, nullptr, nullptr, lane);
+  return rg_mu_pass<2, RG_HT>(kargs, sbs, p0, dt, atol, rtol, mu, nullptr, fl, lane);
+}
+
 // A finished trajectory's row from the tile-row order to the flat
 // parameter order (its first P floats); `tmp` holds the tile-major row.
 __device__ void rg_row_to_flat(const RaggedModel& m, gp<const float> __restrict__ tmp,
@@ -1029,8 +1048,8 @@ __device__ void rg_row_to_flat(const RaggedModel& m, gp<const float> __restrict_
   for (int l = 0; l < m.L; ++l) {
     const int in = m.in[l], out = m.out[l], nti = m.ti[l], nto = m.to[l];
     const gp<const f32x4> src = (gp<const f32x4>)(tmp + m.nw0[l]) + lane;
-    for (int t = 0; t < m.nrow[l] * RG_HT; ++t) {
-      const int a = t / RG_HT, k = t - a * RG_HT;
+    for (int t = 0; t < m.nrow[l] * m.rw; ++t) {
+      const int a = t / m.rw, k = t - a * m.rw;
       const int it = m.rowout[l] ? a : k, ot = m.rowout[l] ? k : a;
       if (it >= nti || ot >= nto) continue;   // padding tile
       const f32x4 v4 = src[t * 64];
@@ -1287,9 +1306,7 @@ __global__ void __launch_bounds__(256) k_ragged_adj(RaggedAdjArgs a) {
       const gp<const float> mu = mfs ? tmp : row;
       const gp<float> mun = mfs ? row : tmp;
       double fl = 0.0, sm;
-      if (ms == 0) sm = rg_mu_pass<0>(kargs, sbs, p0s, dts, a.abstol, a.reltol, mu, mun, nullptr, lane);
-      else if (ms == 1) sm = rg_mu_pass<1>(kargs, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, nullptr, lane);
-      else sm = rg_mu_pass<2>(kargs, sbs, p0s, dts, a.abstol, a.reltol, mu, nullptr, &fl, lane);
+      sm = rg_mu_call(kargs, m.rw, ms, sbs, p0s, dts, a.abstol, a.reltol, mu, mun, &fl, lane);
       if (r == s) { smu = sm; flmu = fl; }
     }
     // ---- consume, part 2: control
